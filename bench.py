@@ -1,0 +1,112 @@
+"""Benchmark: packets/s of the MCMax transport pass on one MI355X.
+
+A step is one Monte Carlo pass (MCRadiation.f:6) of `--nphot` packets over the
+production disk model (200 r x 100 theta cells, 200 wavelengths, dust mass
+1e-3 Msun) followed by the Lucy temperature inversion (Main.f:498-549), with all
+tables resident in HBM before the timed region.  Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "packets/s on 1 MI355X + time to converged T-field at ≤1% median |ΔT|/T"
+HBM_PEAK = 8.0e12
+
+
+def algorithmic_bytes(t, ng, nlam):
+    """Bytes the transport must move per launch, from on-device event counts:
+    per cell step rho, w(ng), EJv add, EJv2 add, Ni bit; per absorption the
+    Eabs add, Kp reads and two CDF searches; per scattering the Mueller pick
+    and the angle search; 8 B per escape."""
+    Bx = 8 + 8 * ng + 8 + 8 + 4
+    Ba = 8 + 24 * ng + 16 * ng * math.ceil(math.log2(nlam)) + 16
+    Bs = 8 * ng + 8 * math.ceil(math.log2(180))
+    return t["nsteps"] * Bx + t["nabs"] * Ba + t["nscat"] * Bs + 8 * t["nescape"], (Bx, Ba, Bs)
+
+
+def cpu_baseline(m, seconds=12.0):
+    """Oracle (FP64 C restatement of the reference loop) on one host core, on
+    a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    o = pyoracle.Oracle(m)
+    n = 500
+    t0 = time.perf_counter()
+    o.mcradiation(n, rng="philox", seed=99)
+    dt = time.perf_counter() - t0
+    n2 = max(1000, int(n * seconds / max(dt, 1e-3)))
+    t0 = time.perf_counter()
+    o.mcradiation(n2, rng="philox", seed=98)
+    dt = time.perf_counter() - t0
+    return {"value": n2 / dt, "unit": "packets/s", "cores": 1, "kind": "port",
+            "sample": f"{n2} packets of the same model on one host core ({dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nphot", type=int, default=10_000_000)
+    ap.add_argument("--config", default="production")
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: every run here uses a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+
+    from mcmax_amd import configs, transport
+    work = tempfile.mkdtemp(prefix="mcx_bench_")
+    m = configs.load_config(a.config, work)
+    print(f"[bench] model {a.config}: {m.summary()}", file=sys.stderr, flush=True)
+    tr = transport.Transport(0)
+    tr.set_model(m)
+    for w in range(a.warmup):
+        tr.transport(a.nphot, seed=a.seed + 1000 + w)
+        tr.temperature()
+        print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
+    kms = []
+    t0 = time.perf_counter()
+    for s in range(a.steps):
+        rc = tr.transport(a.nphot, seed=a.seed + s)
+        k, _ = tr.timing()
+        kms.append(k)
+        tr.temperature()
+        print(f"[bench] step {s}: kernel {k:.1f} ms rc={rc}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    tal = tr.tallies()
+    ms_step = 1e3 * wall / a.steps
+    value = a.nphot * a.steps / wall
+    kern_s = sum(kms) / len(kms) * 1e-3
+    nbytes, (Bx, Ba, Bs) = algorithmic_bytes(tal, m.ng, m.nlam)
+    achieved = nbytes / kern_s
+    out = {
+        "metric": METRIC, "value": value, "unit": "packets/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{a.config} disk model, one MC pass + T inversion per step",
+                   "nphot": a.nphot, "nr": m.nr, "nt": m.nt, "nlam": m.nlam, "ng": m.ng},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_transport", "kernel_ms": kern_s * 1e3,
+                     "bytes_per_event": {"step": Bx, "absorption": Ba, "scattering": Bs, "escape": 8},
+                     "events": {k: int(tal[k]) for k in ("nsteps", "nabs", "nscat", "nescape")}},
+        "energy_error": abs(tal["Eescape"] / m.Lstar - 1.0),
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(m)
+    tr.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,143 @@
+/*
+ * mcx.h -- C-ABI of the MI355X Monte Carlo photon-transport library
+ * (libmcx.so).  It replaces one MCMax stage: the Monte Carlo pass
+ *     subroutine MCRadiation(NphotTot,niter,dofastvisc)   MCRadiation.f:6
+ * together with the Lucy temperature inversion that follows it in the
+ * driver (Main.f:498-558).  The reference passes everything through module
+ * globals (Modules.f:141-150); here the caller hands the same quantities over
+ * once as plain host arrays (mcx_set_tables, mcx_set_model) and the library
+ * keeps them resident in HBM.
+ *
+ * Conventions
+ *  - every call returns 0 on success or a negative MCX_E* code; the text of
+ *    the last error is available from mcx_last_error(); nothing aborts;
+ *  - all arrays are host memory owned by the caller and read only during the
+ *    call; the context owns its device buffers;
+ *  - one host thread per context; calls on one context are not re-entrant.
+ *
+ * Index conventions (identical to the reference's arrays, 0-based):
+ *   radial cells  i = 0..nr-1 (i = 0 is the void between R* and Rin),
+ *   theta cells   j = 0..nt-1 (reference j+1), cell c = i*nt + j;
+ *   R[0..nr] radial edges in AU, R[0] = Rstar/AU          (Init.f:2787-2852)
+ *   Theta[0..nt] cos(theta) edges, Theta[0]=1, Theta[nt]=0 (Init.f:2876-2886)
+ *   BB[T*nlam + l], T = 0..tmax                          (Init.f:2616-2622)
+ *   per-species tables [s*nlam + l]; Kp[s*(tmax+1) + T]  (Init.f:3784-3790)
+ *   Mueller F[((s*nlam + l)*6 + k)*180 + a], k = F11,F12,F22,F33,F34,F44
+ *                                                        (Modules.f:53-57)
+ *   SED bins [a*nlam + l], a = 0..30 (MCRadiation.f:21, spec(nlam,0:nangle))
+ */
+#ifndef MCX_H
+#define MCX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCX_OK 0
+#define MCX_EINVAL (-1)   /* bad argument or inconsistent sizes          */
+#define MCX_EDEVICE (-2)  /* HIP runtime error                           */
+#define MCX_ENOMEM (-3)   /* device allocation failed                    */
+#define MCX_ESTATE (-4)   /* call out of order (e.g. no model set)       */
+#define MCX_EGEOM (-5)    /* packets lost to "nothing to hit"
+                             (MCRadiation.f:1223-1232 stops there)       */
+
+#define MCX_TMAX 10000    /* Modules.f:5 */
+#define MCX_NANGLE 30     /* MCRadiation.f:20 */
+
+typedef struct mcx_ctx mcx_ctx;
+
+/* Wavelength, dust and star tables: the module globals lam, nu, BB, Grain(:)
+ * and D%Fstar/D%Lstar read by MCRadiation (Modules.f:13, 59-83, 109-113). */
+typedef struct {
+    int32_t nlam, ng, tmax;          /* tmax must equal MCX_TMAX            */
+    const double *lam, *nu;          /* [nlam] micron, Hz                   */
+    const double *BB;                /* [(tmax+1)*nlam]                     */
+    const double *Kabs, *Ksca;       /* [ng*nlam] cm^2/g                    */
+    const double *F;                 /* [ng*nlam*6*180] (may be NULL when
+                                        scat_how != 2)                      */
+    const double *IF11, *IF12;       /* [ng*nlam]                           */
+    const double *Kp;                /* [ng*(tmax+1)]                       */
+    const double *Fstar;             /* [nlam]                              */
+    double Lstar;                    /* trapezoid integral of Fstar         */
+    const double *cos2phi, *sin2phi; /* [361] (Init.f:3777-3781)            */
+    int32_t scat_how;                /* 0 NONE, 1 ISOTROPIC, 2 FULL         */
+} mcx_tables;
+
+/* Grid and density: D%R, D%Theta and C(i,j)%dens/%V/%w (Modules.f:85-121). */
+typedef struct {
+    int32_t nr, nt;
+    const double *R, *Theta;         /* [nr+1], [nt+1]                      */
+    const double *dens, *V;          /* [nr*nt] g/cm^3, cm^3                */
+    const double *w;                 /* [nr*nt*ng] species abundances       */
+    double Rstar_cm, Tstar;          /* void-cell temperature, Main.f:546   */
+} mcx_grid;
+
+/* One MC pass: MCRadiation(NphotTot, ...) with nruns = 1. */
+typedef struct {
+    int64_t nphot;                   /* NphotTot                            */
+    uint64_t seed;                   /* Philox4x32-10 key; packet p uses the
+                                        counter stream (block, p)           */
+    int32_t maxinteract;             /* >0: roulette, MCRadiation.f:568-571 */
+    int32_t nlanes;                  /* 0: fill the device; 1: a single lane
+                                        (packets in order, as the serial
+                                        reference loop MCRadiation.f:320)   */
+    int64_t bitmap_bytes_max;        /* cap for the distinct-packet bitmap
+                                        behind Ni (0: 32 GiB)               */
+} mcx_run;
+
+/* Pass tallies and statistics (MCRadiation.f:656-715). */
+typedef struct {
+    double *EJv, *EJv2, *Eabs, *Tbw; /* [nr*nt] raw sums; Tbw = the cell's
+                                        Bjorkman-Wood T at the end          */
+    int32_t *Ni;                     /* [nr*nt] distinct packets per cell   */
+    double *spec, *scatspec;         /* [(MCX_NANGLE+1)*nlam], energy sums
+                                        already weighted as spec(l,a)       */
+    int32_t *ispec, *iscatspec;      /* [(MCX_NANGLE+1)*nlam] counts        */
+    int64_t nemit, ninteract, nabs, nscat, nescape, nstarhit, nsteps,
+            nraar, nmaxinteract, ngeomfail;
+    double tautot, Eescape;          /* sum of optical depth; escaped energy
+                                        (equals Lstar: energy conservation) */
+} mcx_tallies;
+
+/* Temperature products of Main.f:498-549 and the void cell (Main.f:546). */
+typedef struct {
+    double *T, *dT, *dEJv;           /* [nr*nt]                             */
+} mcx_temps;
+
+int  mcx_create(mcx_ctx **ctx, int device);
+void mcx_destroy(mcx_ctx *ctx);
+const char *mcx_last_error(const mcx_ctx *ctx);
+
+/* Replaces the reads of lam, nu, BB, Grain(:)%Kabs/Ksca/F/Kp, D%Fstar/Lstar
+ * that MCRadiation and interact make (MCRadiation.f:210-229,
+ * Radiation.f:41-355, 940-1001).  Builds the device-resident derived tables
+ * (emission CDFs, scattering-angle CDFs). */
+int mcx_set_tables(mcx_ctx *ctx, const mcx_tables *tab);
+
+/* Replaces the reads of D%R, D%Theta, C(i,j)%dens/V/w (MCRadiation.f:901,
+ * 1150).  Builds KabsTot/KscaTot (MCRadiation.f:210-229) and the per-cell
+ * Planck-mean table sum_s w_s Kp_s(T) (computeE, Radiation.f:611-627). */
+int mcx_set_model(mcx_ctx *ctx, const mcx_grid *grid);
+
+/* One transport pass: MCRadiation.f:269-668 on the device.  Tallies stay in
+ * HBM until mcx_get_tallies.  Returns MCX_EGEOM if any packet hit the
+ * reference's "nothing to hit" stop (the pass still completes). */
+int mcx_transport(mcx_ctx *ctx, const mcx_run *run);
+
+/* Copy the last pass's tallies to the host (any pointer may be NULL). */
+int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *out);
+
+/* Lucy temperature inversion of the last pass on the device
+ * (Main.f:498-549, determineT Radiation.f:629-702); copies T, dT, dEJv to
+ * the host when the pointers are non-NULL. */
+int mcx_temperature(mcx_ctx *ctx, mcx_temps *out);
+
+/* Duration in milliseconds of the last mcx_transport's packet kernel and of
+ * the whole pass, both measured with HIP events on the context's stream. */
+int mcx_last_timing(const mcx_ctx *ctx, double *kernel_ms, double *pass_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

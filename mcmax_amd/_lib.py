@@ -1,0 +1,107 @@
+"""ctypes binding of libmcx.so (include/mcx.h).
+
+The library is built in-tree (mcmax_amd/libmcx.so) by ``mcmax_amd.build``.
+There is no CPU fallback: if the library or a GPU is missing, loading fails
+with an error.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmcx.so")
+
+MCX_OK, MCX_EINVAL, MCX_EDEVICE, MCX_ENOMEM, MCX_ESTATE, MCX_EGEOM = 0, -1, -2, -3, -4, -5
+TMAX = 10000
+NANGLE = 30
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class Tables(C.Structure):
+    _fields_ = [
+        ("nlam", C.c_int32), ("ng", C.c_int32), ("tmax", C.c_int32),
+        ("lam", _dp), ("nu", _dp), ("BB", _dp), ("Kabs", _dp), ("Ksca", _dp),
+        ("F", _dp), ("IF11", _dp), ("IF12", _dp), ("Kp", _dp), ("Fstar", _dp),
+        ("Lstar", C.c_double), ("cos2phi", _dp), ("sin2phi", _dp),
+        ("scat_how", C.c_int32),
+    ]
+
+
+class Grid(C.Structure):
+    _fields_ = [
+        ("nr", C.c_int32), ("nt", C.c_int32), ("R", _dp), ("Theta", _dp),
+        ("dens", _dp), ("V", _dp), ("w", _dp), ("Rstar_cm", C.c_double), ("Tstar", C.c_double),
+    ]
+
+
+class Run(C.Structure):
+    _fields_ = [
+        ("nphot", C.c_int64), ("seed", C.c_uint64), ("maxinteract", C.c_int32),
+        ("nlanes", C.c_int32), ("bitmap_bytes_max", C.c_int64),
+    ]
+
+
+class Tallies(C.Structure):
+    _fields_ = [
+        ("EJv", _dp), ("EJv2", _dp), ("Eabs", _dp), ("Tbw", _dp), ("Ni", _ip),
+        ("spec", _dp), ("scatspec", _dp), ("ispec", _ip), ("iscatspec", _ip),
+        ("nemit", C.c_int64), ("ninteract", C.c_int64), ("nabs", C.c_int64),
+        ("nscat", C.c_int64), ("nescape", C.c_int64), ("nstarhit", C.c_int64),
+        ("nsteps", C.c_int64), ("nraar", C.c_int64), ("nmaxinteract", C.c_int64),
+        ("ngeomfail", C.c_int64), ("tautot", C.c_double), ("Eescape", C.c_double),
+    ]
+
+
+class Temps(C.Structure):
+    _fields_ = [("T", _dp), ("dT", _dp), ("dEJv", _dp)]
+
+
+EXPORTS = (
+    "mcx_create", "mcx_destroy", "mcx_last_error", "mcx_set_tables", "mcx_set_model",
+    "mcx_transport", "mcx_get_tallies", "mcx_temperature", "mcx_last_timing",
+)
+
+_lib = None
+
+
+def load():
+    """Load libmcx.so; raises RuntimeError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    lib.mcx_create.argtypes = [C.POINTER(vp), C.c_int]
+    lib.mcx_create.restype = C.c_int
+    lib.mcx_destroy.argtypes = [vp]
+    lib.mcx_destroy.restype = None
+    lib.mcx_last_error.argtypes = [vp]
+    lib.mcx_last_error.restype = C.c_char_p
+    lib.mcx_set_tables.argtypes = [vp, C.POINTER(Tables)]
+    lib.mcx_set_model.argtypes = [vp, C.POINTER(Grid)]
+    lib.mcx_transport.argtypes = [vp, C.POINTER(Run)]
+    lib.mcx_get_tallies.argtypes = [vp, C.POINTER(Tallies)]
+    lib.mcx_temperature.argtypes = [vp, C.POINTER(Temps)]
+    lib.mcx_last_timing.argtypes = [vp, _dp, _dp]
+    for n in ("mcx_set_tables", "mcx_set_model", "mcx_transport", "mcx_get_tallies",
+              "mcx_temperature", "mcx_last_timing"):
+        getattr(lib, n).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def ptr(a):
+    """Pointer to a C-contiguous numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_dp)
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(_ip)
+    raise TypeError(a.dtype)

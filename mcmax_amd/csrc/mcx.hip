@@ -1,0 +1,1262 @@
+// mcx.hip -- MI355X (gfx950) Monte Carlo photon transport for MCMax-style
+// axisymmetric disk models, behind the C-ABI of include/mcx.h.
+//
+// Design (one HIP stream per context, everything resident in HBM):
+//  * one lane per photon packet; lanes of a wavefront pull packet numbers
+//    from a device counter with one wave-aggregated atomic, so a lane whose
+//    packet escapes starts the next packet at once (persistent kernel);
+//  * each packet draws from its own Philox4x32-10 counter stream
+//    (key = seed, counter = (block, packet)), so a packet's random numbers do
+//    not depend on scheduling;
+//  * the Lucy path-length tallies EJv, EJv2 and the absorbed energy Eabs are
+//    integer fixed-point sums (per-cell units), accumulated with 64-bit
+//    integer atomics: the sums do not depend on the order of additions;
+//  * Bjorkman-Wood reemission takes each absorption's slice of the cell's
+//    cumulative absorbed energy from the value the Eabs fetch-add returns,
+//    i.e. every absorption corrects its own interval of the cell's energy,
+//    exactly as in the serial loop (Radiation.f:173-258);
+//  * reemission wavelengths are drawn by binary search in per-species
+//    cumulative tables int_0^lambda B_nu(T) kappa dnu (one row per integer
+//    T), instead of rebuilding the O(nlam) difference spectrum per
+//    absorption (Radiation.f:187-213, 966-989);
+//  * Ni (distinct packets per cell, MCRadiation.f:943-946) comes from a
+//    cell x packet bitmap: a packet sets its bit when it enters a cell, and a
+//    popcount per cell after the pass gives the exact count.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string>
+#include <vector>
+
+#include "../../include/mcx.h"
+#include "mcx_device.h"
+
+#define TMAXD MCX_DTMAX
+#define NANGD MCX_DNANG
+#define MAXNG 16
+
+// fixed-point resolutions
+#define EJV_BITS 31   // one EJv event of the cell's bound maps to 2^31
+#define EJV2_BITS 40  // one squared bound maps to 2^40 (3 x 32-bit limbs)
+#define EABS_BITS 24  // one nominal packet energy maps to 2^24
+
+enum { C_NEMIT = 0, C_NINTERACT, C_NABS, C_NSCAT, C_NESCAPE, C_NSTARHIT, C_NSTEPS, C_NRAAR, C_NMAXINT,
+       C_NGEOM, C_NOVERFLOW, C_NCOUNT };
+
+struct DevTables {
+    int nlam, ng, scat_how, scattering;
+    const double *nu;        // [nlam]
+    const double *cumStar;   // [nlam-1]
+    double Lstar;
+    const double *CUM;       // [ng][TMAX+1][nlam-1]
+    const double *Kp;        // [ng][TMAX+1]
+    const double *Ksca;      // [ng][nlam]
+    const double *F;         // [ng][nlam][6][180]
+    const double *C11, *C12; // [ng][nlam][180]
+    const double *IF11, *IF12;
+    const double *cos2phi, *sin2phi; // [361]
+};
+
+struct DevModel {
+    int nr, nt, ncell;
+    const double *R2, *T2, *Theta; // [nr+1], [nt+1], [nt+1]
+    double R0;                     // R[0] in AU
+    const double *dens, *mass;     // [ncell]
+    const double *w;               // [ncell*ng]
+    const double2 *KT;             // [ncell*nlam] {KabsTot, KscaTot}
+    const double *KpCell;          // [ncell*(TMAX+1)]
+};
+
+struct DevTally {
+    unsigned long long *EJv;    // [ncell]
+    unsigned long long *EJv2;   // [ncell*3] limbs of weight 2^0, 2^32, 2^64
+    unsigned long long *Eabs;   // [ncell]
+    const double *invU1, *invU2;
+    unsigned long long *Tcell;  // [ncell] bits of the latest BW T (monotone)
+    unsigned *bitmap;           // [ncell*nwords]
+    unsigned long long *specE;  // [2][(NANG+1)*nlam]
+    unsigned *ispec;            // [2][(NANG+1)*nlam]
+    unsigned long long *counters;
+    double *tautot;
+};
+
+struct TransportArgs {
+    DevTables tb;
+    DevModel md;
+    DevTally ty;
+    unsigned long long *next_pid;
+    long long pid_end;
+    long long pid_base;
+    long long nphot;
+    unsigned long long seed;
+    int maxinteract;
+    int nwords;
+    double invE;   // 2^EABS_BITS / E0
+    double unitE;  // E0 / 2^EABS_BITS
+};
+
+enum { S_NEED = 0, S_EMIT, S_STEP, S_INTERACT, S_ESCAPE, S_DONE };
+
+// ---------------------------------------------------------------------------
+// Temperature from absorbed energy per gram (Radiation.f:537-562, 629-702).
+__device__ double dev_determineT(const double *__restrict__ kpc, double E1, double Tcell) {
+    int iTmin = 1, iTmax = TMAXD;
+    int iT = (int)(Tcell / 1.0);
+    if (iT < 1) iT = 1;
+    if (iT > TMAXD - 1) iT = TMAXD - 1;
+    double E = kpc[iT];
+    double Emax = kpc[iTmax];
+    double Emin = kpc[1];
+    if (isnan(E1)) return 3.0;
+    if (E1 > Emax) return (double)(TMAXD - 1);
+    if (E1 < Emin) return 1.0;
+    int iT0 = iT, niter = 0;
+    while (abs(iTmax - iTmin) > 1 && niter <= TMAXD) {
+        niter++;
+        iT = (int)(pow(E1 / E, 0.25) * (double)iT);
+        if (iT == iT0) {
+            if (E1 < E) iT = iT0 - 1;
+            if (E1 > E) iT = iT0 + 1;
+        }
+        if (iT <= iTmin) iT = iTmin + 1;
+        if (iT >= iTmax) iT = iTmax - 1;
+        E = kpc[iT];
+        if (E >= E1) { iTmax = iT; Emax = E; }
+        if (E <= E1) { iTmin = iT; Emin = E; }
+        iT0 = iT;
+    }
+    if (iTmin == iTmax) return (double)iTmin;
+    float a4 = fpow4(iTmin), b4 = fpow4(iTmax);
+    float d4 = b4 - a4;
+    return pow((double)a4 + ((double)d4 * (E1 - Emin)) / (Emax - Emin), 0.25);
+}
+
+// increaseT: walk up from int(Tstart); when the start is already above E1
+// the reference's walk can only fail and fall back to determineT, so do that.
+__device__ double dev_increaseT(const double *__restrict__ kpc, double E1, double Tstart) {
+    int j = (int)(Tstart / 1.0);
+    if (j > TMAXD) j = TMAXD;
+    if (j < 1) j = 1;
+    double kp0 = kpc[j];
+    if (kp0 <= E1) {
+        for (int i = j; i <= TMAXD - 1; i++) {
+            double kp1 = kpc[i + 1];
+            if (kp0 <= E1 && kp1 >= E1) {
+                float a4 = fpow4(i), b4 = fpow4(i + 1);
+                float d4 = b4 - a4;
+                return pow((double)a4 + ((double)d4 * (E1 - kp0)) / (kp1 - kp0), 0.25);
+            }
+            kp0 = kp1;
+        }
+    }
+    return dev_determineT(kpc, E1, Tstart);
+}
+
+// Radiation.f:1007-1024
+__device__ __forceinline__ void random_direction(PhiloxStream &g, double &x, double &y, double &z) {
+    double r;
+    for (;;) {
+        x = 2.0 * g.next() - 1.0;
+        y = 2.0 * g.next() - 1.0;
+        z = 2.0 * g.next() - 1.0;
+        r = x * x + y * y + z * z;
+        if (!(r > 1.0)) break;
+    }
+    r = sqrt(r);
+    x = x / r;
+    y = y / r;
+    z = z / r;
+}
+
+// First bin b in [0, n) with cum(b) >= Lr; n when none.
+template <typename F>
+__device__ __forceinline__ int lower_bound_bins(int n, double Lr, F cum) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cum(mid) >= Lr) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// emit()'s wavelength rule (Radiation.f:966-989): cumulative trapezoid, the
+// weight of the lower grid point rises linearly through the chosen bin.
+template <typename F>
+__device__ __forceinline__ void pick_lambda(int nbins, double Lr, F cum, int &l1, double &wl1) {
+    int b = lower_bound_bins(nbins, Lr, cum);
+    if (b >= nbins) {
+        l1 = nbins - 1;
+        wl1 = 0.0;
+        return;
+    }
+    double Lt = cum(b);
+    double Ltold = (b > 0) ? cum(b - 1) : 0.0;
+    wl1 = (Lr - Ltold) / (Lt - Ltold);
+    l1 = b;
+}
+
+struct Packet {
+    double x, y, z, vx, vy, vz, E, wl1, Q, U, V, Sx, Sy, Sz;
+    double tau0, tau, nEJv;
+    int i, jj, l1, edgeNr, ntrace, lastc;
+    bool onEdge, scatt, pol;
+};
+
+__device__ __forceinline__ void tally_ejv(const DevTally &ty, int c, double x) {
+    double q = rint(x * ty.invU1[c]);
+    if (q > 0.0) atomicAdd(&ty.EJv[c], (unsigned long long)q);
+}
+
+__device__ __forceinline__ void tally_ejv2(const DevTally &ty, int c, double x2, unsigned long long *scnt) {
+    double q = x2 * ty.invU2[c];
+    if (!(q > 0.0)) return;
+    double l2 = floor(q * 0x1p-64);
+    if (l2 > 0x1p31) { atomicAdd(&scnt[C_NOVERFLOW], 1ull); l2 = 0x1p31; }
+    q = q - l2 * 0x1p64;
+    double l1 = floor(q * 0x1p-32);
+    double l0 = rint(q - l1 * 0x1p32);
+    unsigned long long *b = ty.EJv2 + 3 * (size_t)c;
+    if (l0 > 0.0) atomicAdd(b, (unsigned long long)l0);
+    if (l1 > 0.0) atomicAdd(b + 1, (unsigned long long)l1);
+    if (l2 > 0.0) atomicAdd(b + 2, (unsigned long long)l2);
+}
+
+__global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
+    extern __shared__ double lds[];
+    const DevTables &tb = A.tb;
+    const DevModel &md = A.md;
+    const DevTally &ty = A.ty;
+    const int nr = md.nr, nt = md.nt, nlam = tb.nlam, ng = tb.ng;
+    const int nbins = nlam - 1;
+    double *sR2 = lds;
+    double *sT2 = sR2 + (nr + 1);
+    double *sTh = sT2 + (nt + 1);
+    double *sC2 = sTh + (nt + 1);
+    double *sS2 = sC2 + 361;
+    for (int k = threadIdx.x; k <= nr; k += blockDim.x) sR2[k] = md.R2[k];
+    for (int k = threadIdx.x; k <= nt; k += blockDim.x) {
+        sT2[k] = md.T2[k];
+        sTh[k] = md.Theta[k];
+    }
+    for (int k = threadIdx.x; k < 361; k += blockDim.x) {
+        sC2[k] = tb.cos2phi[k];
+        sS2[k] = tb.sin2phi[k];
+    }
+    __syncthreads();
+
+    const int lane = __lane_id();
+    // event counters live in LDS (one set per block) to keep them out of VGPRs
+    __shared__ unsigned long long scnt[C_NCOUNT];
+    __shared__ double stau;
+    for (int k = threadIdx.x; k < C_NCOUNT; k += blockDim.x) scnt[k] = 0ull;
+    if (threadIdx.x == 0) stau = 0.0;
+    __syncthreads();
+#define CNT(k) atomicAdd(&scnt[k], 1ull)
+    double tautot = 0.0;
+
+    Packet p;
+    memset(&p, 0, sizeof(p));
+    PhiloxStream g;
+    g.init(A.seed, 0);
+    long long pid = -1;
+    int st = S_NEED;
+    const long long raar_limit = (long long)nr * (long long)(nt + 1) * 2;
+    const double AU = MCX_AU;
+
+    for (;;) {
+        // ---- wave-aggregated packet acquisition -------------------------
+        {
+            bool need = (st == S_NEED);
+            unsigned long long mask = __ballot(need);
+            if (mask) {
+                int leader = __ffsll((long long)mask) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(A.next_pid, (unsigned long long)__popcll(mask));
+                unsigned lo = __shfl((unsigned)base, leader);
+                unsigned hi = __shfl((unsigned)(base >> 32), leader);
+                base = ((unsigned long long)hi << 32) | lo;
+                if (need) {
+                    unsigned long long below = mask & ((1ull << lane) - 1ull);
+                    long long mine = (long long)(base + (unsigned long long)__popcll(below));
+                    if (mine >= A.pid_end) {
+                        st = S_DONE;
+                    } else {
+                        pid = mine;
+                        g.init(A.seed, (uint64_t)pid);
+                        p.lastc = -1;
+                        st = S_EMIT;
+                    }
+                }
+            }
+        }
+        if (__all(st == S_DONE)) break;
+        if (st == S_DONE) continue;
+
+        // ---- stellar emission (MCRadiation.f:347-383, 500-513, 554) ------
+        if (st == S_EMIT) {
+            p.nEJv = 0.0;
+            (void)g.next(); // where_emit
+            random_direction(g, p.x, p.y, p.z);
+            p.x = md.R0 * p.x;
+            p.y = md.R0 * p.y;
+            p.z = md.R0 * p.z;
+            CNT(C_NEMIT);
+            random_direction(g, p.vx, p.vy, p.vz); // inside emit()
+            double Lr = g.next() * tb.Lstar;
+            pick_lambda(nbins, Lr, [&](int b) { return tb.cumStar[b]; }, p.l1, p.wl1);
+            p.E = tb.Lstar / ((double)(float)A.nphot * ((1.0 - 0.0) - 0.0));
+            random_direction(g, p.vx, p.vy, p.vz);
+            if (p.x * p.vx + p.y * p.vy + p.z * p.vz < 0.0) {
+                p.vx = -p.vx;
+                p.vy = -p.vy;
+                p.vz = -p.vz;
+            }
+            p.edgeNr = 1;
+            p.onEdge = true;
+            double ct = fabs(p.z) / md.R0;
+            for (int j = 0; j < nt; j++)
+                if (ct < sTh[j] && ct > sTh[j + 1]) {
+                    p.i = 0;
+                    p.jj = j;
+                }
+            p.pol = false;
+            p.scatt = false;
+            p.Q = 0.0;
+            p.U = 0.0;
+            p.V = 0.0;
+            {
+                double sx = -p.vy, sy = p.vx, sz = 0.0;
+                double r = sqrt(sx * sx + sy * sy + sz * sz);
+                p.Sx = sx / r;
+                p.Sy = sy / r;
+                p.Sz = sz / r;
+            }
+            p.E = p.E * 1.0;
+            p.tau0 = -log(g.next());
+            p.tau = 0.0;
+            p.ntrace = 0;
+            st = S_STEP;
+        }
+
+        // ---- one cell step of trace2d (MCRadiation.f:941-1138) ----------
+        if (st == S_STEP) {
+            const int c = p.i * nt + p.jj;
+            if (c != p.lastc) {
+                long long rel = pid - A.pid_base;
+                atomicOr(&ty.bitmap[(size_t)c * A.nwords + (size_t)(rel >> 5)], 1u << (rel & 31));
+                p.lastc = c;
+            }
+            const double2 k1 = md.KT[(size_t)c * nlam + p.l1];
+            const double2 k2 = md.KT[(size_t)c * nlam + p.l1 + 1];
+            const double wl2 = 1.0 - p.wl1;
+            double Kabs = k1.x * p.wl1 + k2.x * wl2;
+            double Kext = tb.scattering ? ((k1.x + k1.y) * p.wl1 + (k2.x + k2.y) * wl2) : Kabs;
+            double v;
+            int inext, jnext;
+            bool ok = exit_face(sR2, sT2, nt, p.x, p.y, p.z, p.vx, p.vy, p.vz, p.i, p.jj, p.onEdge, p.edgeNr, v,
+                                inext, jnext);
+            if (!ok) {
+                CNT(C_NGEOM);
+                st = S_NEED;
+            } else {
+                CNT(C_NSTEPS);
+                double tau1 = ((Kext * v) * md.dens[c]) * AU;
+                if ((p.tau + tau1) > p.tau0) {
+                    v = (v * (p.tau0 - p.tau)) / tau1;
+                    tautot += 2.0 * (p.tau0 - p.tau);
+                    p.x = p.x + p.vx * v;
+                    p.y = p.y + p.vy * v;
+                    p.z = p.z + p.vz * v;
+                    p.onEdge = false;
+                    double ejv = ((p.E * v) * AU) * Kabs;
+                    tally_ejv(ty, c, ejv);
+                    p.nEJv = p.nEJv + ejv;
+                    st = S_INTERACT;
+                } else {
+                    p.x = p.x + p.vx * v;
+                    p.y = p.y + p.vy * v;
+                    p.z = p.z + p.vz * v;
+                    double ejv = ((p.E * v) * AU) * Kabs;
+                    p.nEJv = p.nEJv + ejv;
+                    tally_ejv2(ty, c, p.nEJv * p.nEJv, scnt);
+                    p.nEJv = 0.0;
+                    tally_ejv(ty, c, ejv);
+                    p.onEdge = true;
+                    p.tau = p.tau + tau1;
+                    tautot += tau1;
+                    if (inext >= nr) {
+                        st = S_ESCAPE;
+                    } else if (inext < 0) {
+                        CNT(C_NSTARHIT);
+                        st = S_EMIT;
+                    } else {
+                        p.i = inext;
+                        p.jj = jnext;
+                        p.ntrace++;
+                        if (p.ntrace > raar_limit) {
+                            CNT(C_NRAAR);
+                            st = S_EMIT;
+                        }
+                    }
+                }
+            }
+        }
+
+        // ---- interaction (Radiation.f:41-355) ---------------------------
+        if (st == S_INTERACT) {
+            CNT(C_NINTERACT);
+            const int c = p.i * nt + p.jj;
+            const int l1 = p.l1, l2 = p.l1 + 1;
+            const double2 k1 = md.KT[(size_t)c * nlam + l1];
+            const double2 k2 = md.KT[(size_t)c * nlam + l2];
+            const double wl2 = 1.0 - p.wl1;
+            double Ksca = k1.y * p.wl1 + k2.y * wl2;
+            double Kext = Ksca + (k1.x * p.wl1 + k2.x * wl2);
+            Kext = Kext + 0.0;
+            double KabsL = Kext - Ksca;
+            double Albedo = Ksca / Kext;
+            bool absorb = true;
+            if (tb.scattering) absorb = (g.next() > Albedo);
+            if (absorb) {
+                CNT(C_NABS);
+                p.scatt = false;
+                p.pol = false;
+                p.Q = 0.0;
+                p.U = 0.0;
+                p.V = 0.0;
+                double Ex = (p.E * (KabsL - 0.0)) / KabsL;
+                double qd = rint(Ex * A.invE);
+                unsigned long long q = (unsigned long long)qd;
+                unsigned long long old = atomicAdd(&ty.Eabs[c], q);
+                const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
+                double mass = md.mass[c];
+                double Thint = __longlong_as_double((long long)__hip_atomic_load(
+                    &ty.Tcell[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                double T0 = 0.0;
+                if (old != 0ull) T0 = dev_increaseT(kpc, ((double)old * A.unitE) / mass, Thint);
+                double T1 = dev_increaseT(kpc, ((double)(old + q) * A.unitE) / mass, T0);
+                atomicMax(&ty.Tcell[c], (unsigned long long)__double_as_longlong(T1));
+                int iT0 = (int)(T0 / 1.0), iT1 = (int)(T1 / 1.0);
+                double e0 = T0 - (double)(float)iT0;
+                double e1 = T1 - (double)(float)iT1;
+                if (iT0 >= TMAXD - 1) iT0 = TMAXD - 2;
+                if (iT1 >= TMAXD - 1) iT1 = TMAXD - 1;
+                if (iT0 < 1) iT0 = 1;
+                if (iT1 < 2) iT1 = 2;
+                const bool same = (iT0 == iT1);
+                const double *wc = md.w + (size_t)c * ng;
+                // coefficient of the four Planck rows of the difference spectrum
+                const double a1 = e1, a2 = 1.0 - e1, a3 = e0, a4 = 1.0 - e0;
+                // Planck-mean difference of species s, times its weight
+                auto kps = [&](int s) {
+                    const double *Kp = tb.Kp + (size_t)s * (TMAXD + 1);
+                    double d = same ? (Kp[iT0 + 1] - Kp[iT0])
+                                    : (((a1 * Kp[iT1 + 1] + a2 * Kp[iT1]) - a3 * Kp[iT0 + 1]) - a4 * Kp[iT0]);
+                    return (d * wc[s]) * 1.0;
+                };
+                double kp = 0.0;
+                for (int s = 0; s < ng; s++) kp = kp + kps(s);
+                // emit(): new direction, then the wavelength
+                CNT(C_NEMIT);
+                random_direction(g, p.vx, p.vy, p.vz);
+                {
+                    double sx = -p.vy, sy = p.vx, sz = 0.0;
+                    double r = sqrt(sx * sx + sy * sy + sz * sz);
+                    p.Sx = sx / r;
+                    p.Sy = sy / r;
+                    p.Sz = sz / r;
+                }
+                double Lr = g.next() * kp;
+                int s = 0;
+                double Lrs = Lr;
+                if (ng > 1) {
+                    double acc = 0.0;
+                    int last = -1;
+                    s = -1;
+                    for (int k = 0; k < ng; k++) {
+                        double kk = kps(k);
+                        if (kk > 0.0) last = k;
+                        double prev = acc;
+                        acc = acc + kk;
+                        if (kk > 0.0 && acc >= Lr) {
+                            s = k;
+                            Lrs = Lr - prev;
+                            break;
+                        }
+                    }
+                    if (s < 0) {
+                        s = (last >= 0) ? last : 0;
+                        Lrs = 2.0 * kps(s) + 1.0; // beyond the species total: top bin rule
+                    }
+                }
+                const double ws = wc[s];
+                const double *Cs = tb.CUM + (size_t)s * (TMAXD + 1) * nbins;
+                const double *r0 = Cs + (size_t)iT0 * nbins, *r0p = Cs + (size_t)(iT0 + 1) * nbins;
+                const double *r1 = Cs + (size_t)iT1 * nbins, *r1p = Cs + (size_t)(iT1 + 1) * nbins;
+                if (same) {
+                    pick_lambda(nbins, Lrs, [&](int b) { return ((r0p[b] - r0[b]) * ws) * 1.0; }, p.l1, p.wl1);
+                } else {
+                    pick_lambda(nbins, Lrs,
+                                [&](int b) {
+                                    return ((((a1 * r1p[b] + a2 * r1[b]) - a3 * r0p[b]) - a4 * r0[b]) * ws) * 1.0;
+                                },
+                                p.l1, p.wl1);
+                }
+            } else {
+                CNT(C_NSCAT);
+                p.scatt = true;
+                if (tb.scat_how == 1) {
+                    random_direction(g, p.vx, p.vy, p.vz);
+                } else {
+                    // pick the (species, lambda) Mueller matrix (Radiation.f:273-290)
+                    double KscaR = Ksca * g.next();
+                    const double *wc = md.w + (size_t)c * ng;
+                    size_t moff = (size_t)(ng - 1) * nlam + l1;
+                    for (int s = 0; s < ng; s++) {
+                        double w1 = wc[s] * 1.0;
+                        double w2 = (w1 * wl2) * tb.Ksca[(size_t)s * nlam + l2];
+                        KscaR = KscaR - w2;
+                        if (KscaR < 0.0) { moff = (size_t)s * nlam + l2; break; }
+                        w1 = (w1 * p.wl1) * tb.Ksca[(size_t)s * nlam + l1];
+                        KscaR = KscaR - w1;
+                        if (KscaR < 0.0) { moff = (size_t)s * nlam + l1; break; }
+                    }
+                    const double *M = tb.F + moff * 6 * 180;
+                    const double *C11 = tb.C11 + moff * 180, *C12 = tb.C12 + moff * 180;
+                    const double MIF11 = tb.IF11[moff], MIF12 = tb.IF12[moff];
+                    // scatangle (Radiation.f:381-495)
+                    double IF11 = MIF11 * 180.0, IF12 = MIF12 * 180.0;
+                    double phi;
+                    if (p.pol) {
+                        double Itot = 0.0;
+                        for (int k = 1; k <= 180; k++)
+                            Itot = Itot + IF11 * p.E + IF12 * (p.Q * sC2[k] + p.U * sS2[k]);
+                        Itot = g.next() * Itot;
+                        double I1 = 0.0, I0 = 0.0;
+                        phi = MCX_PI;
+                        for (int k = 1; k <= 180; k++) {
+                            I0 = I0 + IF11 * p.E + IF12 * (p.Q * sC2[k] + p.U * sS2[k]);
+                            if (I0 > Itot) {
+                                double wp1 = (I0 - Itot) / (I0 - I1);
+                                phi = (MCX_PI * ((double)k - wp1)) / 180.0;
+                                break;
+                            }
+                            I1 = I0;
+                        }
+                    } else {
+                        phi = g.next() * MCX_PI;
+                    }
+                    double sx = p.Sx, sy = p.Sy, sz = p.Sz;
+                    rotate_about(sx, sy, sz, p.vx, p.vy, p.vz, phi);
+                    if (p.pol) {
+                        // RotateStokes (Radiation.f:506-534)
+                        double P = (p.U / p.E) * (p.U / p.E) + (p.Q / p.E) * (p.Q / p.E);
+                        if (P != 0.0) {
+                            double cost = p.Sx * sx + p.Sy * sy + p.Sz * sz;
+                            double sint = (sx - (p.Sx * (p.vy * p.vy + p.vz * p.vz) -
+                                                 p.vx * (p.vy * p.Sy + p.vz * p.Sz)) * cost) /
+                                          (p.vy * p.Sz - p.vz * p.Sy);
+                            double cos2t = cost * cost - sint * sint;
+                            double sin2t = (2.0 * sint) * cost;
+                            double Qn = p.Q * cos2t + p.U * sin2t;
+                            double Un = -p.Q * sin2t + p.U * cos2t;
+                            double P2 = (Un / p.E) * (Un / p.E) + (Qn / p.E) * (Qn / p.E);
+                            double sPP2 = sqrt(P / P2);
+                            p.U = sPP2 * Un;
+                            p.Q = sPP2 * Qn;
+                        }
+                    }
+                    p.Sx = sx;
+                    p.Sy = sy;
+                    p.Sz = sz;
+                    double Fr = 180.0 * (p.E * MIF11 + p.Q * MIF12);
+                    Fr = g.next() * Fr;
+                    // first angle bin whose cumulative exceeds Fr
+                    const double E = p.E, Qs = p.Q;
+                    int lo = 0, hi = 180;
+                    while (lo < hi) {
+                        int mid = (lo + hi) >> 1;
+                        if (E * C11[mid] + Qs * C12[mid] > Fr) hi = mid;
+                        else lo = mid + 1;
+                    }
+                    double theta;
+                    int ia;
+                    if (lo >= 180) {
+                        theta = MCX_PI;
+                        ia = 180;
+                    } else {
+                        double Fi = E * C11[lo] + Qs * C12[lo];
+                        double FiOld = (lo > 0) ? (E * C11[lo - 1] + Qs * C12[lo - 1]) : 0.0;
+                        ia = lo + 1;
+                        theta = (double)ia - (Fi - Fr) / (Fi - FiOld);
+                        theta = (theta * MCX_PI) / 180.0;
+                    }
+                    if (g.next() < 0.5) theta = -theta;
+                    const double *F11 = M, *F12 = M + 180, *F22 = M + 360, *F33 = M + 540, *F34 = M + 720,
+                                 *F44 = M + 900;
+                    double En = 0.0, Qn = 0.0, Un = 0.0, Vn = 0.0;
+                    En = En + F11[ia - 1] * p.E;
+                    En = En + F12[ia - 1] * p.Q;
+                    Qn = Qn + F12[ia - 1] * p.E;
+                    Qn = Qn + F22[ia - 1] * p.Q;
+                    Un = Un + F33[ia - 1] * p.U;
+                    double Fs = (theta >= 0.0) ? fabs(F34[ia - 1]) : -fabs(F34[ia - 1]);
+                    Un = Un + Fs * p.V;
+                    Vn = Vn - Fs * p.U;
+                    Vn = Vn + F44[ia - 1] * p.V;
+                    if (En != 0.0) {
+                        p.Q = (p.E * Qn) / En;
+                        p.U = (p.E * Un) / En;
+                        p.V = (p.E * Vn) / En;
+                    }
+                    rotate_about(p.vx, p.vy, p.vz, p.Sx, p.Sy, p.Sz, theta);
+                    double r = sqrt(p.vx * p.vx + p.vy * p.vy + p.vz * p.vz);
+                    p.vx /= r;
+                    p.vy /= r;
+                    p.vz /= r;
+                    r = sqrt(p.Sx * p.Sx + p.Sy * p.Sy + p.Sz * p.Sz);
+                    p.Sx /= r;
+                    p.Sy /= r;
+                    p.Sz /= r;
+                    if (ia != 1 && ia != 180) p.pol = true;
+                }
+            }
+            st = S_STEP;
+            if (A.maxinteract > 0) {
+                if (g.next() < (1.0 / (double)(float)A.maxinteract)) {
+                    CNT(C_NMAXINT);
+                    st = S_NEED;
+                } else {
+                    p.E = p.E * (double)((float)A.maxinteract / (float)(A.maxinteract - 1));
+                }
+            }
+            if (st == S_STEP) {
+                p.tau0 = -log(g.next());
+                p.tau = 0.0;
+                p.ntrace = 0;
+            }
+        }
+
+        // ---- escape: MC spectrum (MCRadiation.f:576-646) ----------------
+        if (st == S_ESCAPE) {
+            CNT(C_NESCAPE);
+            int ia = (p.vz > 0.0) ? (int)(p.vz * 30.0 + 1.0) : (int)(-p.vz * 30.0 + 1.0);
+            if (ia > NANGD) ia = NANGD;
+            int il = (p.wl1 > 1.0 - p.wl1) ? p.l1 : p.l1 + 1;
+            size_t bin = (size_t)ia * nlam + il;
+            unsigned long long q = (unsigned long long)rint(p.E * A.invE);
+            atomicAdd(&ty.specE[bin], q);
+            atomicAdd(&ty.ispec[bin], 1u);
+            if (p.scatt) {
+                size_t off = (size_t)(NANGD + 1) * nlam;
+                atomicAdd(&ty.specE[off + bin], q);
+                atomicAdd(&ty.ispec[off + bin], 1u);
+            }
+            st = S_NEED;
+        }
+    }
+
+    // ---- counters: one global atomic per block -------------------------
+    if (tautot != 0.0) atomicAdd(&stau, tautot);
+    __syncthreads();
+    for (int k = threadIdx.x; k < C_NCOUNT; k += blockDim.x)
+        if (scnt[k]) atomicAdd(&ty.counters[k], scnt[k]);
+    if (threadIdx.x == 0 && stau != 0.0) atomicAdd(ty.tautot, stau);
+#undef CNT
+}
+
+// ---------------------------------------------------------------------------
+// per-cell tables (MCRadiation.f:210-229; Radiation.f:611-627)
+__global__ void k_cell_opacity(int ncell, int nlam, int ng, const double *__restrict__ Kabs,
+                               const double *__restrict__ Ksca, const double *__restrict__ w, double2 *KT) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)ncell * nlam) return;
+    int c = (int)(idx / nlam), l = (int)(idx % nlam);
+    double ka = 0.0, ks = 0.0;
+    for (int s = 0; s < ng; s++) {
+        ka = ka + (Kabs[(size_t)s * nlam + l] * w[(size_t)c * ng + s]) * 1.0;
+        ks = ks + (Ksca[(size_t)s * nlam + l] * w[(size_t)c * ng + s]) * 1.0;
+    }
+    KT[idx] = make_double2(ka, ks);
+}
+
+__global__ void k_cell_kp(int ncell, int ng, const double *__restrict__ Kp, const double *__restrict__ w,
+                          double *KpCell) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)ncell * (TMAXD + 1)) return;
+    int c = (int)(idx / (TMAXD + 1)), T = (int)(idx % (TMAXD + 1));
+    double e = 0.0;
+    for (int s = 0; s < ng; s++) e = e + (Kp[(size_t)s * (TMAXD + 1) + T] * w[(size_t)c * ng + s]) * 1.0;
+    KpCell[idx] = e;
+}
+
+// fixed-point units: a path segment inside a spherical shell is at most the
+// tangent chord 2 sqrt(R2^2 - R1^2); with the cell's largest KabsTot this
+// bounds one EJv contribution.
+__global__ void k_cell_units(int nr, int nt, int nlam, const double *__restrict__ R2, const double2 *__restrict__ KT,
+                             double E0, double *invU1, double *invU2) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nr * nt) return;
+    int i = c / nt;
+    double kmax = 0.0;
+    for (int l = 0; l < nlam; l++) kmax = fmax(kmax, KT[(size_t)c * nlam + l].x);
+    double chord = 2.0 * sqrt(fmax(R2[i + 1] - R2[i], 0.0));
+    double B = E0 * MCX_AU * chord * kmax * (1.0 + 1.0 / 64.0);
+    if (!(B > 0.0)) B = 1e-300;
+    invU1[c] = ldexp(1.0, EJV_BITS) / B;
+    invU2[c] = ldexp(1.0, EJV2_BITS) / (B * B);
+}
+
+__global__ void k_popcount(int ncell, int nwords, const unsigned *__restrict__ bitmap, int *Ni) {
+    int c = blockIdx.x;
+    if (c >= ncell) return;
+    const unsigned *row = bitmap + (size_t)c * nwords;
+    int s = 0;
+    for (int k = threadIdx.x; k < nwords; k += blockDim.x) s += __popc(row[k]);
+    __shared__ int red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) Ni[c] += red[0];
+}
+
+__global__ void k_finalize(int ncell, const unsigned long long *EJvq, const unsigned long long *EJv2q,
+                           const unsigned long long *Eabsq, const double *invU1, const double *invU2, double unitE,
+                           const unsigned long long *Tbits, double *EJv, double *EJv2, double *Eabs, double *Tbw) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    EJv[c] = (double)EJvq[c] / invU1[c];
+    const unsigned long long *L = EJv2q + 3 * (size_t)c;
+    double s = ((double)L[2] * 0x1p64 + (double)L[1] * 0x1p32) + (double)L[0];
+    EJv2[c] = s / invU2[c];
+    Eabs[c] = (double)Eabsq[c] * unitE;
+    Tbw[c] = __longlong_as_double((long long)Tbits[c]);
+}
+
+// Main.f:498-549 per cell; void cell temperature Main.f:546-548.
+__global__ void k_temperature(int nr, int nt, int ng, const double *__restrict__ EJv, const double *__restrict__ EJv2,
+                              const int *__restrict__ Ni, const double *__restrict__ Tbw, const double *__restrict__ V,
+                              const double *__restrict__ KpCell, const double *__restrict__ w,
+                              const double *__restrict__ Kp, double Tvoid, double *T, double *dT, double *dEJv) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nr * nt) return;
+    int i = c / nt;
+    if (i == 0) {
+        T[c] = Tvoid;
+        dT[c] = 1.0;
+        dEJv[c] = 0.0;
+        return;
+    }
+    double Vc = V[c];
+    double E = EJv[c] / Vc;
+    double E2 = EJv2[c] / (Vc * Vc);
+    double Tc = dev_determineT(KpCell + (size_t)c * (TMAXD + 1), E, Tbw[c]);
+    if (Tc < 1.0) Tc = 1.0;
+    double dTc, dE;
+    if (Ni[c] > 1) {
+        float fN = (float)Ni[c];
+        dE = (E + sqrt(fabs(E2 - (E * E) / (double)fN))) / (double)sqrtf(fN);
+        dTc = ((0.25 * Tc) * dE) / E;
+    } else {
+        dTc = (double)(float)TMAXD;
+        dE = 0.0;
+        for (int s = 0; s < ng; s++) dE = dE + (w[(size_t)c * ng + s] * 1.0) * Kp[(size_t)s * (TMAXD + 1) + TMAXD];
+    }
+    if (isnan(Tc)) Tc = 1.0;
+    if (isnan(dTc)) dTc = 1.0;
+    T[c] = Tc;
+    dT[c] = dTc;
+    dEJv[c] = dE;
+}
+
+// ===========================================================================
+// host side
+struct mcx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+    std::vector<void *> tab_alloc, mod_alloc, run_alloc;
+    bool have_tables = false, have_model = false, have_pass = false;
+    // tables
+    DevTables tb{};
+    int nlam = 0, ng = 0;
+    double Lstar = 0.0;
+    std::vector<double> h_Kp, h_nu;
+    double *d_Kabs = nullptr, *d_Ksca = nullptr;
+    // model
+    DevModel md{};
+    double *d_V = nullptr, *d_R2 = nullptr;
+    double Tvoid = 1.0;
+    // run
+    DevTally ty{};
+    size_t bitmap_bytes = 0;
+    int *d_Ni = nullptr;
+    double *d_EJv = nullptr, *d_EJv2 = nullptr, *d_Eabs = nullptr, *d_Tbw = nullptr;
+    double *d_T = nullptr, *d_dT = nullptr, *d_dEJv = nullptr;
+    unsigned long long *d_next = nullptr;
+    double unitE = 0.0, E0 = 0.0;
+    double kernel_ms = 0.0, pass_ms = 0.0;
+    int sm_count = 0;
+};
+
+static int fail(mcx_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                                     \
+    do {                                                                                                       \
+        hipError_t e_ = (call);                                                                                \
+        if (e_ != hipSuccess) return fail(ctx, MCX_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+static int dalloc(mcx_ctx *ctx, std::vector<void *> &list, T **p, size_t n) {
+    void *q = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess) return fail(ctx, MCX_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+    list.push_back(q);
+    *p = (T *)q;
+    return 0;
+}
+
+template <typename T>
+static int dupload(mcx_ctx *ctx, std::vector<void *> &list, T **p, const T *h, size_t n) {
+    int r = dalloc(ctx, list, p, n);
+    if (r) return r;
+    HIPCHK(ctx, hipMemcpyAsync(*p, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return 0;
+}
+
+static void free_list(std::vector<void *> &list) {
+    for (void *q : list) (void)hipFree(q);
+    list.clear();
+}
+
+extern "C" int mcx_create(mcx_ctx **out, int device) {
+    if (!out) return MCX_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MCX_EDEVICE;
+    if (device < 0 || device >= ndev) return MCX_EINVAL;
+    mcx_ctx *ctx = new mcx_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return MCX_EDEVICE;
+    }
+    for (int k = 0; k < 4; k++) (void)hipEventCreate(&ctx->ev[k]);
+    hipDeviceProp_t prop;
+    (void)hipGetDeviceProperties(&prop, device);
+    ctx->sm_count = prop.multiProcessorCount;
+    *out = ctx;
+    return MCX_OK;
+}
+
+extern "C" void mcx_destroy(mcx_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_list(ctx->run_alloc);
+    free_list(ctx->mod_alloc);
+    free_list(ctx->tab_alloc);
+    for (int k = 0; k < 4; k++)
+        if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char *mcx_last_error(const mcx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int mcx_set_tables(mcx_ctx *ctx, const mcx_tables *t) {
+    if (!ctx || !t) return MCX_EINVAL;
+    if (t->tmax != TMAXD) return fail(ctx, MCX_EINVAL, "tmax must be 10000");
+    if (t->nlam < 2 || t->ng < 1 || t->ng > MAXNG) return fail(ctx, MCX_EINVAL, "nlam >= 2 and 1 <= ng <= 16 required");
+    if (!t->nu || !t->BB || !t->Kabs || !t->Ksca || !t->Kp || !t->Fstar || !t->IF11 || !t->IF12 || !t->cos2phi ||
+        !t->sin2phi)
+        return fail(ctx, MCX_EINVAL, "missing table");
+    if (t->scat_how == 2 && !t->F) return fail(ctx, MCX_EINVAL, "scat_how=2 needs the Mueller matrices");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    free_list(ctx->run_alloc);
+    free_list(ctx->mod_alloc);
+    free_list(ctx->tab_alloc);
+    ctx->bitmap_bytes = 0;
+    ctx->ty.bitmap = nullptr;
+    ctx->have_model = ctx->have_pass = false;
+    const int nlam = t->nlam, ng = t->ng, nb = nlam - 1;
+    ctx->nlam = nlam;
+    ctx->ng = ng;
+    ctx->Lstar = t->Lstar;
+    ctx->h_nu.assign(t->nu, t->nu + nlam);
+    ctx->h_Kp.assign(t->Kp, t->Kp + (size_t)ng * (TMAXD + 1));
+    // cumulative trapezoids (the running sum emit() forms, Radiation.f:970-971)
+    std::vector<double> cumStar(nb), CUM((size_t)ng * (TMAXD + 1) * nb);
+    {
+        double Lt = 0.0;
+        for (int b = 0; b < nb; b++) {
+            Lt = Lt + (fabs(t->nu[b] - t->nu[b + 1]) * 0.5) * (t->Fstar[b] + t->Fstar[b + 1]);
+            cumStar[b] = Lt;
+        }
+    }
+    for (int s = 0; s < ng; s++) {
+        const double *K = t->Kabs + (size_t)s * nlam;
+        for (int T = 0; T <= TMAXD; T++) {
+            const double *B = t->BB + (size_t)T * nlam;
+            double *row = CUM.data() + ((size_t)s * (TMAXD + 1) + T) * nb;
+            double Lt = 0.0;
+            for (int b = 0; b < nb; b++) {
+                Lt = Lt + (fabs(t->nu[b] - t->nu[b + 1]) * 0.5) * (B[b] * K[b] + B[b + 1] * K[b + 1]);
+                row[b] = Lt;
+            }
+        }
+    }
+    // scattering-angle cumulatives: sum_{m<=a} pi sin(theta_m) F1x(m)
+    std::vector<double> C11, C12;
+    const bool full = (t->scat_how == 2);
+    if (full) {
+        C11.resize((size_t)ng * nlam * 180);
+        C12.resize((size_t)ng * nlam * 180);
+        for (size_t sl = 0; sl < (size_t)ng * nlam; sl++) {
+            const double *M = t->F + sl * 6 * 180;
+            double a11 = 0.0, a12 = 0.0;
+            for (int a = 1; a <= 180; a++) {
+                double thet = (MCX_PI * ((double)a - 0.5)) / 180.0;
+                double ps = MCX_PI * sin(thet);
+                a11 = a11 + ps * M[a - 1];
+                a12 = a12 + ps * M[180 + a - 1];
+                C11[sl * 180 + a - 1] = a11;
+                C12[sl * 180 + a - 1] = a12;
+            }
+        }
+    }
+    int scattering = 0;
+    if (t->scat_how != 0)
+        for (size_t k = 0; k < (size_t)ng * nlam; k++)
+            if (t->Ksca[k] != 0.0) scattering = 1;
+    DevTables &tb = ctx->tb;
+    tb.nlam = nlam;
+    tb.ng = ng;
+    tb.scat_how = t->scat_how;
+    tb.scattering = scattering;
+    tb.Lstar = t->Lstar;
+    double *p;
+    int r;
+#define UP(dst, src, n)                                                   \
+    do {                                                                  \
+        r = dupload(ctx, ctx->tab_alloc, &p, (const double *)(src), (n)); \
+        if (r) return r;                                                  \
+        dst = p;                                                          \
+    } while (0)
+    UP(tb.nu, t->nu, nlam);
+    UP(tb.cumStar, cumStar.data(), nb);
+    UP(tb.CUM, CUM.data(), CUM.size());
+    UP(tb.Kp, t->Kp, (size_t)ng * (TMAXD + 1));
+    UP(tb.Ksca, t->Ksca, (size_t)ng * nlam);
+    UP(tb.IF11, t->IF11, (size_t)ng * nlam);
+    UP(tb.IF12, t->IF12, (size_t)ng * nlam);
+    UP(tb.cos2phi, t->cos2phi, 361);
+    UP(tb.sin2phi, t->sin2phi, 361);
+    if (full) {
+        UP(tb.F, t->F, (size_t)ng * nlam * 6 * 180);
+        UP(tb.C11, C11.data(), C11.size());
+        UP(tb.C12, C12.data(), C12.size());
+    } else {
+        tb.F = tb.C11 = tb.C12 = nullptr;
+    }
+    UP(ctx->d_Kabs, t->Kabs, (size_t)ng * nlam);
+    ctx->d_Ksca = (double *)tb.Ksca;
+#undef UP
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->have_tables = true;
+    return MCX_OK;
+}
+
+extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
+    if (!ctx || !gr) return MCX_EINVAL;
+    if (!ctx->have_tables) return fail(ctx, MCX_ESTATE, "mcx_set_tables first");
+    if (gr->nr < 2 || gr->nt < 1 || !gr->R || !gr->Theta || !gr->dens || !gr->V || !gr->w)
+        return fail(ctx, MCX_EINVAL, "bad grid");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    free_list(ctx->run_alloc);
+    free_list(ctx->mod_alloc);
+    ctx->bitmap_bytes = 0;
+    ctx->ty.bitmap = nullptr;
+    ctx->have_model = ctx->have_pass = false;
+    const int nr = gr->nr, nt = gr->nt, ncell = nr * nt, ng = ctx->ng, nlam = ctx->nlam;
+    std::vector<double> R2(nr + 1), T2(nt + 1), mass(ncell);
+    for (int i = 0; i <= nr; i++) R2[i] = gr->R[i] * gr->R[i];
+    for (int j = 0; j <= nt; j++) T2[j] = gr->Theta[j] * gr->Theta[j];
+    for (int c = 0; c < ncell; c++) mass[c] = gr->V[c] * gr->dens[c];
+    DevModel &md = ctx->md;
+    md.nr = nr;
+    md.nt = nt;
+    md.ncell = ncell;
+    md.R0 = gr->R[0];
+    double *p;
+    int r;
+#define UP(dst, src, n)                                                   \
+    do {                                                                  \
+        r = dupload(ctx, ctx->mod_alloc, &p, (const double *)(src), (n)); \
+        if (r) return r;                                                  \
+        dst = p;                                                          \
+    } while (0)
+    UP(md.R2, R2.data(), nr + 1);
+    UP(md.T2, T2.data(), nt + 1);
+    UP(md.Theta, gr->Theta, nt + 1);
+    UP(md.dens, gr->dens, ncell);
+    UP(md.mass, mass.data(), ncell);
+    UP(md.w, gr->w, (size_t)ncell * ng);
+    UP(ctx->d_V, gr->V, ncell);
+#undef UP
+    ctx->d_R2 = (double *)md.R2;
+    double2 *KT;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &KT, (size_t)ncell * nlam))) return r;
+    double *KpCell;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &KpCell, (size_t)ncell * (TMAXD + 1)))) return r;
+    {
+        size_t n = (size_t)ncell * nlam;
+        hipLaunchKernelGGL(k_cell_opacity, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, ncell, nlam, ng,
+                           ctx->d_Kabs, ctx->d_Ksca, md.w, KT);
+        size_t m = (size_t)ncell * (TMAXD + 1);
+        hipLaunchKernelGGL(k_cell_kp, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream, ncell, ng,
+                           ctx->tb.Kp, md.w, KpCell);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    md.KT = KT;
+    md.KpCell = KpCell;
+    {
+        // Main.f:546
+        const double AU = MCX_AU;
+        double q = gr->Rstar_cm / (gr->R[1] * AU);
+        double T0 = gr->Tstar * 0.5 * (1.0 - sqrt(1.0 - q * q));
+        ctx->Tvoid = (T0 < 1.0) ? 1.0 : T0;
+    }
+    // per-pass tally buffers
+    DevTally &ty = ctx->ty;
+    double *invU1, *invU2;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ty.EJv, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ty.EJv2, (size_t)3 * ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ty.Eabs, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &invU1, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &invU2, ncell))) return r;
+    ty.invU1 = invU1;
+    ty.invU2 = invU2;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ty.Tcell, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ty.specE, (size_t)2 * (NANGD + 1) * nlam))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ty.ispec, (size_t)2 * (NANGD + 1) * nlam))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ty.counters, 16))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ty.tautot, 1))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Ni, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_EJv, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_EJv2, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Eabs, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Tbw, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_T, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_dT, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_dEJv, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_next, 1))) return r;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->have_model = true;
+    return MCX_OK;
+}
+
+extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
+    if (!ctx || !run) return MCX_EINVAL;
+    if (!ctx->have_model) return fail(ctx, MCX_ESTATE, "mcx_set_model first");
+    if (run->nphot <= 0) return fail(ctx, MCX_EINVAL, "nphot must be positive");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const DevModel &md = ctx->md;
+    DevTally &ty = ctx->ty;
+    const int ncell = md.ncell, nlam = ctx->nlam;
+    hipStream_t s = ctx->stream;
+    // distinct-packet bitmap: packets are run in batches whose bitmap fits
+    long long cap = run->bitmap_bytes_max > 0 ? run->bitmap_bytes_max : (32ll << 30);
+    long long per_packet_bytes = (long long)ncell; // ncell bits per 8 packets -> ncell/8 bytes per packet
+    long long batch = run->nphot;
+    long long max_batch = (cap * 8ll) / (per_packet_bytes > 0 ? per_packet_bytes : 1);
+    max_batch = (max_batch / 32) * 32;
+    if (max_batch < 32) max_batch = 32;
+    if (batch > max_batch) batch = max_batch;
+    int nwords = (int)((batch + 31) / 32);
+    size_t need = (size_t)ncell * (size_t)nwords * sizeof(unsigned);
+    if (need > ctx->bitmap_bytes) {
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        free_list(ctx->run_alloc);
+        ty.bitmap = nullptr;
+        int r = dalloc(ctx, ctx->run_alloc, &ty.bitmap, need / sizeof(unsigned));
+        if (r) return r;
+        ctx->bitmap_bytes = need;
+    }
+    const double E0 = ctx->Lstar / ((double)(float)run->nphot * ((1.0 - 0.0) - 0.0));
+    ctx->E0 = E0;
+    ctx->unitE = ldexp(E0, -EABS_BITS);
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
+    HIPCHK(ctx, hipMemsetAsync(ty.EJv, 0, sizeof(unsigned long long) * ncell, s));
+    HIPCHK(ctx, hipMemsetAsync(ty.EJv2, 0, sizeof(unsigned long long) * 3 * ncell, s));
+    HIPCHK(ctx, hipMemsetAsync(ty.Eabs, 0, sizeof(unsigned long long) * ncell, s));
+    HIPCHK(ctx, hipMemsetAsync(ty.Tcell, 0, sizeof(unsigned long long) * ncell, s));
+    HIPCHK(ctx, hipMemsetAsync(ty.specE, 0, sizeof(unsigned long long) * 2 * (NANGD + 1) * nlam, s));
+    HIPCHK(ctx, hipMemsetAsync(ty.ispec, 0, sizeof(unsigned) * 2 * (NANGD + 1) * nlam, s));
+    HIPCHK(ctx, hipMemsetAsync(ty.counters, 0, sizeof(unsigned long long) * 16, s));
+    HIPCHK(ctx, hipMemsetAsync(ty.tautot, 0, sizeof(double), s));
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_Ni, 0, sizeof(int) * ncell, s));
+    hipLaunchKernelGGL(k_cell_units, dim3((ncell + 255) / 256), dim3(256), 0, s, md.nr, md.nt, nlam, md.R2, md.KT, E0,
+                       (double *)ty.invU1, (double *)ty.invU2);
+    HIPCHK(ctx, hipGetLastError());
+    // launch shape: a full device of resident wavefronts, or one lane
+    int threads = 256, blocks = 1;
+    size_t lds = sizeof(double) * ((size_t)(md.nr + 1) + 2 * (size_t)(md.nt + 1) + 2 * 361);
+    if (run->nlanes == 1) {
+        threads = 1;
+        blocks = 1;
+    } else {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport, threads, lds) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        blocks = per_cu * ctx->sm_count;
+        long long need_blocks = (run->nphot + threads - 1) / threads;
+        if (blocks > need_blocks) blocks = (int)need_blocks;
+        if (run->nlanes > 1) {
+            long long b = (run->nlanes + threads - 1) / threads;
+            if (b < blocks) blocks = (int)b;
+        }
+        if (blocks < 1) blocks = 1;
+    }
+    TransportArgs A;
+    A.tb = ctx->tb;
+    A.md = md;
+    A.ty = ty;
+    A.next_pid = ctx->d_next;
+    A.nphot = run->nphot;
+    A.seed = run->seed;
+    A.maxinteract = run->maxinteract;
+    A.invE = ldexp(1.0, EABS_BITS) / E0;
+    A.unitE = ctx->unitE;
+    A.nwords = nwords;
+    float kms_total = 0.f;
+    for (long long base = 0; base < run->nphot; base += batch) {
+        long long end = base + batch;
+        if (end > run->nphot) end = run->nphot;
+        A.pid_base = base;
+        A.pid_end = end;
+        unsigned long long b0 = (unsigned long long)base;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_next, &b0, sizeof(b0), hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemsetAsync(ty.bitmap, 0, (size_t)ncell * nwords * sizeof(unsigned), s));
+        HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
+        hipLaunchKernelGGL(k_transport, dim3(blocks), dim3(threads), lds, s, A);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventRecord(ctx->ev[2], s));
+        hipLaunchKernelGGL(k_popcount, dim3(ncell), dim3(256), 0, s, ncell, nwords, ty.bitmap, ctx->d_Ni);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventSynchronize(ctx->ev[2]));
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
+        kms_total += ms;
+    }
+    hipLaunchKernelGGL(k_finalize, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell, ty.EJv, ty.EJv2, ty.Eabs,
+                       ty.invU1, ty.invU2, ctx->unitE, ty.Tcell, ctx->d_EJv, ctx->d_EJv2, ctx->d_Eabs, ctx->d_Tbw);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->ev[3], s));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[3]));
+    float pms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&pms, ctx->ev[0], ctx->ev[3]));
+    ctx->kernel_ms = kms_total;
+    ctx->pass_ms = pms;
+    ctx->have_pass = true;
+    unsigned long long cnt[16];
+    HIPCHK(ctx, hipMemcpy(cnt, ty.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (cnt[C_NGEOM] > 0) return fail(ctx, MCX_EGEOM, std::to_string(cnt[C_NGEOM]) + " packets found nothing to hit");
+    return MCX_OK;
+}
+
+extern "C" int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *o) {
+    if (!ctx || !o) return MCX_EINVAL;
+    if (!ctx->have_pass) return fail(ctx, MCX_ESTATE, "no transport pass yet");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const int ncell = ctx->md.ncell, nlam = ctx->nlam;
+    if (o->EJv) HIPCHK(ctx, hipMemcpy(o->EJv, ctx->d_EJv, sizeof(double) * ncell, hipMemcpyDeviceToHost));
+    if (o->EJv2) HIPCHK(ctx, hipMemcpy(o->EJv2, ctx->d_EJv2, sizeof(double) * ncell, hipMemcpyDeviceToHost));
+    if (o->Eabs) HIPCHK(ctx, hipMemcpy(o->Eabs, ctx->d_Eabs, sizeof(double) * ncell, hipMemcpyDeviceToHost));
+    if (o->Tbw) HIPCHK(ctx, hipMemcpy(o->Tbw, ctx->d_Tbw, sizeof(double) * ncell, hipMemcpyDeviceToHost));
+    if (o->Ni) HIPCHK(ctx, hipMemcpy(o->Ni, ctx->d_Ni, sizeof(int) * ncell, hipMemcpyDeviceToHost));
+    const size_t nb = (size_t)(NANGD + 1) * nlam;
+    std::vector<unsigned long long> sE(2 * nb);
+    std::vector<unsigned> sc(2 * nb);
+    HIPCHK(ctx, hipMemcpy(sE.data(), ctx->ty.specE, sizeof(unsigned long long) * 2 * nb, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(sc.data(), ctx->ty.ispec, sizeof(unsigned) * 2 * nb, hipMemcpyDeviceToHost));
+    // spec(l,a) = sum over escapes of wangle(a) E wlam(l)  (MCRadiation.f:54-78, 631-645)
+    std::vector<double> wlam(nlam), wangle(NANGD + 1, 0.0), cosangle(NANGD + 1);
+    const std::vector<double> &nu = ctx->h_nu;
+    for (int i = 1; i <= nlam; i++) {
+        if (i == 1) wlam[0] = -1e23 / (nu[1] / 2.0);
+        else if (i == nlam) wlam[nlam - 1] = -1e23 / ((nu[nlam - 1] - nu[nlam - 2]) / 2.0);
+        else wlam[i - 1] = -1e23 / ((nu[i] - nu[i - 2]) / 2.0);
+    }
+    for (int i = 0; i <= NANGD; i++) cosangle[i] = (double)i / (double)NANGD;
+    for (int i = 1; i <= NANGD; i++) wangle[i] = 1.0 / (cosangle[i] - cosangle[i - 1]);
+    double Eesc = 0.0;
+    for (int a = 0; a <= NANGD; a++)
+        for (int l = 0; l < nlam; l++) {
+            size_t k = (size_t)a * nlam + l;
+            double E = (double)sE[k] * ctx->unitE, Es = (double)sE[nb + k] * ctx->unitE;
+            Eesc += E;
+            if (o->spec) o->spec[k] = (wangle[a] * E) * wlam[l];
+            if (o->scatspec) o->scatspec[k] = (wangle[a] * Es) * wlam[l];
+            if (o->ispec) o->ispec[k] = (int32_t)sc[k];
+            if (o->iscatspec) o->iscatspec[k] = (int32_t)sc[nb + k];
+        }
+    unsigned long long cnt[16];
+    double tt = 0.0;
+    HIPCHK(ctx, hipMemcpy(cnt, ctx->ty.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(&tt, ctx->ty.tautot, sizeof(double), hipMemcpyDeviceToHost));
+    o->nemit = (int64_t)cnt[C_NEMIT];
+    o->ninteract = (int64_t)cnt[C_NINTERACT];
+    o->nabs = (int64_t)cnt[C_NABS];
+    o->nscat = (int64_t)cnt[C_NSCAT];
+    o->nescape = (int64_t)cnt[C_NESCAPE];
+    o->nstarhit = (int64_t)cnt[C_NSTARHIT];
+    o->nsteps = (int64_t)cnt[C_NSTEPS];
+    o->nraar = (int64_t)cnt[C_NRAAR];
+    o->nmaxinteract = (int64_t)cnt[C_NMAXINT];
+    o->ngeomfail = (int64_t)cnt[C_NGEOM];
+    o->tautot = tt;
+    o->Eescape = Eesc;
+    return MCX_OK;
+}
+
+extern "C" int mcx_temperature(mcx_ctx *ctx, mcx_temps *o) {
+    if (!ctx) return MCX_EINVAL;
+    if (!ctx->have_pass) return fail(ctx, MCX_ESTATE, "no transport pass yet");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const DevModel &md = ctx->md;
+    const int ncell = md.ncell;
+    hipLaunchKernelGGL(k_temperature, dim3((ncell + 255) / 256), dim3(256), 0, ctx->stream, md.nr, md.nt, ctx->ng,
+                       ctx->d_EJv, ctx->d_EJv2, ctx->d_Ni, ctx->d_Tbw, ctx->d_V, md.KpCell, md.w, ctx->tb.Kp,
+                       ctx->Tvoid, ctx->d_T, ctx->d_dT, ctx->d_dEJv);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (o) {
+        if (o->T) HIPCHK(ctx, hipMemcpy(o->T, ctx->d_T, sizeof(double) * ncell, hipMemcpyDeviceToHost));
+        if (o->dT) HIPCHK(ctx, hipMemcpy(o->dT, ctx->d_dT, sizeof(double) * ncell, hipMemcpyDeviceToHost));
+        if (o->dEJv) HIPCHK(ctx, hipMemcpy(o->dEJv, ctx->d_dEJv, sizeof(double) * ncell, hipMemcpyDeviceToHost));
+    }
+    return MCX_OK;
+}
+
+extern "C" int mcx_last_timing(const mcx_ctx *ctx, double *kernel_ms, double *pass_ms) {
+    if (!ctx) return MCX_EINVAL;
+    if (kernel_ms) *kernel_ms = ctx->kernel_ms;
+    if (pass_ms) *pass_ms = ctx->pass_ms;
+    return MCX_OK;
+}

@@ -1,0 +1,188 @@
+// mcx_device.h -- device-side building blocks of the MI355X transport:
+// Philox4x32-10 packet streams and the cell geometry of the (r, theta) grid.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MCX_DTMAX 10000
+#define MCX_DNANG 30
+
+// Modules.f:5-11.  AU is a single-precision literal in the reference.
+#define MCX_AU ((double)1.49598e13f)
+#define MCX_PI 3.14159265358979323846264338328
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  Stream of packet p under key `seed`:
+// counter = (block, p); each block yields two doubles, taken in order
+// (low 64 bits first).  Each double keeps the top 53 bits, centred: (0, 1).
+struct PhiloxStream {
+    uint32_t k0, k1;
+    uint64_t pid;
+    uint64_t blk;
+    double cache;
+    bool have;
+
+    __device__ void init(uint64_t seed, uint64_t p) {
+        k0 = (uint32_t)seed;
+        k1 = (uint32_t)(seed >> 32);
+        pid = p;
+        blk = 0;
+        have = false;
+        cache = 0.0;
+    }
+    __device__ double next() {
+        if (have) {
+            have = false;
+            return cache;
+        }
+        uint32_t c0 = (uint32_t)blk, c1 = (uint32_t)(blk >> 32);
+        uint32_t c2 = (uint32_t)pid, c3 = (uint32_t)(pid >> 32);
+        uint32_t a = k0, b = k1;
+#pragma unroll
+        for (int r = 0; r < 10; r++) {
+            uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+            uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+            uint32_t n0 = hi1 ^ c1 ^ a;
+            uint32_t n2 = hi0 ^ c3 ^ b;
+            c0 = n0;
+            c1 = lo1;
+            c2 = n2;
+            c3 = lo0;
+            a += 0x9E3779B9u;
+            b += 0xBB67AE85u;
+        }
+        blk++;
+        uint64_t v0 = ((uint64_t)c1 << 32) | c0;
+        uint64_t v1 = ((uint64_t)c3 << 32) | c2;
+        cache = ((double)(v1 >> 11) + 0.5) * 0x1p-53;
+        have = true;
+        return ((double)(v0 >> 11) + 0.5) * 0x1p-53;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Ray-sphere and ray-cone crossings, in the numerically stable q-form of the
+// reference (MCRadiation.f:1274-1355).  Thet is cos^2 of the cone edge.
+__device__ __forceinline__ bool hit_sphere(double Rad2, double r, double b, double &v) {
+    bool hit = false;
+    v = 1e200;
+    double cc = r - Rad2;
+    double discr = b * b - 4.0 * cc;
+    if (discr >= 0.0) {
+        discr = sqrt(discr);
+        double q = (b > 0.0) ? -0.5 * (b + discr) : -0.5 * (b - discr);
+        double vr1 = q, vr2 = cc / q;
+        if (vr1 > 0.0) { v = vr1; hit = true; }
+        if (vr2 > 0.0 && vr2 < v) { v = vr2; hit = true; }
+    }
+    return hit;
+}
+
+__device__ __forceinline__ bool hit_cone(double Thet, double r, double b, double z, double vz, double &v) {
+    bool hit = false;
+    v = 1e200;
+    double at = Thet - vz * vz;
+    double bt = Thet * b - (2.0 * z) * vz;
+    double ct = Thet * r - z * z;
+    double discr = bt * bt - (4.0 * at) * ct;
+    if (discr >= 0.0) {
+        discr = sqrt(discr);
+        double q = (bt > 0.0) ? -0.5 * (bt + discr) : -0.5 * (bt - discr);
+        double vt1 = q / at, vt2 = ct / q;
+        if (vt1 > 0.0) { v = vt1; hit = true; }
+        if (vt2 > 0.0 && vt2 < v) { v = vt2; hit = true; }
+    }
+    return hit;
+}
+
+__device__ __forceinline__ bool hit_cone_same(double Thet, double b, double z, double vz, double &v) {
+    double bt = Thet * b - (2.0 * z) * vz;
+    double at = Thet - vz * vz;
+    v = -bt / at;
+    return !(v <= 0.0);
+}
+
+// Distance to the exit face of cell (i, j) (MCRadiation.f:1150-1272).
+// R2/T2 are squared radial / cos-theta edges; ntheta_edges = nt + 1.
+// Returns false on "nothing to hit".  edgeNr is updated; jnext is 0-based.
+__device__ __forceinline__ bool exit_face(const double *__restrict__ R2, const double *__restrict__ T2,
+                                          int nt, double x, double y, double z, double vx, double vy,
+                                          double vz, int i, int jj, bool onEdge, int &edgeNr, double &v,
+                                          int &inext, int &jnext) {
+    const int NT = nt + 1;
+    double r = x * x + y * y + z * z;
+    double R1s = R2[i], R2s = R2[i + 1];
+    double T1s = T2[jj], T2s = T2[jj + 1];
+    double b = 2.0 * (x * vx + y * vy + z * vz);
+    int j = jj + 1;
+    bool h1, h2, h3, h4;
+    double vR1, vR2, vT1, vT2;
+    if (!onEdge) {
+        h1 = hit_sphere(R1s, r, b, vR1);
+        h2 = hit_sphere(R2s, r, b, vR2);
+        h3 = hit_cone(T1s, r, b, z, vz, vT1);
+        h4 = hit_cone(T2s, r, b, z, vz, vT2);
+    } else if (edgeNr == 1) {
+        h1 = false; vR1 = 1e200;
+        h2 = hit_sphere(R2s, r, b, vR2);
+        h3 = hit_cone(T1s, r, b, z, vz, vT1);
+        h4 = hit_cone(T2s, r, b, z, vz, vT2);
+    } else if (edgeNr == 2) {
+        h1 = hit_sphere(R1s, r, b, vR1);
+        h2 = true; vR2 = -b;
+        h3 = hit_cone(T1s, r, b, z, vz, vT1);
+        h4 = hit_cone(T2s, r, b, z, vz, vT2);
+    } else if (edgeNr == 3) {
+        h1 = hit_sphere(R1s, r, b, vR1);
+        h2 = hit_sphere(R2s, r, b, vR2);
+        h3 = false; vT1 = 1e200;
+        if (j != NT - 1) {
+            h4 = hit_cone(T2s, r, b, z, vz, vT2);
+        } else {
+            vT2 = -z / vz;
+            h4 = (vT2 > 0.0);
+        }
+    } else {
+        h1 = hit_sphere(R1s, r, b, vR1);
+        h2 = hit_sphere(R2s, r, b, vR2);
+        if (j != 1) h3 = hit_cone(T1s, r, b, z, vz, vT1);
+        else { h3 = false; vT1 = 1e200; }
+        if (j != NT - 1) h4 = hit_cone_same(T2s, b, z, vz, vT2);
+        else { h4 = false; vT2 = 1e200; }
+    }
+    if (!h1 && !h2 && !h3 && !h4) return false;
+    v = 1e200;
+    int in = i, jn = j;
+    if (h1 && vR1 < v) { v = vR1; in = i - 1; jn = j; edgeNr = 2; }
+    if (h2 && vR2 < v) { v = vR2; in = i + 1; jn = j; edgeNr = 1; }
+    if (h3 && vT1 < v) { v = vT1; in = i; jn = j - 1; edgeNr = 4; }
+    if (h4 && vT2 < v) { v = vT2; in = i; jn = j + 1; edgeNr = 3; }
+    if (jn == NT) { jn = NT - 1; edgeNr = 4; }
+    if (jn == 0) { jn = 1; edgeNr = 3; }
+    inext = in;
+    jnext = jn - 1;
+    return true;
+}
+
+// Subroutines.f:1648-1672: rotate (x,y,z) about the unit axis (u,v,w).
+__device__ __forceinline__ void rotate_about(double &x, double &y, double &z, double u, double v, double w,
+                                             double theta) {
+    double cost = cos(theta), sint = sin(theta);
+    double u2 = u * u, v2 = v * v, w2 = w * w;
+    double X = x, Y = y, Z = z;
+    double inp = X * u + Y * v + Z * w;
+    double y1 = u * inp + (X * (v2 + w2) - u * (v * Y + w * Z)) * cost + (v * Z - w * Y) * sint;
+    double y2 = v * inp + (Y * (u2 + w2) - v * (u * X + w * Z)) * cost + (w * X - u * Z) * sint;
+    double y3 = w * inp + (Z * (u2 + v2) - w * (u * X + v * Y)) * cost + (u * Y - v * X) * sint;
+    x = y1;
+    y = y2;
+    z = y3;
+}
+
+// (real(i))**4 in single precision, as the reference evaluates it in the
+// T^4 interpolation (Radiation.f:551, 698).
+__device__ __forceinline__ float fpow4(int i) {
+    float x = (float)i;
+    float x2 = x * x;
+    return x2 * x2;
+}

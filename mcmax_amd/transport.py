@@ -1,0 +1,123 @@
+"""Host interface of the transport stage, mirroring the reference's call
+sequence: ``MCRadiation(NphotTot, ...)`` (MCRadiation.f:6) followed by the
+Lucy temperature inversion (Main.f:498-558).  Everything runs in libmcx.so on
+the GPU; this module only moves arrays across the C-ABI.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import NANGLE, TMAX, ptr
+
+
+class MCXError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mcx error {code}: {msg}")
+        self.code = code
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Transport:
+    """One libmcx context on one GPU."""
+
+    def __init__(self, device=0):
+        self.lib = _lib.load()
+        self.ctx = C.c_void_p()
+        rc = self.lib.mcx_create(C.byref(self.ctx), int(device))
+        if rc != 0:
+            raise MCXError(rc, "mcx_create failed (no GPU visible?)")
+        self._keep = []
+        self.model = None
+
+    def close(self):
+        if self.ctx:
+            self.lib.mcx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MCXError(rc, self.lib.mcx_last_error(self.ctx).decode())
+        return rc
+
+    def set_model(self, m):
+        """Hand a model's tables and grid to the device (mcx_set_tables,
+        mcx_set_model)."""
+        keep = dict(
+            lam=_c(m.lam), nu=_c(m.nu), BB=_c(m.BB), Kabs=_c(m.Kabs), Ksca=_c(m.Ksca),
+            F=_c(m.F), IF11=_c(m.IF11), IF12=_c(m.IF12), Kp=_c(m.Kp), Fstar=_c(m.Fstar),
+            c2=_c(m.cos2phi), s2=_c(m.sin2phi), R=_c(m.R), Th=_c(m.Theta), dens=_c(m.dens),
+            V=_c(m.V), w=_c(m.w))
+        t = _lib.Tables(nlam=m.nlam, ng=m.ng, tmax=TMAX, lam=ptr(keep["lam"]), nu=ptr(keep["nu"]),
+                        BB=ptr(keep["BB"]), Kabs=ptr(keep["Kabs"]), Ksca=ptr(keep["Ksca"]),
+                        F=ptr(keep["F"]), IF11=ptr(keep["IF11"]), IF12=ptr(keep["IF12"]),
+                        Kp=ptr(keep["Kp"]), Fstar=ptr(keep["Fstar"]), Lstar=float(m.Lstar),
+                        cos2phi=ptr(keep["c2"]), sin2phi=ptr(keep["s2"]), scat_how=int(m.scat_how))
+        self._check(self.lib.mcx_set_tables(self.ctx, C.byref(t)))
+        g = _lib.Grid(nr=m.nr, nt=m.nt, R=ptr(keep["R"]), Theta=ptr(keep["Th"]), dens=ptr(keep["dens"]),
+                      V=ptr(keep["V"]), w=ptr(keep["w"]), Rstar_cm=float(m.Rstar_cm), Tstar=float(m.Tstar))
+        self._check(self.lib.mcx_set_model(self.ctx, C.byref(g)))
+        self.model = m
+
+    def transport(self, nphot, seed=1, nlanes=0, maxinteract=0, bitmap_bytes_max=0):
+        """One MC pass (mcx_transport); tallies stay on the device."""
+        r = _lib.Run(nphot=int(nphot), seed=int(seed), maxinteract=int(maxinteract), nlanes=int(nlanes),
+                     bitmap_bytes_max=int(bitmap_bytes_max))
+        rc = self.lib.mcx_transport(self.ctx, C.byref(r))
+        if rc not in (0, _lib.MCX_EGEOM):
+            self._check(rc)
+        return rc
+
+    def tallies(self):
+        m = self.model
+        n = m.nr * m.nt
+        nb = (NANGLE + 1) * m.nlam
+        out = {k: np.zeros(n) for k in ("EJv", "EJv2", "Eabs", "Tbw")}
+        out["Ni"] = np.zeros(n, dtype=np.int32)
+        for k in ("spec", "scatspec"):
+            out[k] = np.zeros(nb)
+        for k in ("ispec", "iscatspec"):
+            out[k] = np.zeros(nb, dtype=np.int32)
+        t = _lib.Tallies(**{k: ptr(v) for k, v in out.items()})
+        self._check(self.lib.mcx_get_tallies(self.ctx, C.byref(t)))
+        for k in ("EJv", "EJv2", "Eabs", "Tbw", "Ni"):
+            out[k] = out[k].reshape(m.nr, m.nt)
+        for k in ("spec", "scatspec", "ispec", "iscatspec"):
+            out[k] = out[k].reshape(NANGLE + 1, m.nlam)
+        for k in ("nemit", "ninteract", "nabs", "nscat", "nescape", "nstarhit", "nsteps", "nraar",
+                  "nmaxinteract", "ngeomfail", "tautot", "Eescape"):
+            out[k] = getattr(t, k)
+        return out
+
+    def temperature(self):
+        m = self.model
+        n = m.nr * m.nt
+        T, dT, dE = np.zeros(n), np.zeros(n), np.zeros(n)
+        o = _lib.Temps(T=ptr(T), dT=ptr(dT), dEJv=ptr(dE))
+        self._check(self.lib.mcx_temperature(self.ctx, C.byref(o)))
+        return T.reshape(m.nr, m.nt), dT.reshape(m.nr, m.nt), dE.reshape(m.nr, m.nt)
+
+    def timing(self):
+        k, p = C.c_double(), C.c_double()
+        self._check(self.lib.mcx_last_timing(self.ctx, C.byref(k), C.byref(p)))
+        return k.value, p.value
+
+
+def mc_radiation(tr, nphot, seed=1, nlanes=0, maxinteract=0):
+    """MCRadiation + temperature inversion; returns a dict with the tallies
+    and T, dT, dEJv."""
+    rc = tr.transport(nphot, seed=seed, nlanes=nlanes, maxinteract=maxinteract)
+    out = tr.tallies()
+    out["T"], out["dT"], out["dEJv"] = tr.temperature()
+    out["rc"] = rc
+    out["kernel_ms"], out["pass_ms"] = tr.timing()
+    return out

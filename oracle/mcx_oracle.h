@@ -1,0 +1,132 @@
+/*
+ * mcx_oracle.h -- CPU restatement (FP64, plain C) of MCMax's Monte Carlo
+ * photon-packet transport stage.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in mcmax_amd/: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links or calls
+ * it, and has no CPU fallback.
+ *
+ * Every routine cites the reference routine (file:line in michielmin/MCMax)
+ * that it restates.  Index conventions:
+ *   radial cells  i = 0 .. nr-1   (i = 0 is the void between R* and Rin;
+ *                                  reference C(0:nR-1, .), MCRadiation.f:199)
+ *   theta cells   jj = 0 .. nt-1  (reference j = jj+1, 1..nTheta-1)
+ *   cell index    c = i*nt + jj
+ *   R[0..nr]      radial edges in AU (R[0] = Rstar/AU)        (Init.f:2787)
+ *   Theta[0..nt]  cos(theta) edges, Theta[0]=1 pole, Theta[nt]=0 midplane
+ *                 (reference D%Theta(1..nTheta), Init.f:2876-2886)
+ *   BB[T*nlam+l], T = 0..TMAX                                 (Init.f:2616)
+ *   species tables are [ii*nlam + l]; Kp is [ii*(TMAX+1) + T]
+ *   Mueller tables F are [((ii*nlam + l)*6 + k)*180 + a], k = F11,F12,F22,
+ *   F33,F34,F44 (Modules.f:53-57)
+ */
+#ifndef MCX_ORACLE_H
+#define MCX_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_TMAX 10000
+#define ORC_NANGLE 30
+
+/* Physical constants exactly as the reference declares them (Modules.f:5-11):
+ * AU, parsec, Rsun, Msun, Lsun are single-precision literals widened to
+ * double; pi, kb, sigma are double literals. */
+double orc_const(int which); /* 0 pi, 1 AU, 2 parsec, 3 Rsun, 4 Msun, 5 Lsun */
+
+typedef struct {
+    int nlam;
+    const double *lam, *nu;       /* [nlam] micron, Hz                    */
+    const double *BB;             /* [(TMAX+1)*nlam]                      */
+    int ng;                       /* species (nopac = 1 each)             */
+    const double *Kabs, *Ksca, *Kext; /* [ng*nlam]                        */
+    const double *F;              /* [ng*nlam*6*180] Mueller elements     */
+    const double *IF11, *IF12;    /* [ng*nlam]                            */
+    const double *Kp;             /* [ng*(TMAX+1)]                        */
+    const double *Fstar;          /* [nlam]                               */
+    double Lstar;                 /* integral of Fstar (= L/4pi)          */
+    double Rstar_cm, Tstar;       /* for the void-cell temperature        */
+    int nr, nt;
+    const double *R, *Theta;      /* [nr+1], [nt+1]                       */
+    const double *dens, *V;       /* [nr*nt] g/cm^3, cm^3                 */
+    const double *w;              /* [nr*nt*ng] species weights           */
+    int scat_how;                 /* 0 none, 1 isotropic, 2 full Mueller  */
+    const double *cos2phi, *sin2phi; /* [361] (Init.f:3777-3781)          */
+} orc_model;
+
+typedef struct {
+    int64_t nphot;
+    int rng;                /* 0: NR ran1 stream (Subroutines.f:1481)
+                               1: Philox4x32-10, one stream per packet    */
+    uint64_t seed;          /* Philox key                                 */
+    int maxinteract;        /* > 0 enables the overflow roulette          */
+} orc_run;
+
+typedef struct {
+    double *EJv, *EJv2, *Eabs, *T; /* [nr*nt]: raw pass tallies; T = BW T  */
+    int32_t *Ni;                   /* [nr*nt]                               */
+    double *spec, *scatspec;       /* [(NANGLE+1)*nlam], bin a*nlam + l     */
+    int32_t *ispec, *iscatspec;    /* [(NANGLE+1)*nlam]                     */
+    int64_t nemit, ninteract, nabs, nscat, nescape, nstarhit, nsteps,
+            nraar, nmaxinteract;
+    double tautot, Eescape;
+    int32_t *nev1, *nev2;          /* optional [nr*nt]: number of additions
+                                      to EJv and to EJv2 (test diagnostics) */
+} orc_tally;
+
+/* NR ran1 state (iv, iy SAVEd and THREADPRIVATE in the reference). */
+typedef struct { int32_t idum, iy, iv[32]; } orc_ran1_state;
+
+/* ---- primitives ------------------------------------------------------ */
+double orc_ran1(orc_ran1_state *s);
+void   orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* uniform double in (0,1) from draw number k of stream (seed, pid)        */
+double orc_philox_uniform(uint64_t seed, uint64_t pid, uint64_t k);
+double orc_planck(double T, double lam);
+double orc_luminosity(double T, double R);
+double orc_integrate(const double *nu, const double *spec, int nlam);
+double orc_hg(double g, double theta);
+void   orc_rotate(double *x, double *y, double *z, double u, double v, double w, double theta);
+
+/* ---- table builders (Init.f) ----------------------------------------- */
+void orc_lam_grid(int nlam, double lam1, double lam2, double *lam, double *nu);
+void orc_bb_table(int nlam, const double *lam, const double *nu, double *BB, double *BBint);
+void orc_kp_table(int nlam, const double *nu, const double *BB, const double *Kabs, double *Kp);
+void orc_star(int nlam, const double *lam, const double *nu, double Rstar_cm, double Tstar,
+              double *Fstar, double *Lstar);
+int  orc_theta_grid(int ntheta_cells, double *Theta, double *theta_av);
+void orc_r_grid(int nrad, double Rin, double Rout, double Rstar_cm, double *R, double *R_av);
+void orc_min_density(int nr, int nt, const double *R, const double *Theta, const double *R_av,
+                     const double *theta_av, double Mdust_g, double Rin, double Rout,
+                     double sh1AU, double shpow, double denspow, double *dens, double *V);
+void orc_normalize_mueller(int nlam, double *F /* [nlam*6*180] one species */);
+void orc_mueller_integrals(int nlam, const double *F, double *IF11, double *IF12);
+void orc_phi_tables(double *cos2phi, double *sin2phi);
+
+/* ---- the transport pass and the temperature inversion ---------------- */
+/* Restates MCRadiation (MCRadiation.f:6-887) for the thermal-contact, plain
+ * stellar emission path.  Returns 0, or a negative error code.             */
+int orc_mcradiation(const orc_model *m, const orc_run *run, orc_ran1_state *rs, orc_tally *t);
+
+/* Restates Main.f:498-558 (with determineT, Radiation.f:629-702):
+ * EJv, EJv2 are the raw pass tallies; Tbw the Bjorkman-Wood cell T.
+ * Writes T, dT, dEJv for cells i>=1 and the void-cell T for i=0.        */
+void orc_temperature(const orc_model *m, const double *EJv, const double *EJv2,
+                     const int32_t *Ni, const double *Tbw,
+                     double *T, double *dT, double *dEJv);
+double orc_determineT(const orc_model *m, int c, double E1, double Tcell);
+
+/* Straight-ray geometry through the grid (trace2d with tau0 = inf):
+ * records up to maxseg (cell, length) pairs; returns number of segments,
+ * -1 on a geometry failure.  onEdge/edgeNr as the reference photon.       */
+int orc_trace_ray(const orc_model *m, double x, double y, double z,
+                  double vx, double vy, double vz, int i, int jj, int onEdge, int edgeNr,
+                  int maxseg, int32_t *cells, double *lens, int *exit_code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

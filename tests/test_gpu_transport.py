@@ -1,0 +1,109 @@
+"""GPU parity of the transport pass against the oracle.
+
+Serial tests run the device kernel with a single lane, so packets are traced
+one after another in packet order; the oracle in Philox mode draws from the
+same per-packet streams, so both passes follow the same histories and agree to
+floating-point rounding (the device uses fixed-point tallies and binary-search
+CDFs, the oracle double sums and the reference's linear scans).
+Parallel tests compare statistics.
+"""
+import numpy as np
+import pytest
+
+from mcmax_amd import transport
+
+pytestmark = pytest.mark.gpu
+
+COUNTERS = ("nemit", "ninteract", "nabs", "nscat", "nescape", "nstarhit", "nsteps", "nraar")
+
+
+def fixed_point_units(m, nphot):
+    """Per-cell resolution of the device's EJv and EJv2 integer sums: one
+    event of the cell's bound B = E0 AU (2 sqrt(R2^2-R1^2)) max(KabsTot)
+    maps to 2^31 (EJv) and B^2 to 2^40 (EJv2)."""
+    AU = float(np.float32(1.49598e13))
+    E0 = m.Lstar / float(np.float32(nphot))
+    KabsTot = np.einsum("ijs,sl->ijl", m.w, m.Kabs)
+    chord = 2.0 * np.sqrt(np.maximum(m.R[1:] ** 2 - m.R[:-1] ** 2, 0.0))
+    B = E0 * AU * chord[:, None] * KabsTot.max(axis=2) * (1 + 1 / 64)
+    return B * 2.0 ** -31, B * B * 2.0 ** -40
+
+
+def _serial_compare(tr, m, oracle_lib, n, seed, rtol=1e-6):
+    tr.set_model(m)
+    g = transport.mc_radiation(tr, n, seed=seed, nlanes=1)
+    o = oracle_lib.Oracle(m).mcradiation(n, rng="philox", seed=seed)
+    for k in COUNTERS:
+        if k == "nsteps":
+            # the z=0 plane is the cone cos^2(theta)=0, whose discriminant
+            # (MCRadiation.f:1318) is zero in exact arithmetic: whether a step
+            # registers the midplane crossing is decided by rounding, in the
+            # reference as well.  A crossing there keeps the cell and draws no
+            # random number, so it only splits a segment.
+            assert abs(g[k] - o[k]) <= 0.005 * o[k], (k, g[k], o[k])
+        else:
+            assert g[k] == o[k], (k, g[k], o[k])
+    assert np.array_equal(g["Ni"], o["Ni"])
+    u1, u2 = fixed_point_units(m, n)
+    # EJv: each device addition rounds to half a unit (split segments add at
+    # most one more addition each); the rest is double rounding
+    bound = 0.5 * u1 * (2 * o["nev1"] + 2) + 1e-9 * np.abs(o["EJv"]) + 1e-300
+    assert np.all(np.abs(g["EJv"] - o["EJv"]) <= bound), np.max(np.abs(g["EJv"] - o["EJv"]) / bound)
+    # EJv2 sums squares of segments: exact away from the midplane, only of the
+    # same order in the midplane cells where a split changes the squares
+    off = np.ones_like(o["EJv2"], dtype=bool)
+    off[:, -1] = False
+    bound2 = 0.5 * u2 * o["nev2"] + 1e-9 * np.abs(o["EJv2"]) + 1e-300
+    d2 = np.abs(g["EJv2"] - o["EJv2"])
+    assert np.all(d2[off] <= bound2[off]), np.max(d2[off] / bound2[off])
+    mid = o["EJv2"][:, -1] > 0
+    ratio = g["EJv2"][:, -1][mid] / o["EJv2"][:, -1][mid]
+    assert np.all((ratio > 0.5) & (ratio < 2.0)), ratio
+    for k, tol in (("Eabs", 1e-9), ("Tbw", 1e-9)):
+        a, b = g[k], o[k]
+        mask = np.abs(b) > 0
+        assert np.array_equal(a == 0, b == 0), k
+        rel = np.abs(a[mask] - b[mask]) / np.abs(b[mask])
+        assert rel.max() < tol, (k, rel.max())
+    assert np.array_equal(g["ispec"], o["ispec"])
+    assert np.array_equal(g["iscatspec"], o["iscatspec"])
+    mask = o["spec"] != 0
+    assert np.max(np.abs(g["spec"][mask] / o["spec"][mask] - 1)) < 1e-9
+    To, dTo, dEo = oracle_lib.Oracle(m).temperature(o)
+    # T ~ EJv^(1/4) at worst: a quarter of EJv's relative bound (the
+    # piecewise-linear T^4 interpolation keeps dT/T <= dE/E)
+    erel = (0.5 * u1 * o["nev1"] + 1e-9 * o["EJv"]) / np.maximum(o["EJv"], 1e-300)
+    trel = np.abs(g["T"] / To - 1)
+    assert np.all(trel <= np.minimum(erel, 1.0) + 1e-9), np.max(trel - erel)
+    return g, o
+
+
+def test_serial_parity_tiny(gpu, tiny_model, oracle_lib):
+    _serial_compare(gpu, tiny_model, oracle_lib, 400, seed=5)
+
+
+def test_serial_parity_small(gpu, small_model, oracle_lib):
+    _serial_compare(gpu, small_model, oracle_lib, 150, seed=17)
+
+
+def test_energy_conservation(gpu, small_model):
+    gpu.set_model(small_model)
+    g = transport.mc_radiation(gpu, 200000, seed=2)
+    assert g["ngeomfail"] == 0
+    assert g["nescape"] == 200000
+    assert abs(g["Eescape"] / small_model.Lstar - 1) < 1e-9
+
+
+def test_parallel_statistics_small(gpu, small_model, oracle_lib):
+    n = 100000
+    gpu.set_model(small_model)
+    g = transport.mc_radiation(gpu, n, seed=101)
+    o = oracle_lib.Oracle(small_model).mcradiation(n, rng="philox", seed=202)
+    To, _, _ = oracle_lib.Oracle(small_model).temperature(o)
+    Tg = g["T"]
+    sel = (o["Ni"][1:] > 50)
+    rel = np.abs(Tg[1:][sel] / To[1:][sel] - 1)
+    assert np.median(rel) < 0.01, np.median(rel)
+    # reemissions per packet agree within statistics
+    assert abs(g["nemit"] / o["nemit"] - 1) < 0.05
+    assert abs(g["nscat"] / o["nscat"] - 1) < 0.05
