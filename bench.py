@@ -58,7 +58,19 @@ def main():
     ap.add_argument("--config", default="production")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (rocprof counter calibration)")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        import torch
+        n = a.calibrate_copy // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n * 4}))
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: every run here uses a single MI355X "
               "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)

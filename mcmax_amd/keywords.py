@@ -17,7 +17,7 @@ _PLAIN = {
     "mdust", "iter", "maxiter", "epsiter", "fweight", "lam1", "lam2", "nlam", "scattype",
     "tcontact", "maxinteract", "nphotdiffuse", "dtdiffuse", "idum", "nspan", "nlev",
     "ntspan", "ntlev", "storescatt", "randomwalk", "forcefirst", "nruns", "dirparticle",
-    "outputfits", "radpress",
+    "outputfits", "radpress", "factrw",
 }
 _INDEXED = ("part", "abun", "computepart")
 
@@ -71,8 +71,19 @@ def parse_lines(lines):
         if len(val) >= 2 and val[0] in "\"'":
             val = val[1:-1]
         _check_key(key)
-        out[key] = parse_value(val) if key not in ("startype", "denstype", "scattype") and not key.startswith(("part", "computepart")) else val
+        out[key] = val if _is_text_key(key) else parse_value(val)
     return out
+
+
+def _is_text_key(key):
+    """Keys whose value is a name or a file, read as text by Init.f."""
+    if key in ("startype", "denstype", "scattype", "dirparticle"):
+        return True
+    if key.startswith("part") and key[4:].isdigit():
+        return True
+    if key.startswith("computepart") and (key.endswith(":file") or ":tfile" in key):
+        return True
+    return False
 
 
 def parse_file(path, overrides=()):

@@ -37,6 +37,7 @@ DEFAULTS = {  # Init.f:66-438 (the keys this build reads)
     "lam1": float(np.float32(0.1)), "lam2": 10000.0, "nlam": 100, "scattype": "ISOTROPIC",
     "tcontact": True, "maxinteract": 0, "nphotdiffuse": 25, "dtdiffuse": 1.0, "idum": -42,
     "ntspan": 0, "ntlev": 0, "nruns": 1, "forcefirst": False, "randomwalk": False,
+    "factrw": 10.0,
 }
 
 SCAT_HOW = {"NONE": 0, "ISOTROPIC": 1, "FULL": 2, "RAYLEIGH": 2}
@@ -110,12 +111,15 @@ def theta_grid(ntheta_cells):
     for i in range(1, NT):
         tav[i] = float(np.arccos((Th[NT - i + 1] + Th[NT - i]) / 2.0))
     Th[1] = 1.0
+    thet = [0.0] * (NT + 2)
     for i in range(2, NT):
+        thet[i] = (tav[i - 1] + tav[i]) / 2.0
         Th[i] = float(np.cos((tav[i - 1] + tav[i]) / 2.0))
     Th[NT] = 0.0
+    thet[NT] = PI / 2.0
     for i in range(1, NT):
         tav[i] = float(np.arccos((Th[i] + Th[i + 1]) / 2.0))
-    return np.array(Th[1:NT + 1]), np.array(tav[1:NT])
+    return np.array(Th[1:NT + 1]), np.array(tav[1:NT]), np.array(thet[1:NT + 1])
 
 
 def r_grid(nrad, Rin, Rout, Rstar_cm):
@@ -257,6 +261,18 @@ def mueller_integrals(F):
     return IF11, IF12
 
 
+def asymmetry(F):
+    """g = sum F11 cos sin / sum F11 sin per (species, lambda) (Init.f:4037-4050)."""
+    th = (PI * (np.arange(1, 181, dtype=np.float64) - 0.5)) / 180.0
+    c, s = np.cos(th), np.sin(th)
+    g = np.zeros(F.shape[:-2])
+    tot = np.zeros(F.shape[:-2])
+    for a in range(180):
+        g = g + (F[..., 0, a] * c[a]) * s[a]
+        tot = tot + F[..., 0, a] * s[a]
+    return g / tot
+
+
 def phi_tables():
     i = np.arange(361, dtype=np.float64)
     phi = (PI * (i - 1.0)) / 179.5
@@ -321,7 +337,7 @@ def build_model(kw, basedir="."):
     m.Rin, m.Rout = float(p["rin"]), float(p["rout"])
     if int(p.get("ntspan", 0)) * int(p.get("ntlev", 0)) != 0:
         raise SetupError("ntspan/ntlev refinement is not supported")
-    m.Theta, m.theta_av = theta_grid(int(p["ntheta"]))
+    m.Theta, m.theta_av, m.thet = theta_grid(int(p["ntheta"]))
     m.R, m.R_av = r_grid(int(p["nrad"]), m.Rin, m.Rout, m.Rstar_cm)
     m.nr, m.nt = len(m.R) - 1, len(m.Theta) - 1
     m.dens, m.V = min_density(m.R, m.Theta, m.R_av, m.theta_av, m.Mdust_g, m.Rin, m.Rout,
@@ -347,6 +363,8 @@ def build_model(kw, basedir="."):
         m.Kabs[k - 1], m.Ksca[k - 1], m.F[k - 1] = Ka, Ks, F
         m.Kext[k - 1] = Ka + Ks  # Init.f:3428-3436
     finish_opacities(m, p)
+    m.randomwalk = bool(p.get("randomwalk", False))
+    m.factRW = float(p.get("factrw", 10.0))
     warg = np.array([float(p.get(f"abun{k}", 1.0)) for k in range(1, ng + 1)])
     m.w = np.broadcast_to(warg / warg.sum(), (m.nr, m.nt, ng)).copy()
     return m
@@ -362,6 +380,7 @@ def finish_opacities(m, p=None):
         m.F[:, :, 0, :] = 1.0
     m.IF11, m.IF12 = mueller_integrals(m.F)
     m.cos2phi, m.sin2phi = phi_tables()
+    m.gsym = asymmetry(m.F)
     m.Kp = np.zeros((m.ng, TMAX + 1))
     for s in range(m.ng):
         m.Kp[s] = integrate(m.nu, m.BB * m.Kabs[s][None, :])

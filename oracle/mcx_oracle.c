@@ -358,6 +358,7 @@ typedef struct {
     int onEdge, scatt, pol;
     int64_t nr;
     double Q, U, V, Sx, Sy, Sz;
+    int irg; /* theta sub-cell for the SED rays (TraceMono.f) */
 } phot_t;
 
 typedef struct {
@@ -371,6 +372,11 @@ typedef struct {
     double nEJv;
     orc_tally *t;
     rng_t g;
+    /* modified random walk state */
+    int32_t *thick;
+    double *KDext, *KDabs;
+    int32_t *iTD;
+    double ry[1000], rphi[1000];
 } ctx_t;
 
 /* Radiation.f:1007-1024 */
@@ -541,7 +547,176 @@ static int trace2edge(const orc_model *m, int NT, phot_t *p, double *v, int *ine
     return 0;
 }
 
-/* MCRadiation.f:901-1141 (locfield, no etrace/RNDW/LRF/multiwav).
+/* ---- modified random walk (RandomWalk.f, Dtable.f) -------------------- */
+static double increaseT(const ctx_t *C, int c, double E1, double Tcell);
+
+void orc_diffcoeff(const orc_model *m, int c, int iT, int opacity_set, double *KDext, double *KDabs)
+{
+    /* Dtable.f:97-182 without QHP grains */
+    const int nlam = m->nlam, ng = m->ng;
+    if (iT < 1) iT = 1;
+    if (iT > TMAX - 1) iT = TMAX - 1;
+    double *Ksca = calloc(nlam, sizeof(double)), *Kext = calloc(nlam, sizeof(double));
+    double *Kabs = calloc(nlam, sizeof(double)), *g = calloc(nlam, sizeof(double));
+    double *dBB = calloc(nlam, sizeof(double)), *spec = calloc(nlam, sizeof(double));
+    const double *w = m->w + (size_t)c * ng;
+    int scattering = 0;
+    if (m->scat_how != 0)
+        for (int k = 0; k < ng * nlam; k++)
+            if (m->Ksca[k] != 0.0) scattering = 1;
+    for (int l = 0; l < nlam; l++) {
+        if (opacity_set) {
+            double ka = 0.0, ks = 0.0;
+            for (int s = 0; s < ng; s++) {
+                ka = ka + (m->Kabs[(size_t)s * nlam + l] * w[s]) * 1.0;
+                ks = ks + (m->Ksca[(size_t)s * nlam + l] * w[s]) * 1.0;
+            }
+            Ksca[l] = ks;
+            Kabs[l] = ka;
+            Kext[l] = Kabs[l] + Ksca[l];
+        } else {
+            double ke = 0.0, ks = 0.0;
+            for (int s = 0; s < ng; s++) {
+                ke = ke + (m->Kext[(size_t)s * nlam + l] * w[s]) * 1.0;
+                ks = ks + (m->Ksca[(size_t)s * nlam + l] * w[s]) * 1.0;
+            }
+            Kext[l] = ke;
+            Ksca[l] = ks;
+            Kabs[l] = Kext[l] - Ksca[l];
+        }
+        double gg = 0.0;
+        for (int s = 0; s < ng; s++)
+            gg = gg + ((m->gsym[(size_t)s * nlam + l] * m->Ksca[(size_t)s * nlam + l]) * w[s]) * 1.0;
+        g[l] = scattering ? gg / Ksca[l] : 0.0;
+        dBB[l] = m->BB[(size_t)(iT + 1) * nlam + l] - m->BB[(size_t)iT * nlam + l];
+    }
+    for (int l = 0; l < nlam; l++) spec[l] = dBB[l] / (Kext[l] - (g[l] * g[l]) * Ksca[l]);
+    double int1 = orc_integrate(m->nu, spec, nlam);
+    double int2 = orc_integrate(m->nu, dBB, nlam);
+    *KDext = int2 / int1;
+    for (int l = 0; l < nlam; l++) spec[l] = dBB[l] * (Kabs[l] - 0.0);
+    int1 = orc_integrate(m->nu, spec, nlam);
+    *KDabs = int1 / int2;
+    free(Ksca); free(Kext); free(Kabs); free(g); free(dBB); free(spec);
+}
+
+void orc_init_random_walk(const orc_model *m, double *y, double *phi, int32_t *thick)
+{
+    /* RandomWalk.f:152-194 */
+    const int NY = 1000, nmax = 1000;
+    y[0] = 1.0;
+    phi[0] = 1.0;
+    for (int i = 2; i <= NY; i++) {
+        y[i - 1] = (double)((float)(NY - i + 1) / (float)(NY - 1)) * 0.75;
+        double s = 0.0;
+        for (int n = 1; n <= nmax; n++) {
+            double sg = ((n + 1) % 2 == 0) ? 1.0 : -1.0;
+            s = s + sg * pow(y[i - 1], (double)(n * n));
+        }
+        phi[i - 1] = s * 2.0;
+    }
+    if (!thick) return;
+    const int iT1 = 100, iT2 = 1600, diT = (iT2 - iT1) / 3;
+    for (int i = 0; i < m->nr; i++)
+        for (int jj = 0; jj < m->nt; jj++) {
+            int c = i * m->nt + jj;
+            thick[c] = 0;
+            if (i == 0) continue;
+            for (int iT = iT1; iT <= iT2; iT += diT) {
+                double KDext, KDabs;
+                orc_diffcoeff(m, c, iT, 0, &KDext, &KDabs);
+                double lr = 1.0 / ((AU_C * KDext) * m->dens[c]);
+                if ((m->R[i + 1] - m->R[i]) > m->factRW * lr) thick[c] = 1;
+            }
+        }
+}
+
+/* Subroutines.f:2086-2130 for the decreasing phi table, starting at jlo = 1 */
+static int hunt_desc(const double *xx, int n, double x)
+{
+    int jlo = 1, jhi, inc = 1;
+    int ascnd = (xx[n - 1] >= xx[0]);
+    if ((x >= xx[jlo - 1]) == ascnd) {
+        for (;;) {
+            jhi = jlo + inc;
+            if (jhi > n) { jhi = n + 1; break; }
+            if ((x >= xx[jhi - 1]) == ascnd) { jlo = jhi; inc += inc; }
+            else break;
+        }
+    } else {
+        jhi = jlo;
+        for (;;) {
+            jlo = jhi - inc;
+            if (jlo < 1) { jlo = 0; break; }
+            if ((x < xx[jlo - 1]) == ascnd) { jhi = jlo; inc += inc; }
+            else break;
+        }
+    }
+    while (jhi - jlo != 1) {
+        int jm = (jhi + jlo) / 2;
+        if ((x >= xx[jm - 1]) == ascnd) jlo = jm;
+        else jhi = jm;
+    }
+    if (x == xx[n - 1]) jlo = n - 1;
+    if (x == xx[0]) jlo = 1;
+    return jlo;
+}
+
+/* RandomWalk.f:15-43 */
+static double min_dist(const orc_model *m, const phot_t *p)
+{
+    double rho = p->x * p->x + p->y * p->y;
+    double r = sqrt(rho + p->z * p->z);
+    rho = sqrt(rho);
+    double z = fabs(p->z);
+    double R1 = m->R[p->i], R2 = m->R[p->i + 1];
+    double T1 = m->Theta[p->jj], T2 = m->Theta[p->jj + 1];
+    double ST1 = sin(m->thet[p->jj]), ST2 = sin(m->thet[p->jj + 1]);
+    double vR1 = fabs(r - R1), vR2 = fabs(R2 - r);
+    double vT1 = fabs(T1 * rho - ST1 * z), vT2 = fabs(T2 * rho - ST2 * z);
+    double d = vR1;
+    if (vR2 < d) d = vR2;
+    if (vT1 < d) d = vT1;
+    if (vT2 < d) d = vT2;
+    return d;
+}
+
+/* RandomWalk.f:47-147 (tcontact, no QHP/LRF) */
+static int random_walk(ctx_t *C, phot_t *p, double dmin)
+{
+    const orc_model *m = C->m;
+    orc_tally *t = C->t;
+    int c = p->i * C->nt + p->jj;
+    int iT = (int)(t->T[c] / 1.0);
+    if (iT == 0) iT = 1;
+    if (iT > TMAX) iT = TMAX;
+    if (iT > TMAX - 1) iT = TMAX - 1; /* DiffCoeffCell clamps its argument */
+    if (iT != C->iTD[c]) {
+        orc_diffcoeff(m, c, iT, 1, &C->KDext[c], &C->KDabs[c]);
+        C->iTD[c] = iT;
+    }
+    double Kext = C->KDext[c];
+    double lr = 1.0 / ((AU_C * Kext) * m->dens[c]);
+    if (dmin <= m->factRW * lr) return 0;
+    double x = ran2(&C->g);
+    int iy = hunt_desc(C->rphi, 1000, x);
+    double v = -((3.0 * log(C->ry[iy - 1])) * (dmin * dmin)) / (lr * (PI * PI));
+    double Kabs = C->KDabs[c];
+    randomdirection(&C->g, &p->vx, &p->vy, &p->vz);
+    p->x = p->x + dmin * p->vx;
+    p->y = p->y + dmin * p->vy;
+    p->z = p->z + dmin * p->vz;
+    double EJv = (p->E * v) * AU_C;
+    t->Eabs[c] = t->Eabs[c] + (EJv * Kabs) * m->dens[c];
+    t->EJv[c] = t->EJv[c] + EJv * Kabs;
+    C->nEJv = C->nEJv + EJv * Kabs;
+    if (t->nev1) t->nev1[c]++;
+    t->T[c] = increaseT(C, c, t->Eabs[c] / C->mass[c], t->T[c]);
+    t->tautot = t->tautot + v / dmin;
+    return 1;
+}
+
+/* MCRadiation.f:901-1141 (locfield, no etrace/LRF/multiwav).
  * Returns 0 (interaction inside the cell, or escape/hitstar flagged), or
  * -1 on a geometry failure. */
 static int trace2d(ctx_t *C, phot_t *p, double tau0, int *escape, int *hitstar)
@@ -554,6 +729,22 @@ static int trace2d(ctx_t *C, phot_t *p, double tau0, int *escape, int *hitstar)
     *hitstar = 0;
     double tau = 0.0;
     int ntrace = 0;
+    /* MCRadiation.f:916-929 */
+    if (m->rndw && C->thick[p->i * nt + p->jj] && !p->onEdge) {
+        for (;;) {
+            double dmin = min_dist(m, p);
+            if (!random_walk(C, p, dmin)) break;
+            double vx, vy, vz;
+            for (;;) {
+                randomdirection(&C->g, &vx, &vy, &vz);
+                double inp = p->vx * vx + p->vy * vy + p->vz * vz;
+                if (!(inp < 0.0)) break;
+            }
+            p->vx = vx;
+            p->vy = vy;
+            p->vz = vz;
+        }
+    }
     for (;;) {
         int c = p->i * nt + p->jj;
         if (C->lastphotnr[c] != p->nr) {
@@ -952,6 +1143,11 @@ int orc_mcradiation(const orc_model *m, const orc_run *run, orc_ran1_state *rs, 
     C.KextL = (double *)calloc(ng, sizeof(double));
     C.spec = (double *)calloc(nlam, sizeof(double));
     C.lastphotnr = (int64_t *)calloc(ncell, sizeof(int64_t));
+    C.thick = (int32_t *)calloc(ncell, sizeof(int32_t));
+    C.KDext = (double *)calloc(ncell, sizeof(double));
+    C.KDabs = (double *)calloc(ncell, sizeof(double));
+    C.iTD = (int32_t *)calloc(ncell, sizeof(int32_t));
+    if (m->rndw) orc_init_random_walk(m, C.ry, C.rphi, C.thick);
     C.g.mode = run->rng;
     C.g.r1 = rs;
     C.g.seed = run->seed;
@@ -1035,6 +1231,7 @@ int orc_mcradiation(const orc_model *m, const orc_run *run, orc_ran1_state *rs, 
     }
     free(C.KabsTot); free(C.KscaTot); free(C.mass); free(C.KabsL); free(C.KextL);
     free(C.spec); free(C.lastphotnr);
+    free(C.thick); free(C.KDext); free(C.KDabs); free(C.iTD);
     return err;
 }
 
@@ -1114,4 +1311,930 @@ int orc_trace_ray(const orc_model *m, double x, double y, double z, double vx, d
     }
     *exit_code = 3;
     return n;
+}
+
+/* ====================================================================== */
+/* SED rays: Telescope.f:138-164 (SPECTRUM) and TraceMono.f                */
+/* ====================================================================== */
+#define NPHIS 180
+
+void orc_theta_edges(int ntheta_cells, double *thet)
+{
+    /* Init.f:2788-2815 then 2877-2885: thet(i) = (theta_av(i-1)+theta_av(i))/2
+     * with the first-pass centres */
+    int NT = ntheta_cells + 1;
+    double *Th = (double *)calloc(NT + 2, sizeof(double));
+    double *tav = (double *)calloc(NT + 2, sizeof(double));
+    int i = 0, k = NT / 2;
+    for (int j = 1; j <= k; j++) {
+        i++;
+        Th[i] = (double)((float)(j - 1) / (float)(k - 1));
+        i++;
+        Th[i] = cos(((0.5 * PI) * (double)j) / (double)(k + 1));
+    }
+    sort_asc(Th + 1, i);
+    if (i < NT)
+        for (int j = i + 1; j <= NT; j++) Th[j] = (double)((float)(j - i + 1) / (float)(NT - i + 1));
+    sort_asc(Th + 1, NT);
+    for (i = 1; i <= NT - 1; i++) tav[i] = acos((Th[NT - i + 1] + Th[NT - i]) / 2.0);
+    thet[0] = 0.0;
+    for (i = 2; i <= NT - 1; i++) thet[i - 1] = (tav[i - 1] + tav[i]) / 2.0;
+    thet[NT - 1] = PI / 2.0;
+    free(Th);
+    free(tav);
+}
+
+double orc_arrinterpol_log(double x, const double *xarr, const double *yarr, int n)
+{
+    /* the reference returns the abscissa outside the table and picks the
+     * last bracketing index of its scan; both kept */
+    if (x <= xarr[0]) return xarr[0];
+    if (x >= xarr[n - 1]) return xarr[n - 1];
+    int ix = 1;
+    for (int i = 2; i <= n; i++)
+        if (x <= xarr[i - 1]) ix = i;
+    double x1 = xarr[ix - 2], x2 = xarr[ix - 1], y1 = yarr[ix - 2], y2 = yarr[ix - 1];
+    double pw = log10(y2 / y1) / log10(x2 / x1);
+    return y1 * pow(x / x1, pw);
+}
+
+/* monochromatic lambda weights (TraceMono.f:622-647) */
+static void mono_weights(const orc_model *m, double lam0, int *l1, int *l2, double *wl1, double *wl2)
+{
+    int nlam = m->nlam;
+    if (lam0 <= m->lam[0]) { *l1 = 1; *l2 = 2; *wl1 = 1.0; *wl2 = 0.0; }
+    for (int i = 1; i <= nlam - 1; i++)
+        if (lam0 >= m->lam[i - 1] && lam0 <= m->lam[i]) {
+            *l1 = i; *l2 = i + 1;
+            *wl1 = (m->lam[i] - lam0) / (m->lam[i] - m->lam[i - 1]);
+            *wl2 = (lam0 - m->lam[i - 1]) / (m->lam[i] - m->lam[i - 1]);
+        }
+    if (lam0 >= m->lam[nlam - 1]) { *l1 = nlam - 1; *l2 = nlam; *wl1 = 0.0; *wl2 = 1.0; }
+}
+
+double orc_tau01_midplane(const orc_model *m, double lam0)
+{
+    /* tau01R (TraceMono.f:2300-2345) for the midplane cell row: a radial ray
+     * from R(1) until tau = 0.1.  The opacity is taken at lam0 (the
+     * reference's trace2d reads the module-global per-species opacity of the
+     * last emitted packet there instead). */
+    int l1, l2;
+    double wl1, wl2;
+    mono_weights(m, lam0, &l1, &l2, &wl1, &wl2);
+    int nt = m->nt, ng = m->ng, NT = nt + 1;
+    int jj = nt - 1;
+    double r = m->R[1];
+    double ct = (m->Theta[jj] + m->Theta[jj + 1]) / 2.0;
+    phot_t p;
+    memset(&p, 0, sizeof(p));
+    p.x = sqrt(1.0 - ct * ct) * r; p.y = 0.0; p.z = ct * r;
+    p.edgeNr = 1; p.onEdge = 1;
+    p.vx = p.x / r; p.vy = 0.0; p.vz = p.z / r;
+    p.i = 1; p.jj = jj;
+    double tau0 = 0.1, tau = 0.0;
+    for (int guard = 0; guard < 100000; guard++) {
+        int c = p.i * nt + p.jj;
+        double Kext = 0.0;
+        for (int s = 0; s < ng; s++) {
+            const double *Ke = m->Kext + (size_t)s * m->nlam;
+            Kext = Kext + ((Ke[l1 - 1] * wl1 + Ke[l2 - 1] * wl2) * m->w[(size_t)c * ng + s]) * 1.0;
+        }
+        double v;
+        int in, jn;
+        if (trace2edge(m, NT, &p, &v, &in, &jn) != 0) break;
+        double tau1 = ((Kext * v) * m->dens[c]) * AU_C;
+        if (tau + tau1 > tau0) {
+            v = (v * (tau0 - tau)) / tau1;
+            p.x += p.vx * v; p.y += p.vy * v; p.z += p.vz * v;
+            break;
+        }
+        p.x += p.vx * v; p.y += p.vy * v; p.z += p.vz * v;
+        p.onEdge = 1;
+        tau += tau1;
+        if (in >= m->nr || in < 0) { p.z = 0.0; break; }
+        p.i = in; p.jj = jn;
+    }
+    double R01 = sqrt(p.x * p.x + p.z * p.z);
+    if (R01 > m->R[m->nr - 1]) R01 = m->R[1];
+    return R01;
+}
+
+int orc_image_radii(const orc_model *m, const double *theta_av, const orc_obs *o, double R01, double *Rimg,
+                    int nmax)
+{
+    /* TracePath (TraceMono.f:2098-2229), nRfix = 0, no inner gas */
+    const int nstar = 30, nt = m->nt, NR = m->nr;
+    int nimg = (o->nj0 > 0) ? o->nj0 * (NR - 1) + nt * 2 * 2 + 1 + nstar
+                            : (NR - 2) / (-o->nj0) + 1 + nt * 2 * 2 + 1 + nstar;
+    if (m->scat_how == 2) nimg += 180;
+    if (nimg > nmax) return -1;
+    double *R = Rimg;
+    int nj = 0;
+    double ang = o->angle;
+    for (int j = 0; j < nt; j++) {
+        R[nj++] = R01 * fabs(cos(PI / 2.0 - theta_av[j] + ang));
+        R[nj++] = fabs(R01 * cos(theta_av[j] - PI / 2.0 + ang));
+    }
+    for (int j = 0; j < nt; j++) {
+        R[nj++] = R01 * fabs(cos(PI / 2.0 - theta_av[j]));
+        R[nj++] = fabs(R01 * cos(theta_av[j] - PI / 2.0));
+    }
+    if (m->scat_how == 2)
+        for (int j = 1; j <= 180; j++) {
+            double theta = (double)j * (PI / 2.0) / 181.0;
+            R[nj++] = m->R[1] * fabs(cos(PI / 2.0 - theta));
+        }
+    sort_asc(R, nj);
+    if (o->nj0 > 0) {
+        for (int i = 1; i <= NR - 1; i++)
+            for (int j = 1; j <= o->nj0; j++)
+                R[nj++] = m->R[i] + (m->R[i + 1] - m->R[i]) * ((double)j - 0.5) / (double)o->nj0;
+    } else {
+        for (int i = 1; i <= NR - 1; i += -o->nj0) R[nj++] = pow(10.0, log10(m->R[i] * m->R[i + 1]) / 2.0);
+    }
+    for (int i = 1; i <= nstar - 1; i++)
+        R[nj++] = ((0.999 * m->R[0]) * (double)(i - 1)) / (double)(nstar - 2);
+    R[nj++] = m->R[0] * (double)1.001f;
+    R[nimg - 1] = m->R[NR] * (double)0.9999f;
+    sort_asc(R, nimg);
+    /* separate coincident radii (TraceMono.f:2217-2229) */
+    int k = 0;
+restart:
+    k++;
+    for (int i = nimg; i >= 2; i--) {
+        if (R[i - 1] == R[i - 2]) {
+            if (i != nimg) R[i - 1] = (R[i] + R[i - 1]) / 2.0;
+            else R[i - 2] = (R[i - 2] + R[i - 1]) / 2.0;
+            if (k < nimg * 10) goto restart;
+        }
+    }
+    return nimg;
+}
+
+/* ---- SED context ------------------------------------------------------ */
+typedef struct {
+    const orc_model *m;
+    const orc_obs *o;
+    const double *thet;
+    int nr, nt, NT, ncell, ng, nlam;
+    int *nrg;        /* [nt] sub-cells of theta row j                     */
+    int *rgoff;      /* [nt] offset of row j's sub-cells                  */
+    int NRG;
+    double *thetrg;  /* [nt][MAXRG+1] cos^2 of the sub-cell edges         */
+    int MAXRG;
+    int l1, l2;
+    double wl1, wl2;
+    double *Kabs, *Kext, *Ksca, *Alb; /* [ncell] monochromatic            */
+    double *F;       /* [ncell][6][180] cell Mueller mix (TraceMono.f:706) */
+    double *MIF11, *MIF12;           /* [ncell]                          */
+    double *sf, *sQ, *sU, *sV;       /* [nr][NRG][91][2]                  */
+    double xsf[NPHIS], ysf[NPHIS], zsf[NPHIS], xin[NPHIS], yin[NPHIS], zin[NPHIS];
+    rng_t g;
+} sed_t;
+
+static size_t sf_idx(const sed_t *S, int i, int jj, int irg, int a, int side)
+{
+    return ((((size_t)i * S->NRG + S->rgoff[jj] + (irg - 1)) * 91 + a) * 2 + (side - 1));
+}
+
+static int irg_of(const sed_t *S, int jj, double theta)
+{
+    int n = S->nrg[jj];
+    if (n <= 1) return 1;
+    int irg = (int)((double)(float)n * (theta - S->thet[jj]) / (S->thet[jj + 1] - S->thet[jj])) + 1;
+    if (irg > n) irg = n;
+    if (irg < 1) irg = 1;
+    return irg;
+}
+
+/* TraceMono.f:2487-2653 */
+static int trace2edge_rg(const sed_t *S, phot_t *p, double *v, int *inext, int *jnext, int *irgnext)
+{
+    const orc_model *m = S->m;
+    const int NT = S->NT;
+    double r = p->x * p->x + p->y * p->y + p->z * p->z;
+    double R1 = m->R[p->i] * m->R[p->i];
+    double R2 = m->R[p->i + 1] * m->R[p->i + 1];
+    int j = p->jj + 1;
+    int n = S->nrg[p->jj];
+    double T1, T2;
+    if (n <= 1 || j == 0 || j == NT) {
+        T1 = m->Theta[p->jj] * m->Theta[p->jj];
+        T2 = m->Theta[p->jj + 1] * m->Theta[p->jj + 1];
+    } else {
+        T1 = S->thetrg[(size_t)p->jj * (S->MAXRG + 1) + p->irg - 1];
+        T2 = S->thetrg[(size_t)p->jj * (S->MAXRG + 1) + p->irg];
+    }
+    double b = 2.0 * (p->x * p->vx + p->y * p->vy + p->z * p->vz);
+    int h1, h2, h3, h4;
+    double vR1, vR2, vT1, vT2;
+    if (!p->onEdge) {
+        h1 = hitR(R1, r, b, &vR1); h2 = hitR(R2, r, b, &vR2);
+        h3 = hitT(p, T1, r, b, &vT1); h4 = hitT(p, T2, r, b, &vT2);
+    } else if (p->edgeNr == 1) {
+        h1 = 0; vR1 = 1e200;
+        h2 = hitR(R2, r, b, &vR2); h3 = hitT(p, T1, r, b, &vT1); h4 = hitT(p, T2, r, b, &vT2);
+    } else if (p->edgeNr == 2) {
+        h1 = hitR(R1, r, b, &vR1); h2 = 1; vR2 = -b;
+        h3 = hitT(p, T1, r, b, &vT1); h4 = hitT(p, T2, r, b, &vT2);
+    } else if (p->edgeNr == 3) {
+        h1 = hitR(R1, r, b, &vR1); h2 = hitR(R2, r, b, &vR2);
+        h3 = 0; vT1 = 1e200;
+        if (j != NT - 1 || p->irg != n) {
+            h4 = hitT(p, T2, r, b, &vT2);
+        } else {
+            vT2 = -p->z / p->vz;
+            h4 = (vT2 > 0.0);
+        }
+    } else {
+        h1 = hitR(R1, r, b, &vR1); h2 = hitR(R2, r, b, &vR2);
+        if (j != 1 || p->irg != 1) h3 = hitT(p, T1, r, b, &vT1);
+        else { h3 = 0; vT1 = 1e200; }
+        if (j != NT - 1 || p->irg != n) h4 = hitTsame(p, T2, r, b, &vT2);
+        else { h4 = 0; vT2 = 1e200; }
+    }
+    if (!h1 && !h2 && !h3 && !h4) return -1;
+    *v = 1e200;
+    int in = p->i, jn = j, rn = p->irg;
+    if (h1 && vR1 < *v) { *v = vR1; in = p->i - 1; jn = j; rn = p->irg; p->edgeNr = 2; }
+    if (h2 && vR2 < *v) { *v = vR2; in = p->i + 1; jn = j; rn = p->irg; p->edgeNr = 1; }
+    if (h3 && vT1 < *v) {
+        *v = vT1; in = p->i;
+        if (n > 1) {
+            rn = p->irg - 1; jn = j;
+            if (rn < 1) { jn = j - 1; rn = (jn >= 1) ? S->nrg[jn - 1] : 1; }
+        } else {
+            jn = j - 1;
+            if (jn > 0) rn = S->nrg[jn - 1];
+        }
+        p->edgeNr = 4;
+    }
+    if (h4 && vT2 < *v) {
+        *v = vT2; in = p->i;
+        if (n > 1) {
+            rn = p->irg + 1; jn = j;
+            if (rn > n) {
+                jn = j + 1; rn = 1;
+                if (jn == NT) rn = S->nrg[NT - 2];
+            }
+        } else {
+            jn = j + 1; rn = 1;
+        }
+        p->edgeNr = 3;
+    }
+    if (jn == NT) { jn = NT - 1; rn = S->nrg[NT - 2]; p->edgeNr = 4; }
+    if (jn == 0) { jn = 1; rn = 1; p->edgeNr = 3; }
+    *inext = in;
+    *jnext = jn - 1;
+    *irgnext = rn;
+    return 0;
+}
+
+/* TraceMono.f:1561-1580 */
+static void make_rotate_stokes(const phot_t *p, double x, double y, double z, double *sin2t, double *cos2t)
+{
+    double cost = p->Sx * x + p->Sy * y + p->Sz * z;
+    double sint = (x - (p->Sx * (p->vy * p->vy + p->vz * p->vz) - p->vx * (p->vy * p->Sy + p->vz * p->Sz)) * cost) /
+                  (p->vy * p->Sz - p->vz * p->Sy);
+    double c2 = cost * cost - sint * sint;
+    double s2 = (2.0 * sint) * cost;
+    double een = sqrt(s2 * s2 + c2 * c2);
+    *sin2t = s2 / een;
+    *cos2t = c2 / een;
+}
+
+/* per observer direction j: scattering-angle bin and the rotated photon
+ * (TraceMono.f:1309-1335 and 1646-1669) */
+typedef struct {
+    int iangle[NPHIS];
+    double E[NPHIS], Q[NPHIS], U[NPHIS], V[NPHIS], s2[NPHIS], c2[NPHIS];
+} obsdir_t;
+
+static void setup_obsdirs(const sed_t *S, const phot_t *p, int rotate_pol, obsdir_t *od)
+{
+    for (int i = 0; i < NPHIS; i++) {
+        od->iangle[i] = (int)((180.0 * acos(p->vx * S->xsf[i] + p->vy * S->ysf[i] + p->vz * S->zsf[i])) / PI + 1);
+        double xn = p->vy * S->zsf[i] - p->vz * S->ysf[i];
+        double yn = p->vz * S->xsf[i] - p->vx * S->zsf[i];
+        double zn = p->vx * S->ysf[i] - p->vy * S->xsf[i];
+        double r = sqrt(xn * xn + yn * yn + zn * zn);
+        xn /= r; yn /= r; zn /= r;
+        phot_t q = *p;
+        if (rotate_pol && p->pol) rotateStokes(&q, xn, yn, zn);
+        q.Sx = xn; q.Sy = yn; q.Sz = zn;
+        q.vx = S->xsf[i]; q.vy = S->ysf[i]; q.vz = S->zsf[i];
+        make_rotate_stokes(&q, S->xin[i], S->yin[i], S->zin[i], &od->s2[i], &od->c2[i]);
+        od->E[i] = q.E; od->Q[i] = q.Q; od->U[i] = q.U; od->V[i] = q.V;
+    }
+}
+
+/* angular part of makescattfield (TraceMono.f:1440-1509) and of
+ * MakeStarScatter (1700-1750, star=1: F11/F12 only) */
+static void add_scatt_angles(sed_t *S, const phot_t *p, double v, int irg, int side, const obsdir_t *od, int star)
+{
+    int c = p->i * S->nt + p->jj;
+    const double *F = S->F + (size_t)c * 6 * 180;
+    double x = p->x + p->vx * v / 2.0, y = p->y + p->vy * v / 2.0;
+    double phi0 = (180.0 * acos(x / sqrt(x * x + y * y))) / PI;
+    if (y < 0.0) phi0 = 360.0 - phi0;
+    double phi = 180.0 / (double)(float)NPHIS;
+    int j = (int)(((double)(float)NPHIS * (phi + phi0)) / 360.0);
+    double vAU = (v * AU_C) / 2.0;
+    int usepol = S->o->usepol;
+    for (int i = 1; i <= NPHIS; i++) {
+        if (j < 1) j = j + NPHIS;
+        if (j > NPHIS) j = j - NPHIS;
+        int ia = od->iangle[j - 1];
+        double F11 = F[ia - 1], F12 = F[180 + ia - 1], F22 = F[360 + ia - 1], F33 = F[540 + ia - 1],
+               F34 = F[720 + ia - 1], F44 = F[900 + ia - 1];
+        double E1 = od->E[j - 1], Q1 = od->Q[j - 1], U1 = od->U[j - 1], V1 = od->V[j - 1];
+        double sI, sQ = 0, sU = 0, sV = 0;
+        if (star) {
+            sI = F11 * E1;
+            if (usepol) {
+                double Qt = F12 * E1;
+                sQ = Qt * od->c2[j - 1];
+                sU = -Qt * od->s2[j - 1];
+            }
+        } else {
+            sI = F11 * E1 + F12 * Q1;
+            if (usepol) {
+                double Qt = F12 * E1 + F22 * Q1;
+                double Ut = F34 * V1 + F33 * U1;
+                sQ = Qt * od->c2[j - 1] + Ut * od->s2[j - 1];
+                sU = -Qt * od->s2[j - 1] + Ut * od->c2[j - 1];
+                sV = -F34 * U1 + F44 * V1;
+            }
+        }
+        int a = (i <= NPHIS / 2) ? i : NPHIS + 1 - i;
+        double sgn = (i <= NPHIS / 2) ? 1.0 : -1.0;
+        size_t k = sf_idx(S, p->i, p->jj, irg, a, side);
+        S->sf[k] = S->sf[k] + sI * vAU;
+        if (usepol) {
+            S->sQ[k] = S->sQ[k] + sQ * vAU;
+            S->sU[k] = S->sU[k] + sgn * sU * vAU;
+            if (!star) S->sV[k] = S->sV[k] + sgn * sV * vAU;
+        }
+        j = j + 1;
+    }
+}
+
+/* EmissionDistribution (TraceMono.f:956-1119) with tcontact, no gas/QHP;
+ * trace2exit (2659-2692) for the escape optical depth. */
+static void emission_distribution(sed_t *S, const double *Tcell, double *EmisDis, double *vismass, double *Etot,
+                                  double *Etot2, double *Estar)
+{
+    const orc_model *m = S->m;
+    const int nt = S->nt, ng = S->ng, nlam = S->nlam, NT = S->NT;
+    const int l1 = S->l1, l2 = S->l2;
+    const double wl1 = S->wl1, wl2 = S->wl2;
+    *Estar = m->Fstar[l1 - 1] * wl1 + m->Fstar[l2 - 1] * wl2;
+    *Etot = 0.0;
+    *Etot2 = 0.0;
+    for (int i = 1; i <= S->nr - 1; i++)
+        for (int jj = 0; jj < nt; jj++) {
+            int c = i * nt + jj;
+            double e = 0.0;
+            if (Tcell[c] != 0.0) {
+                int iT = (int)(Tcell[c] / 1.0);
+                double wT1 = (double)(float)(iT + 1) - Tcell[c] / 1.0;
+                double wT2 = Tcell[c] / 1.0 - (double)(float)iT;
+                if (iT < 1) iT = 1;
+                if (iT > TMAX - 1) iT = TMAX - 1;
+                for (int s = 0; s < ng; s++) {
+                    const double *K = m->Kabs + (size_t)s * nlam;
+                    const double *B0 = m->BB + (size_t)iT * nlam, *B1 = m->BB + (size_t)(iT + 1) * nlam;
+                    e = e + ((wl1 * (wT1 * B0[l1 - 1] + wT2 * B1[l1 - 1]) * K[l1 - 1] +
+                              wl2 * (wT1 * B0[l2 - 1] + wT2 * B1[l2 - 1]) * K[l2 - 1]) *
+                             m->w[(size_t)c * ng + s]) * 1.0;
+                }
+            }
+            e = (e * m->dens[c]) * m->V[c];
+            double taumin = 0.0;
+            int nex = S->o->nexits;
+            for (int k = 1; k <= abs(nex); k++) {
+                phot_t q;
+                memset(&q, 0, sizeof(q));
+                double Rad = 0.5;
+                Rad = sqrt(m->R[i] * m->R[i] * Rad + m->R[i + 1] * m->R[i + 1] * (1.0 - Rad));
+                double Th = 0.5;
+                Th = m->Theta[jj] * Th + m->Theta[jj + 1] * (1.0 - Th);
+                q.z = Rad * Th;
+                double phi = 1e-4;
+                q.x = Rad * sqrt(1.0 - Th * Th) * sin(phi);
+                q.y = Rad * sqrt(1.0 - Th * Th) * cos(phi);
+                phi = (PI * (double)(k - 1)) / (double)(nex - 1);
+                q.vx = 0.0; q.vy = cos(phi); q.vz = sin(phi);
+                q.i = i; q.jj = jj; q.onEdge = 0;
+                double tau = 0.0;
+                for (int guard = 0; guard < 1000000; guard++) {
+                    double v;
+                    int in, jn;
+                    if (trace2edge(m, NT, &q, &v, &in, &jn) != 0) break;
+                    q.onEdge = 1;
+                    q.x += q.vx * v; q.y += q.vy * v; q.z += q.vz * v;
+                    int cq = q.i * nt + q.jj;
+                    tau = tau + ((v * m->dens[cq]) * AU_C) * S->Kabs[cq];
+                    if (in >= S->nr || in <= 0) break;
+                    q.i = in; q.jj = jn;
+                }
+                if (tau < taumin || k == 1) taumin = tau;
+            }
+            double tauexit = (nex == 0) ? 0.0 : taumin;
+            vismass[c] = (nex >= 0) ? exp(-tauexit) : (1.0 - exp(-tauexit));
+            *Etot = *Etot + e;
+            e = e * vismass[c];
+            *Etot2 = *Etot2 + e;
+            EmisDis[c] = e;
+        }
+}
+
+static void star_position(sed_t *S, phot_t *p)
+{
+    const orc_model *m = S->m;
+    randomdirection(&S->g, &p->x, &p->y, &p->z);
+    p->x = m->R[0] * p->x; p->y = m->R[0] * p->y; p->z = m->R[0] * p->z;
+    randomdirection(&S->g, &p->vx, &p->vy, &p->vz);
+    double inp = p->x * p->vx + p->y * p->vy + p->z * p->vz;
+    if (inp < 0.0) { p->vx = -p->vx; p->vy = -p->vy; p->vz = -p->vz; }
+    p->edgeNr = 1;
+    p->onEdge = 1;
+}
+
+/* MakeStarScatter (TraceMono.f:1585-1773) */
+static void make_star_scatter(sed_t *S, double Estar, int nphot, uint64_t pid0)
+{
+    const orc_model *m = S->m;
+    const int nt = S->nt;
+    obsdir_t od;
+    for (int ip = 1; ip <= nphot; ip++) {
+        if (S->g.mode == 1) { S->g.pid = pid0 + (uint64_t)(ip - 1); S->g.k = 0; }
+        phot_t p;
+        memset(&p, 0, sizeof(p));
+        p.E = Estar / (double)(float)nphot;
+        p.scatt = 0;
+        star_position(S, &p);
+        double ct = fabs(p.z) / m->R[0];
+        for (int j = 1; j <= S->NT - 1; j++)
+            if (ct < m->Theta[j - 1] && ct > m->Theta[j]) { p.i = 0; p.jj = j - 1; }
+        p.irg = irg_of(S, p.jj, acos(ct));
+        int angular = (m->scat_how != 1);
+        if (angular) setup_obsdirs(S, &p, 0, &od);
+        for (int guard = 0; guard < 10000000; guard++) {
+            double v;
+            int in, jn, rn;
+            if (trace2edge_rg(S, &p, &v, &in, &jn, &rn) != 0) break;
+            int c = p.i * nt + p.jj;
+            double tau = (m->dens[c] * S->Kext[c]) * AU_C;
+            double w = ((tau * v) > 1e-5) ? (1.0 - exp(-tau * v)) / tau : v;
+            int irg = p.irg < 1 ? 1 : p.irg;
+            int side = (p.z > 0.0) ? 1 : 2;
+            if (!angular) {
+                size_t k1 = sf_idx(S, p.i, p.jj, 1, 0, 1), k2 = sf_idx(S, p.i, p.jj, 1, 0, 2);
+                S->sf[k1] = S->sf[k1] + ((p.E * w) * AU_C) / 2.0;
+                S->sf[k2] = S->sf[k2] + ((p.E * w) * AU_C) / 2.0;
+            } else {
+                size_t k0 = sf_idx(S, p.i, p.jj, irg, 0, side);
+                S->sf[k0] = S->sf[k0] + (p.E * w) * AU_C;
+                add_scatt_angles(S, &p, v, irg, side, &od, 1);
+            }
+            w = exp(-tau * v);
+            p.E = p.E * w;
+            for (int k = 0; k < NPHIS; k++) od.E[k] = od.E[k] * w;
+            p.x += p.vx * v; p.y += p.vy * v; p.z += p.vz * v;
+            p.onEdge = 1;
+            if (in >= S->nr || in < 0) break;
+            p.i = in; p.jj = jn; p.irg = rn;
+        }
+    }
+}
+
+/* emitposition (TraceMono.f:1124-1286), star and thermal emission only */
+static void emit_position(sed_t *S, phot_t *p, const double *EmisDis, double Etot2, double Estar,
+                          const double *vismass)
+{
+    const orc_model *m = S->m;
+    const int nt = S->nt;
+    for (;;) {
+        p->pol = 0;
+        double Etot = Etot2 + Estar + 0.0 + 0.0;
+        double Er = Etot * ran2(&S->g);
+        if (Er < Estar) {
+            p->E = Etot;
+            p->scatt = 0;
+            star_position(S, p);
+            double ct = fabs(p->z) / m->R[0];
+            for (int j = 1; j <= S->NT - 1; j++)
+                if (ct < m->Theta[j - 1] && ct > m->Theta[j]) {
+                    p->i = 0;
+                    p->jj = j - 1;
+                    double th = acos(fabs(p->z) / sqrt(p->x * p->x + p->y * p->y + p->z * p->z));
+                    p->irg = irg_of(S, p->jj, th);
+                    return;
+                }
+        } else {
+            p->scatt = 1;
+            Er = Er - Estar - 0.0 - 0.0;
+            double Et = 0.0;
+            for (int i = 1; i <= S->nr - 1; i++)
+                for (int jj = 0; jj < nt; jj++) {
+                    int c = i * nt + jj;
+                    Et = Et + EmisDis[c];
+                    if (Er < Et) {
+                        double Rad = ran2(&S->g);
+                        Rad = sqrt(m->R[i] * m->R[i] * Rad + m->R[i + 1] * m->R[i + 1] * (1.0 - Rad));
+                        double Th = ran2(&S->g);
+                        Th = m->Theta[jj] * Th + m->Theta[jj + 1] * (1.0 - Th);
+                        p->z = Rad * Th;
+                        if (ran2(&S->g) < 0.5) p->z = -p->z;
+                        double phi = (ran2(&S->g) * PI) * 2.0;
+                        p->x = Rad * sqrt(1.0 - Th * Th) * sin(phi);
+                        p->y = Rad * sqrt(1.0 - Th * Th) * cos(phi);
+                        randomdirection(&S->g, &p->vx, &p->vy, &p->vz);
+                        p->i = i;
+                        p->jj = jj;
+                        double th = acos(fabs(p->z) / sqrt(p->x * p->x + p->y * p->y + p->z * p->z));
+                        p->irg = irg_of(S, jj, th);
+                        p->onEdge = 0;
+                        p->E = Etot / vismass[c];
+                        return;
+                    }
+                }
+        }
+    }
+}
+
+/* trace2dmono (TraceMono.f:1291-1411) with makescattfield (1416-1556) */
+static void trace2dmono(sed_t *S, phot_t *p, double tau0, int *escape, int *hitstar)
+{
+    const orc_model *m = S->m;
+    const int nt = S->nt;
+    *escape = 0;
+    *hitstar = 0;
+    double tau = 0.0;
+    obsdir_t od;
+    int angular = (m->scat_how != 1);
+    if (angular && p->scatt) setup_obsdirs(S, p, 1, &od);
+    int maxvisit = (S->nr + S->NT + 360) * 2, nvisit = 0;
+    for (;;) {
+        nvisit++;
+        if (nvisit > maxvisit) { *escape = 1; return; }
+        double v;
+        int in, jn, rn;
+        if (trace2edge_rg(S, p, &v, &in, &jn, &rn) != 0) { *escape = 1; return; }
+        int c = p->i * nt + p->jj;
+        double tau1 = ((v * m->dens[c]) * S->Kext[c]) * AU_C;
+        int inside = (tau + tau1) > tau0;
+        if (inside) v = (v * (tau0 - tau)) / tau1;
+        if (p->scatt) {
+            if (!angular) {
+                size_t k1 = sf_idx(S, p->i, p->jj, 1, 0, 1), k2 = sf_idx(S, p->i, p->jj, 1, 0, 2);
+                S->sf[k1] = S->sf[k1] + ((p->E * v) * AU_C) / 2.0;
+                S->sf[k2] = S->sf[k2] + ((p->E * v) * AU_C) / 2.0;
+            } else {
+                int irg = p->irg < 1 ? 1 : p->irg;
+                int side = (p->z > 0.0) ? 1 : 2;
+                size_t k0 = sf_idx(S, p->i, p->jj, irg, 0, side);
+                S->sf[k0] = S->sf[k0] + (p->E * v) * AU_C;
+                add_scatt_angles(S, p, v, irg, side, &od, 0);
+            }
+        }
+        p->x += p->vx * v; p->y += p->vy * v; p->z += p->vz * v;
+        if (inside) { p->onEdge = 0; return; }
+        p->onEdge = 1;
+        tau = tau + tau1;
+        if (in >= S->nr) { *escape = 1; return; }
+        if (in < 0) { *escape = 1; *hitstar = 1; return; }
+        p->i = in; p->jj = jn; p->irg = rn;
+    }
+}
+
+typedef struct {
+    int n, cap, hitstar;
+    int *i, *jj, *k, *irg, *jphi1, *jphi2;
+    double *v;
+} path_t;
+
+static int jphi_of(double x, double y)
+{
+    double phi = (180.0 * acos(x / sqrt(x * x + y * y))) / PI;
+    if (y < 0.0) phi = -phi;
+    int jp = (int)(((double)(float)NPHIS * phi) / 360.0);
+    jp = jp + 1;
+    if (jp < 1) jp = 1 - jp;
+    if (jp > NPHIS / 2) jp = NPHIS + 1 - jp;
+    return jp;
+}
+
+static void path_push(path_t *P, int i, int jj, int k, int irg, double v, int j1, int j2)
+{
+    if (P->n == P->cap) {
+        P->cap = P->cap ? 2 * P->cap : 64;
+        P->i = realloc(P->i, sizeof(int) * P->cap); P->jj = realloc(P->jj, sizeof(int) * P->cap);
+        P->k = realloc(P->k, sizeof(int) * P->cap); P->irg = realloc(P->irg, sizeof(int) * P->cap);
+        P->jphi1 = realloc(P->jphi1, sizeof(int) * P->cap); P->jphi2 = realloc(P->jphi2, sizeof(int) * P->cap);
+        P->v = realloc(P->v, sizeof(double) * P->cap);
+    }
+    P->i[P->n] = i; P->jj[P->n] = jj; P->k[P->n] = k; P->irg[P->n] = irg; P->v[P->n] = v;
+    P->jphi1[P->n] = j1; P->jphi2[P->n] = j2;
+    P->n++;
+}
+
+/* the image ray through (R, phi) at inclination angle: TracePath start
+ * (TraceMono.f:2241-2270) and trace2dpath (2350-2430) */
+static int image_path(const sed_t *S, double Rad, double phi, path_t *P)
+{
+    const orc_model *m = S->m;
+    phot_t p;
+    memset(&p, 0, sizeof(p));
+    double Rout = m->R[S->nr];
+    p.x = Rad * cos(phi); p.y = Rad * sin(phi); p.z = sqrt(Rout * Rout - Rad * Rad);
+    p.edgeNr = 2; p.onEdge = 1;
+    orc_rotate(&p.x, &p.y, &p.z, 0.0, 1.0, 0.0, S->o->angle);
+    double ct = fabs(p.z) / Rout;
+    p.i = S->nr - 1;
+    for (int j = 1; j <= S->NT - 1; j++)
+        if (ct < m->Theta[j - 1] && ct >= m->Theta[j]) p.jj = j - 1;
+    p.irg = irg_of(S, p.jj, acos(ct));
+    p.vx = 0.0; p.vy = 0.0; p.vz = -1.0;
+    orc_rotate(&p.vx, &p.vy, &p.vz, 0.0, 1.0, 0.0, S->o->angle);
+    P->n = 0;
+    P->hitstar = 0;
+    for (int guard = 0; guard < 1000000; guard++) {
+        double v;
+        int in, jn, rn;
+        if (trace2edge_rg(S, &p, &v, &in, &jn, &rn) != 0) return -1;
+        p.onEdge = 1;
+        int j1 = jphi_of(p.x, p.y);
+        double x = p.x + p.vx * v, y = p.y + p.vy * v, z = p.z + p.vz * v;
+        int k = (z > 0.0) ? 1 : 2;
+        int j2 = jphi_of(x, y);
+        path_push(P, p.i, p.jj, k, p.irg, v, j1, j2);
+        p.x += p.vx * v; p.y += p.vy * v; p.z += p.vz * v;
+        if (in >= S->nr) return 0;
+        if (in < 0) { P->hitstar = 1; return 0; }
+        p.i = in; p.jj = jn; p.irg = rn;
+    }
+    return -1;
+}
+
+int orc_trace_flux(const orc_model *m, const double *thet, const double *Tcell, const orc_obs *o,
+                   const double *Rimg, int nimg, double lam0, int rng, uint64_t seed, orc_ran1_state *rs,
+                   double *flux_out, double *scatflux_out, double *fluxQ_out)
+{
+    sed_t S;
+    memset(&S, 0, sizeof(S));
+    S.m = m; S.o = o; S.thet = thet;
+    S.nr = m->nr; S.nt = m->nt; S.NT = m->nt + 1; S.ncell = m->nr * m->nt; S.ng = m->ng; S.nlam = m->nlam;
+    S.g.mode = rng; S.g.r1 = rs; S.g.seed = seed;
+    const int nt = S.nt, ng = S.ng, nlam = S.nlam, ncell = S.ncell;
+    int scattering = 0;
+    if (m->scat_how != 0)
+        for (int k = 0; k < ng * nlam; k++)
+            if (m->Ksca[k] != 0.0) scattering = 1;
+    int nmc = o->tracescat ? o->nphot : 0, nmcs = o->tracescat ? o->nphotstar : 0;
+    /* sub-cells (Telescope.f:93-127) */
+    S.nrg = (int *)calloc(nt, sizeof(int));
+    S.rgoff = (int *)calloc(nt, sizeof(int));
+    S.MAXRG = 1;
+    for (int jj = 0; jj < nt; jj++) {
+        int n = 1;
+        if (scattering && (nmc + nmcs) != 0 && m->scat_how == 2) {
+            int q = (int)((thet[jj + 1] - thet[jj]) * 270.0 / PI);
+            n = (q > 1) ? q : 1;
+        }
+        S.nrg[jj] = n;
+        S.rgoff[jj] = S.NRG;
+        S.NRG += n;
+        if (n > S.MAXRG) S.MAXRG = n;
+    }
+    S.thetrg = (double *)calloc((size_t)nt * (S.MAXRG + 1), sizeof(double));
+    for (int jj = 0; jj < nt; jj++)
+        for (int k = 1; k <= S.nrg[jj] + 1; k++) {
+            double c = cos(thet[jj] + ((double)(float)(k - 1) * (thet[jj + 1] - thet[jj])) / (double)(float)S.nrg[jj]);
+            S.thetrg[(size_t)jj * (S.MAXRG + 1) + k - 1] = c * c;
+        }
+    /* observer directions (TracePath, TraceMono.f:2057-2078) */
+    for (int i = 1; i <= NPHIS; i++) {
+        double phi = PI * ((double)i - 0.5) / ((double)(float)NPHIS / 2.0);
+        double x = 0.0, y = 0.0, z = 1.0;
+        orc_rotate(&x, &y, &z, 0.0, 1.0, 0.0, o->angle);
+        orc_rotate(&x, &y, &z, 0.0, 0.0, 1.0, phi);
+        double r = sqrt(x * x + y * y + z * z);
+        S.xsf[i - 1] = x / r; S.ysf[i - 1] = y / r; S.zsf[i - 1] = z / r;
+        x = 0.0; y = 1.0; z = 0.0;
+        orc_rotate(&x, &y, &z, 0.0, 0.0, 1.0, phi);
+        r = sqrt(x * x + y * y + z * z);
+        S.xin[i - 1] = x / r; S.yin[i - 1] = y / r; S.zin[i - 1] = z / r;
+    }
+    mono_weights(m, lam0, &S.l1, &S.l2, &S.wl1, &S.wl2);
+    const int l1 = S.l1, l2 = S.l2;
+    const double wl1 = S.wl1, wl2 = S.wl2;
+    /* monochromatic cells (TraceMono.f:649-754) */
+    S.Kabs = calloc(ncell, sizeof(double)); S.Kext = calloc(ncell, sizeof(double));
+    S.Ksca = calloc(ncell, sizeof(double)); S.Alb = calloc(ncell, sizeof(double));
+    S.F = calloc((size_t)ncell * 6 * 180, sizeof(double));
+    S.MIF11 = calloc(ncell, sizeof(double)); S.MIF12 = calloc(ncell, sizeof(double));
+    for (int c = 0; c < ncell; c++) {
+        double ka = 0, ke = 0, ks = 0;
+        for (int s = 0; s < ng; s++) {
+            size_t o1 = (size_t)s * nlam + l1 - 1, o2 = (size_t)s * nlam + l2 - 1;
+            double w = m->w[(size_t)c * ng + s];
+            ka = ka + ((wl1 * m->Kabs[o1] + wl2 * m->Kabs[o2]) * w) * 1.0;
+            ke = ke + ((wl1 * m->Kext[o1] + wl2 * m->Kext[o2]) * w) * 1.0;
+            ks = ks + ((wl1 * m->Ksca[o1] + wl2 * m->Ksca[o2]) * w) * 1.0;
+        }
+        S.Kabs[c] = ka; S.Kext[c] = ke; S.Ksca[c] = ks; S.Alb[c] = ks / ke;
+        if (scattering && m->scat_how == 2) {
+            double *F = S.F + (size_t)c * 6 * 180;
+            if (ks != 0.0) {
+                double i11 = 0, i12 = 0;
+                for (int s = 0; s < ng; s++) {
+                    size_t o1 = (size_t)s * nlam + l1 - 1, o2 = (size_t)s * nlam + l2 - 1;
+                    double w = m->w[(size_t)c * ng + s];
+                    double ml11 = (m->IF11[o1] * m->Ksca[o1]) * wl1 + (m->IF11[o2] * m->Ksca[o2]) * wl2;
+                    double ml12 = (m->IF12[o1] * m->Ksca[o1]) * wl1 + (m->IF12[o2] * m->Ksca[o2]) * wl2;
+                    i11 = i11 + (ml11 * w) * 1.0;
+                    i12 = i12 + (ml12 * w) * 1.0;
+                }
+                S.MIF11[c] = i11 / ks;
+                S.MIF12[c] = i12 / ks;
+                int nel = o->usepol ? 6 : 2;
+                for (int k = 0; k < nel; k++)
+                    for (int a = 0; a < 180; a++) {
+                        double acc = 0.0;
+                        for (int s = 0; s < ng; s++) {
+                            size_t o1 = (size_t)s * nlam + l1 - 1, o2 = (size_t)s * nlam + l2 - 1;
+                            double w = m->w[(size_t)c * ng + s];
+                            double ml = (m->F[(o1 * 6 + k) * 180 + a] * m->Ksca[o1]) * wl1 +
+                                        (m->F[(o2 * 6 + k) * 180 + a] * m->Ksca[o2]) * wl2;
+                            acc = acc + (ml * w) * 1.0;
+                        }
+                        F[k * 180 + a] = acc / ks;
+                    }
+            } else {
+                for (int a = 0; a < 180; a++) F[a] = 1.0;
+            }
+        }
+    }
+    /* scattered field */
+    size_t nsf = (size_t)S.nr * S.NRG * 91 * 2;
+    S.sf = calloc(nsf, sizeof(double)); S.sQ = calloc(nsf, sizeof(double));
+    S.sU = calloc(nsf, sizeof(double)); S.sV = calloc(nsf, sizeof(double));
+    if (scattering && (nmc + nmcs) != 0) {
+        double *EmisDis = calloc(ncell, sizeof(double)), *vismass = calloc(ncell, sizeof(double));
+        double Etot, Etot2, Estar;
+        emission_distribution(&S, Tcell, EmisDis, vismass, &Etot, &Etot2, &Estar);
+        if (nmcs > 0) make_star_scatter(&S, Estar, nmcs, 0);
+        for (int ip = 1; ip <= nmc; ip++) {
+            if (S.g.mode == 1) { S.g.pid = (uint64_t)nmcs + (uint64_t)(ip - 1); S.g.k = 0; }
+            phot_t p;
+            memset(&p, 0, sizeof(p));
+            emit_position(&S, &p, EmisDis, Etot2, Estar, vismass);
+            p.E = p.E / (double)(float)nmc;
+            if (m->scat_how == 2) {
+                p.pol = 0; p.Q = 0.0; p.U = 0.0; p.V = 0.0;
+                double x, y, z;
+                if (p.vz != 1.0) { x = -p.vy; y = p.vx; z = 0.0; }
+                else { x = 1.0; y = 0.0; z = 0.0; }
+                double r = sqrt(x * x + y * y + z * z);
+                p.Sx = x / r; p.Sy = y / r; p.Sz = z / r;
+            }
+            for (int guard = 0; guard < 100000000; guard++) {
+                double tau = -log(ran2(&S.g));
+                int escape, hitstar;
+                trace2dmono(&S, &p, tau, &escape, &hitstar);
+                if (hitstar || escape) break;
+                int c = p.i * nt + p.jj;
+                double fstop = pow(S.Alb[c], o->pfstop);
+                double fact = S.Alb[c] / fstop;
+                p.E = p.E * fact;
+                if (m->scat_how == 2) { p.Q *= fact; p.U *= fact; p.V *= fact; }
+                if (ran2(&S.g) > fstop) break;
+                p.scatt = 1;
+                if (m->scat_how == 1) {
+                    randomdirection(&S.g, &p.vx, &p.vy, &p.vz);
+                } else {
+                    ctx_t C;
+                    memset(&C, 0, sizeof(C));
+                    C.m = m;
+                    C.g = S.g;
+                    scatangle(&C, &p, S.F + (size_t)c * 6 * 180, S.MIF11[c], S.MIF12[c]);
+                    S.g = C.g;
+                }
+            }
+        }
+        free(EmisDis);
+        free(vismass);
+    }
+    /* source function (TraceMono.f:229-305, tcontact, no gas) */
+    double *emis = calloc(ncell, sizeof(double));
+    for (int i = 0; i <= S.nr - 1; i++)
+        for (int jj = 0; jj < nt; jj++) {
+            int c = i * nt + jj;
+            double e = 0.0;
+            if (o->traceemis) {
+                double T = Tcell[c];
+                int iT = (int)(T / 1.0);
+                double wT1 = (double)(float)(iT + 1) - T / 1.0;
+                double wT2 = T / 1.0 - (double)(float)iT;
+                if (iT < 1) iT = 1;
+                if (iT > TMAX - 1) iT = TMAX - 1;
+                for (int s = 0; s < ng; s++) {
+                    const double *K = m->Kabs + (size_t)s * nlam;
+                    const double *B0 = m->BB + (size_t)iT * nlam, *B1 = m->BB + (size_t)(iT + 1) * nlam;
+                    e = e + (((wl1 * (wT1 * B0[l1 - 1] + wT2 * B1[l1 - 1]) * K[l1 - 1] +
+                               wl2 * (wT1 * B0[l2 - 1] + wT2 * B1[l2 - 1]) * K[l2 - 1]) *
+                              m->w[(size_t)c * ng + s]) * 1.0) / S.Kabs[c];
+                }
+                e = e * (1.0 - S.Alb[c]);
+            }
+            emis[c] = e;
+        }
+    double *fracirg = calloc((size_t)nt * (S.MAXRG + 1), sizeof(double));
+    for (int jj = 0; jj < nt; jj++)
+        for (int i = 1; i <= S.nrg[jj]; i++)
+            fracirg[(size_t)jj * (S.MAXRG + 1) + i - 1] =
+                (cos(thet[jj] + ((double)(float)(i - 1) * (thet[jj + 1] - thet[jj])) / (double)(float)S.nrg[jj]) -
+                 cos(thet[jj] + ((double)(float)i * (thet[jj + 1] - thet[jj])) / (double)(float)S.nrg[jj])) /
+                (m->Theta[jj] - m->Theta[jj + 1]);
+    /* formal solution along the image rays (TraceMono.f:332-493) */
+    const int nphi = o->nphi;
+    double *image = calloc((size_t)nimg * nphi, sizeof(double));
+    double *scatim = calloc((size_t)nimg * nphi, sizeof(double));
+    double *imageQ = calloc((size_t)nimg * nphi, sizeof(double));
+    path_t P;
+    memset(&P, 0, sizeof(P));
+    const double tau_max = 1e8;
+    int err = 0;
+    for (int ir = 0; ir < nimg && !err; ir++)
+        for (int k = 1; k <= nphi; k++) {
+            double phik = 1e-5 + PI * ((double)((float)k - 0.5f)) / (double)nphi;
+            if (image_path(&S, Rimg[ir], phik, &P) != 0) { err = -1; break; }
+            double img = 0.0, sim = 0.0, imQ = 0.0, tau = 0.0, fact = 1.0;
+            int stopped = 0;
+            for (int s = 0; s < P.n; s++) {
+                int ip = P.i[s], jp = P.jj[s], kp = P.k[s], irg = P.irg[s];
+                int c = ip * nt + jp;
+                double x_scat = 0.0, x_scatQ = 0.0;
+                if (scattering && (nmc + nmcs) != 0) {
+                    int jj1 = P.jphi1[s], jj2 = P.jphi2[s];
+                    int djj = (jj1 <= jj2) ? 1 : -1, njj = 0, jj = jj1;
+                    for (;;) {
+                        double ww = S.Alb[c] / (m->V[c] * fracirg[(size_t)jp * (S.MAXRG + 1) + irg - 1]);
+                        x_scat = x_scat + ww * S.sf[sf_idx(&S, ip, jp, irg, jj, kp)];
+                        if (m->scat_how == 2) x_scatQ = x_scatQ + ww * S.sQ[sf_idx(&S, ip, jp, irg, jj, kp)];
+                        njj++;
+                        if (jj == jj2) break;
+                        jj = jj + djj;
+                    }
+                    x_scat = x_scat / (double)(float)njj;
+                    x_scatQ = x_scatQ / (double)(float)njj;
+                }
+                double tau_e = ((P.v[s] * m->dens[c]) * S.Kext[c]) * AU_C;
+                if (ip != 0) {
+                    if (tau_e < 1e-6) {
+                        img = img + ((2.0 * x_scat + emis[c]) * tau_e) * fact;
+                        imQ = imQ + ((2.0 * x_scatQ) * tau_e) * fact;
+                        sim = sim + ((2.0 * x_scat) * tau_e) * fact;
+                        fact = fact * (1.0 - tau_e);
+                    } else {
+                        double ex = exp(-tau_e);
+                        double frac = 1.0 - ex;
+                        img = img + ((2.0 * x_scat + emis[c]) * frac) * fact;
+                        imQ = imQ + ((2.0 * x_scatQ) * frac) * fact;
+                        sim = sim + ((2.0 * x_scat) * frac) * fact;
+                        fact = fact * ex;
+                    }
+                    tau = tau + tau_e;
+                }
+                if (tau > tau_max) { stopped = 1; break; }
+            }
+            if (!stopped && P.hitstar && o->tracestar)
+                img = img + (((m->Fstar[l1 - 1] * wl1 + m->Fstar[l2 - 1] * wl2) * fact) * 1.0) /
+                                (PI * ((m->R[0] * AU_C) * (m->R[0] * AU_C)));
+            image[(size_t)ir * nphi + k - 1] = img;
+            scatim[(size_t)ir * nphi + k - 1] = sim;
+            imageQ[(size_t)ir * nphi + k - 1] = imQ;
+        }
+    /* flux quadrature (TraceMono.f:499-520), mask = 1 */
+    double flux = 0.0, scatflux = 0.0, fluxQ = 0.0;
+    for (int i = 1; i <= nimg - 1; i++)
+        for (int k = 1; k <= nphi; k++) {
+            double w1 = (((2.0 * PI) * fabs(Rimg[i - 1])) * (AU_C * AU_C)) / (double)nphi;
+            double w2 = (((2.0 * PI) * fabs(Rimg[i])) * (AU_C * AU_C)) / (double)nphi;
+            size_t a = (size_t)(i - 1) * nphi + k - 1, b = (size_t)i * nphi + k - 1;
+            flux = flux + ((Rimg[i] - Rimg[i - 1]) * (w1 * image[a] + w2 * image[b])) / 2.0;
+            scatflux = scatflux + ((Rimg[i] - Rimg[i - 1]) * (w1 * scatim[a] + w2 * scatim[b])) / 2.0;
+            if (m->scat_how == 2) fluxQ = fluxQ + ((Rimg[i] - Rimg[i - 1]) * (w1 * imageQ[a] + w2 * imageQ[b])) / 2.0;
+        }
+    *flux_out = flux;
+    *scatflux_out = scatflux;
+    *fluxQ_out = fluxQ;
+    free(P.i); free(P.jj); free(P.k); free(P.irg); free(P.jphi1); free(P.jphi2); free(P.v);
+    free(image); free(scatim); free(imageQ); free(fracirg); free(emis);
+    free(S.sf); free(S.sQ); free(S.sU); free(S.sV);
+    free(S.Kabs); free(S.Kext); free(S.Ksca); free(S.Alb); free(S.F); free(S.MIF11); free(S.MIF12);
+    free(S.nrg); free(S.rgoff); free(S.thetrg);
+    return err;
 }

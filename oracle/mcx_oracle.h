@@ -55,7 +55,19 @@ typedef struct {
     const double *w;              /* [nr*nt*ng] species weights           */
     int scat_how;                 /* 0 none, 1 isotropic, 2 full Mueller  */
     const double *cos2phi, *sin2phi; /* [361] (Init.f:3777-3781)          */
+    /* modified random walk (RandomWalk.f; keyword randomwalk, factRW)   */
+    int rndw;
+    double factRW;
+    const double *gsym;           /* [ng*nlam] asymmetry g (Init.f:4037)  */
+    const double *thet;           /* [nt+1] theta of the cone edges       */
 } orc_model;
+
+/* DiffCoeffCell (Dtable.f:97-182): Rosseland-type extinction and Planck-type
+ * absorption of cell c at integer temperature iT; opacity_set selects the
+ * per-cell KabsTot/KscaTot path (inside MCRadiation) or the species sums. */
+void orc_diffcoeff(const orc_model *m, int c, int iT, int opacity_set, double *KDext, double *KDabs);
+/* InitRandomWalk (RandomWalk.f:152-194): y/phi tables and the thick flags */
+void orc_init_random_walk(const orc_model *m, double *y, double *phi, int32_t *thick);
 
 typedef struct {
     int64_t nphot;
@@ -118,6 +130,33 @@ void orc_temperature(const orc_model *m, const double *EJv, const double *EJv2,
                      const int32_t *Ni, const double *Tbw,
                      double *T, double *dT, double *dEJv);
 double orc_determineT(const orc_model *m, int c, double E1, double Tcell);
+
+/* ---- SED rays (Telescope.f:138-164, TraceMono.f) --------------------- */
+typedef struct {
+    double angle;                /* inclination [rad] (Telescope.f:27)     */
+    int nphi, nj0;               /* tel%nphi, tel%nr (defaults 45, 2)      */
+    int nphot, nphotstar;        /* tel%Nphot, tel%NphotAngle (1000, 1000) */
+    int nexits;                  /* default 5                              */
+    int usepol;                  /* tel%usepol (default .true.)            */
+    int tracestar, traceemis, tracescat;
+    double pfstop;               /* Init.f:378 default 0.25                */
+} orc_obs;
+
+/* D%thet (theta of the cone edges, Init.f:2877-2885), [nt+1] */
+void orc_theta_edges(int ntheta_cells, double *thet);
+/* radius where tau(0.55um) = 0.1 along the midplane cell (tau01R) */
+double orc_tau01_midplane(const orc_model *m, double lam0);
+/* image radii of TracePath (TraceMono.f:2098-2229); returns their number */
+int orc_image_radii(const orc_model *m, const double *theta_av, const orc_obs *o, double R01,
+                    double *Rimg, int nmax);
+/* One wavelength of the SED (TraceFlux, TraceMono.f:8-578): MC scattered
+ * field (TraceMono/MakeStarScatter), source function and formal solution
+ * along the image rays, flux quadrature.  Tcell: final temperatures.     */
+int orc_trace_flux(const orc_model *m, const double *thet, const double *Tcell, const orc_obs *o,
+                   const double *Rimg, int nimg, double lam0, int rng, uint64_t seed, orc_ran1_state *rs,
+                   double *flux, double *scatflux, double *fluxQ);
+/* arrinterpol(x, xarr, yarr, n, log=1) (Subroutines.f:2506-2532) */
+double orc_arrinterpol_log(double x, const double *xarr, const double *yarr, int n);
 
 /* Straight-ray geometry through the grid (trace2d with tau0 = inf):
  * records up to maxseg (cell, length) pairs; returns number of segments,

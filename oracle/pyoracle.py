@@ -25,7 +25,7 @@ class OrcModel(C.Structure):
         ("Kp", _dp), ("Fstar", _dp), ("Lstar", C.c_double), ("Rstar_cm", C.c_double),
         ("Tstar", C.c_double), ("nr", C.c_int), ("nt", C.c_int), ("R", _dp), ("Theta", _dp),
         ("dens", _dp), ("V", _dp), ("w", _dp), ("scat_how", C.c_int), ("cos2phi", _dp),
-        ("sin2phi", _dp),
+        ("sin2phi", _dp), ("rndw", C.c_int), ("factRW", C.c_double), ("gsym", _dp), ("thet", _dp),
     ]
 
 
@@ -46,6 +46,21 @@ class OrcTally(C.Structure):
 
 class Ran1State(C.Structure):
     _fields_ = [("idum", C.c_int32), ("iy", C.c_int32), ("iv", C.c_int32 * 32)]
+
+
+class OrcObs(C.Structure):
+    _fields_ = [("angle", C.c_double), ("nphi", C.c_int), ("nj0", C.c_int), ("nphot", C.c_int),
+                ("nphotstar", C.c_int), ("nexits", C.c_int), ("usepol", C.c_int), ("tracestar", C.c_int),
+                ("traceemis", C.c_int), ("tracescat", C.c_int), ("pfstop", C.c_double)]
+
+
+def make_obs(angle_deg=45.0, nphi=45, nj0=2, nphot=1000, nphotstar=1000, nexits=5, usepol=True,
+             tracestar=True, traceemis=True, tracescat=True, pfstop=0.25):
+    """Observation defaults of determineoutput (Telescope.f:590-657)."""
+    import math
+    return OrcObs(angle=math.pi * angle_deg / 180.0, nphi=nphi, nj0=nj0, nphot=nphot, nphotstar=nphotstar,
+                  nexits=nexits, usepol=int(usepol), tracestar=int(tracestar), traceemis=int(traceemis),
+                  tracescat=int(tracescat), pfstop=pfstop)
 
 
 def build(force=False):
@@ -97,6 +112,18 @@ def load():
     lib.orc_trace_ray.argtypes = [C.POINTER(OrcModel)] + [C.c_double] * 6 + [C.c_int] * 5 + [_ip, _dp,
                                                                                              C.POINTER(C.c_int)]
     lib.orc_trace_ray.restype = C.c_int
+    lib.orc_theta_edges.argtypes = [C.c_int, _dp]
+    lib.orc_tau01_midplane.argtypes = [C.POINTER(OrcModel), C.c_double]
+    lib.orc_tau01_midplane.restype = C.c_double
+    lib.orc_image_radii.argtypes = [C.POINTER(OrcModel), _dp, C.POINTER(OrcObs), C.c_double, _dp, C.c_int]
+    lib.orc_image_radii.restype = C.c_int
+    lib.orc_trace_flux.argtypes = [C.POINTER(OrcModel), _dp, _dp, C.POINTER(OrcObs), _dp, C.c_int, C.c_double,
+                                   C.c_int, C.c_uint64, C.POINTER(Ran1State), _dp, _dp, _dp]
+    lib.orc_trace_flux.restype = C.c_int
+    lib.orc_arrinterpol_log.argtypes = [C.c_double, _dp, _dp, C.c_int]
+    lib.orc_arrinterpol_log.restype = C.c_double
+    lib.orc_diffcoeff.argtypes = [C.POINTER(OrcModel), C.c_int, C.c_int, C.c_int, _dp, _dp]
+    lib.orc_init_random_walk.argtypes = [C.POINTER(OrcModel), _dp, _dp, _ip]
     _lib = lib
     return lib
 
@@ -117,14 +144,16 @@ class Oracle:
         self.keep = dict(lam=c(m.lam), nu=c(m.nu), BB=c(m.BB), Kabs=c(m.Kabs), Ksca=c(m.Ksca),
                          Kext=c(m.Kext), F=c(m.F), IF11=c(m.IF11), IF12=c(m.IF12), Kp=c(m.Kp),
                          Fstar=c(m.Fstar), R=c(m.R), Th=c(m.Theta), dens=c(m.dens), V=c(m.V), w=c(m.w),
-                         c2=c(m.cos2phi), s2=c(m.sin2phi))
+                         c2=c(m.cos2phi), s2=c(m.sin2phi), gsym=c(m.gsym), thet=c(m.thet))
         k = self.keep
         self.om = OrcModel(nlam=m.nlam, lam=p(k["lam"]), nu=p(k["nu"]), BB=p(k["BB"]), ng=m.ng,
                            Kabs=p(k["Kabs"]), Ksca=p(k["Ksca"]), Kext=p(k["Kext"]), F=p(k["F"]),
                            IF11=p(k["IF11"]), IF12=p(k["IF12"]), Kp=p(k["Kp"]), Fstar=p(k["Fstar"]),
                            Lstar=m.Lstar, Rstar_cm=m.Rstar_cm, Tstar=m.Tstar, nr=m.nr, nt=m.nt,
                            R=p(k["R"]), Theta=p(k["Th"]), dens=p(k["dens"]), V=p(k["V"]), w=p(k["w"]),
-                           scat_how=m.scat_how, cos2phi=p(k["c2"]), sin2phi=p(k["s2"]))
+                           scat_how=m.scat_how, cos2phi=p(k["c2"]), sin2phi=p(k["s2"]),
+                           rndw=int(m.randomwalk), factRW=float(m.factRW), gsym=p(k["gsym"]),
+                           thet=p(k["thet"]))
         self.ran1 = Ran1State(idum=-42, iy=0)
 
     def mcradiation(self, nphot, rng="philox", seed=1, maxinteract=0):
@@ -161,6 +190,36 @@ class Oracle:
         Tbw = np.ascontiguousarray(tal["Tbw"].ravel(), dtype=np.float64)
         self.lib.orc_temperature(C.byref(self.om), p(EJv), p(EJv2), p(Ni), p(Tbw), p(T), p(dT), p(dE))
         return T.reshape(m.nr, m.nt), dT.reshape(m.nr, m.nt), dE.reshape(m.nr, m.nt)
+
+    def image_radii(self, obs, lam_paths=0.55):
+        m = self.m
+        R01 = self.lib.orc_tau01_midplane(C.byref(self.om), lam_paths)
+        nmax = 4 * (m.nr + 4 * m.nt + 400) + 10 * max(obs.nj0, 1) * m.nr
+        R = np.zeros(nmax)
+        tav = np.ascontiguousarray(m.theta_av, dtype=np.float64)
+        n = self.lib.orc_image_radii(C.byref(self.om), p(tav), C.byref(obs), R01, p(R), nmax)
+        if n < 0:
+            raise RuntimeError("image radii buffer too small")
+        return R[:n].copy(), R01
+
+    def sed(self, Tcell, obs, lams, rng="philox", seed=1):
+        """spectrum(lam) of Observe's SPECTRUM branch (Telescope.f:138-164):
+        returns flux, scatflux, fluxQ arrays (erg/s/Hz/sr-type units before
+        the 1e23/d^2 conversion)."""
+        m = self.m
+        Rimg, _ = self.image_radii(obs)
+        thet = np.ascontiguousarray(m.thet, dtype=np.float64)
+        T = np.ascontiguousarray(np.asarray(Tcell).ravel(), dtype=np.float64)
+        out = np.zeros((len(lams), 3))
+        for k, lam0 in enumerate(lams):
+            f, s, q = C.c_double(), C.c_double(), C.c_double()
+            rc = self.lib.orc_trace_flux(C.byref(self.om), p(thet), p(T), C.byref(obs), p(Rimg), len(Rimg),
+                                         float(lam0), 0 if rng == "ran1" else 1, int(seed) + k,
+                                         C.byref(self.ran1), C.byref(f), C.byref(s), C.byref(q))
+            if rc != 0:
+                raise RuntimeError(f"oracle SED failed at {lam0}: {rc}")
+            out[k] = (f.value, s.value, q.value)
+        return out
 
     def trace_ray(self, x, y, z, vx, vy, vz, i, jj, onEdge=0, edgeNr=0, maxseg=100000):
         cells = np.zeros(maxseg, dtype=np.int32)
