@@ -3,7 +3,9 @@
 A step is one Monte Carlo pass (MCRadiation.f:6) of `--nphot` packets over the
 production disk model (200 r x 100 theta cells, 200 wavelengths, dust mass
 1e-3 Msun) followed by the Lucy temperature inversion (Main.f:498-549), with all
-tables resident in HBM before the timed region.  Prints one JSON line.
+tables resident in HBM before the timed region.  The pass uses the modified
+random walk in optically thick cells (keyword randomwalk, factRW = 10), as the
+reference's example disks do; `--plain` switches it off.  Prints one JSON line.
 """
 import argparse
 import json
@@ -24,11 +26,15 @@ def algorithmic_bytes(t, ng, nlam):
     """Bytes the transport must move per launch, from on-device event counts:
     per cell step rho, w(ng), EJv add, EJv2 add, Ni bit; per absorption the
     Eabs add, Kp reads and two CDF searches; per scattering the Mueller pick
-    and the angle search; 8 B per escape."""
+    and the angle search; per random-walk step the Eabs and EJv adds, the cell
+    T load and update, the (KDext, KDabs) pair, rho, two Kp entries and the
+    survival-table search; 8 B per escape."""
     Bx = 8 + 8 * ng + 8 + 8 + 4
     Ba = 8 + 24 * ng + 16 * ng * math.ceil(math.log2(nlam)) + 16
     Bs = 8 * ng + 8 * math.ceil(math.log2(180))
-    return t["nsteps"] * Bx + t["nabs"] * Ba + t["nscat"] * Bs + 8 * t["nescape"], (Bx, Ba, Bs)
+    Brw = 8 + 8 + 16 + 16 + 8 + 16 + 8 * math.ceil(math.log2(1000)) + 8
+    total = t["nsteps"] * Bx + t["nabs"] * Ba + t["nscat"] * Bs + t.get("nrw", 0) * Brw + 8 * t["nescape"]
+    return total, (Bx, Ba, Bs, Brw)
 
 
 def cpu_baseline(m, seconds=12.0):
@@ -58,6 +64,7 @@ def main():
     ap.add_argument("--config", default="production")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain", action="store_true", help="no modified random walk")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many bytes (rocprof counter calibration)")
     a = ap.parse_args()
@@ -78,7 +85,7 @@ def main():
 
     from mcmax_amd import configs, transport
     work = tempfile.mkdtemp(prefix="mcx_bench_")
-    m = configs.load_config(a.config, work)
+    m = configs.load_config(a.config, work, randomwalk=not a.plain)
     print(f"[bench] model {a.config}: {m.summary()}", file=sys.stderr, flush=True)
     tr = transport.Transport(0)
     tr.set_model(m)
@@ -99,19 +106,22 @@ def main():
     ms_step = 1e3 * wall / a.steps
     value = a.nphot * a.steps / wall
     kern_s = sum(kms) / len(kms) * 1e-3
-    nbytes, (Bx, Ba, Bs) = algorithmic_bytes(tal, m.ng, m.nlam)
+    nbytes, (Bx, Ba, Bs, Brw) = algorithmic_bytes(tal, m.ng, m.nlam)
     achieved = nbytes / kern_s
     out = {
         "metric": METRIC, "value": value, "unit": "packets/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{a.config} disk model, one MC pass + T inversion per step",
-                   "nphot": a.nphot, "nr": m.nr, "nt": m.nt, "nlam": m.nlam, "ng": m.ng},
+        "config": {"workload": f"{a.config} disk model, one MC pass + T inversion per step"
+                               + ("" if a.plain else ", modified random walk (factRW=10)"),
+                   "nphot": a.nphot, "nr": m.nr, "nt": m.nt, "nlam": m.nlam, "ng": m.ng,
+                   "randomwalk": not a.plain},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
                      "kernel": "k_transport", "kernel_ms": kern_s * 1e3,
-                     "bytes_per_event": {"step": Bx, "absorption": Ba, "scattering": Bs, "escape": 8},
-                     "events": {k: int(tal[k]) for k in ("nsteps", "nabs", "nscat", "nescape")}},
+                     "bytes_per_event": {"step": Bx, "absorption": Ba, "scattering": Bs, "rw_step": Brw,
+                                         "escape": 8},
+                     "events": {k: int(tal[k]) for k in ("nsteps", "nabs", "nscat", "nrw", "nescape")}},
         "energy_error": abs(tal["Eescape"] / m.Lstar - 1.0),
     }
     if not a.no_cpu:

@@ -62,6 +62,9 @@ typedef struct {
     double Lstar;                    /* trapezoid integral of Fstar         */
     const double *cos2phi, *sin2phi; /* [361] (Init.f:3777-3781)            */
     int32_t scat_how;                /* 0 NONE, 1 ISOTROPIC, 2 FULL         */
+    /* modified random walk only (may be NULL when it is not used):       */
+    const double *Kext;              /* [ng*nlam] as read (Init.f:4886)     */
+    const double *gsym;              /* [ng*nlam] asymmetry g (Init.f:4037) */
 } mcx_tables;
 
 /* Grid and density: D%R, D%Theta and C(i,j)%dens/%V/%w (Modules.f:85-121). */
@@ -71,6 +74,9 @@ typedef struct {
     const double *dens, *V;          /* [nr*nt] g/cm^3, cm^3                */
     const double *w;                 /* [nr*nt*ng] species abundances       */
     double Rstar_cm, Tstar;          /* void-cell temperature, Main.f:546   */
+    const double *thet;              /* [nt+1] theta of the cone edges, for
+                                        MinDist (D%SinTheta, Init.f:2878);
+                                        may be NULL without random walk     */
 } mcx_grid;
 
 /* One MC pass: MCRadiation(NphotTot, ...) with nruns = 1. */
@@ -84,6 +90,9 @@ typedef struct {
                                         reference loop MCRadiation.f:320)   */
     int64_t bitmap_bytes_max;        /* cap for the distinct-packet bitmap
                                         behind Ni (0: 32 GiB)               */
+    int32_t randomwalk;              /* keyword randomwalk (RNDW): modified
+                                        random walk in thick cells          */
+    double factRW;                   /* keyword factRW (Init.f:205: 10)     */
 } mcx_run;
 
 /* Pass tallies and statistics (MCRadiation.f:656-715). */
@@ -98,6 +107,11 @@ typedef struct {
             nraar, nmaxinteract, ngeomfail;
     double tautot, Eescape;          /* sum of optical depth; escaped energy
                                         (equals Lstar: energy conservation) */
+    int64_t nrw;                     /* modified-random-walk steps          */
+    int64_t noverflow;               /* fixed-point adds that saturated (0
+                                        in every valid pass)                */
+    int32_t *thick;                  /* [nr*nt] C%thick of the last pass
+                                        (InitRandomWalk, RandomWalk.f:152)  */
 } mcx_tallies;
 
 /* Temperature products of Main.f:498-549 and the void cell (Main.f:546). */

@@ -26,7 +26,7 @@ class Tables(C.Structure):
         ("lam", _dp), ("nu", _dp), ("BB", _dp), ("Kabs", _dp), ("Ksca", _dp),
         ("F", _dp), ("IF11", _dp), ("IF12", _dp), ("Kp", _dp), ("Fstar", _dp),
         ("Lstar", C.c_double), ("cos2phi", _dp), ("sin2phi", _dp),
-        ("scat_how", C.c_int32),
+        ("scat_how", C.c_int32), ("Kext", _dp), ("gsym", _dp),
     ]
 
 
@@ -34,13 +34,15 @@ class Grid(C.Structure):
     _fields_ = [
         ("nr", C.c_int32), ("nt", C.c_int32), ("R", _dp), ("Theta", _dp),
         ("dens", _dp), ("V", _dp), ("w", _dp), ("Rstar_cm", C.c_double), ("Tstar", C.c_double),
+        ("thet", _dp),
     ]
 
 
 class Run(C.Structure):
     _fields_ = [
         ("nphot", C.c_int64), ("seed", C.c_uint64), ("maxinteract", C.c_int32),
-        ("nlanes", C.c_int32), ("bitmap_bytes_max", C.c_int64),
+        ("nlanes", C.c_int32), ("bitmap_bytes_max", C.c_int64), ("randomwalk", C.c_int32),
+        ("factRW", C.c_double),
     ]
 
 
@@ -52,6 +54,7 @@ class Tallies(C.Structure):
         ("nscat", C.c_int64), ("nescape", C.c_int64), ("nstarhit", C.c_int64),
         ("nsteps", C.c_int64), ("nraar", C.c_int64), ("nmaxinteract", C.c_int64),
         ("ngeomfail", C.c_int64), ("tautot", C.c_double), ("Eescape", C.c_double),
+        ("nrw", C.c_int64), ("noverflow", C.c_int64), ("thick", _ip),
     ]
 
 

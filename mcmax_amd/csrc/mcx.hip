@@ -21,12 +21,17 @@
 //    absorption (Radiation.f:187-213, 966-989);
 //  * Ni (distinct packets per cell, MCRadiation.f:943-946) comes from a
 //    cell x packet bitmap: a packet sets its bit when it enters a cell, and a
-//    popcount per cell after the pass gives the exact count.
+//    popcount per cell after the pass gives the exact count;
+//  * the modified random walk (RandomWalk.f:47-147) reads its diffusion
+//    coefficients from a table per distinct species mix and integer T, built
+//    once per model on the device, instead of recomputing DiffCoeffCell
+//    (Dtable.f:97-182) whenever a cell's temperature moves.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -43,7 +48,9 @@
 #define EABS_BITS 24  // one nominal packet energy maps to 2^24
 
 enum { C_NEMIT = 0, C_NINTERACT, C_NABS, C_NSCAT, C_NESCAPE, C_NSTARHIT, C_NSTEPS, C_NRAAR, C_NMAXINT,
-       C_NGEOM, C_NOVERFLOW, C_NCOUNT };
+       C_NGEOM, C_NOVERFLOW, C_NRW, C_NCOUNT };
+
+#define NY_RW 1000 // survival-table length (RandomWalk.f:6-7)
 
 struct DevTables {
     int nlam, ng, scat_how, scattering;
@@ -82,10 +89,23 @@ struct DevTally {
     double *tautot;
 };
 
+// modified random walk (RandomWalk.f)
+struct DevRW {
+    int on;
+    double factRW;
+    const int *thick;     // [ncell]
+    const int *cls;       // [ncell] species-mix class of the cell
+    const double2 *KD;    // [nclass*(TMAX+1)] {KDext, KDabs} (DiffCoeffCell)
+    const double *R;      // [nr+1] radial edges (AU)
+    const double *sinT;   // [nt+1] sin(theta) of the cone edges
+    const double *y, *phi; // [NY_RW] (InitRandomWalk)
+};
+
 struct TransportArgs {
     DevTables tb;
     DevModel md;
     DevTally ty;
+    DevRW rw;
     unsigned long long *next_pid;
     long long pid_end;
     long long pid_base;
@@ -97,7 +117,7 @@ struct TransportArgs {
     double unitE;  // E0 / 2^EABS_BITS
 };
 
-enum { S_NEED = 0, S_EMIT, S_STEP, S_INTERACT, S_ESCAPE, S_DONE };
+enum { S_NEED = 0, S_EMIT, S_STEP, S_INTERACT, S_ESCAPE, S_RW, S_DONE };
 
 // ---------------------------------------------------------------------------
 // Temperature from absorbed energy per gram (Radiation.f:537-562, 629-702).
@@ -205,9 +225,71 @@ struct Packet {
     bool onEdge, scatt, pol;
 };
 
-__device__ __forceinline__ void tally_ejv(const DevTally &ty, int c, double x) {
+__device__ __forceinline__ void tally_ejv(const DevTally &ty, int c, double x, unsigned long long *scnt) {
     double q = rint(x * ty.invU1[c]);
+    if (!(q < 0x1p62)) {
+        if (!(q > 0.0)) return;
+        atomicAdd(&scnt[C_NOVERFLOW], 1ull);
+        q = 0x1p62;
+    }
     if (q > 0.0) atomicAdd(&ty.EJv[c], (unsigned long long)q);
+}
+
+// Fixed-point Eabs add; returns the cell's total before the add.
+__device__ __forceinline__ unsigned long long add_eabs(const DevTally &ty, int c, double E, double invE,
+                                                       unsigned long long &q, unsigned long long *scnt) {
+    double qd = rint(E * invE);
+    if (!(qd < 0x1p62)) {
+        atomicAdd(&scnt[C_NOVERFLOW], 1ull);
+        qd = 0x1p62;
+    }
+    q = (unsigned long long)qd;
+    return atomicAdd(&ty.Eabs[c], q);
+}
+
+// hunt (Subroutines.f:2086-2130) in the decreasing survival table, from jlo = 1;
+// returns the 1-based jlo.
+__device__ int hunt_desc(const double *__restrict__ xx, int n, double x) {
+    int jlo = 1, jhi, inc = 1;
+    const bool ascnd = (xx[n - 1] >= xx[0]);
+    if ((x >= xx[jlo - 1]) == ascnd) {
+        for (;;) {
+            jhi = jlo + inc;
+            if (jhi > n) {
+                jhi = n + 1;
+                break;
+            }
+            if ((x >= xx[jhi - 1]) == ascnd) {
+                jlo = jhi;
+                inc += inc;
+            } else {
+                break;
+            }
+        }
+    } else {
+        jhi = jlo;
+        for (;;) {
+            jlo = jhi - inc;
+            if (jlo < 1) {
+                jlo = 0;
+                break;
+            }
+            if ((x < xx[jlo - 1]) == ascnd) {
+                jhi = jlo;
+                inc += inc;
+            } else {
+                break;
+            }
+        }
+    }
+    while (jhi - jlo != 1) {
+        int jm = (jhi + jlo) / 2;
+        if ((x >= xx[jm - 1]) == ascnd) jlo = jm;
+        else jhi = jm;
+    }
+    if (x == xx[n - 1]) jlo = n - 1;
+    if (x == xx[0]) jlo = 1;
+    return jlo;
 }
 
 __device__ __forceinline__ void tally_ejv2(const DevTally &ty, int c, double x2, unsigned long long *scnt) {
@@ -372,7 +454,7 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                     p.z = p.z + p.vz * v;
                     p.onEdge = false;
                     double ejv = ((p.E * v) * AU) * Kabs;
-                    tally_ejv(ty, c, ejv);
+                    tally_ejv(ty, c, ejv, scnt);
                     p.nEJv = p.nEJv + ejv;
                     st = S_INTERACT;
                 } else {
@@ -383,7 +465,7 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                     p.nEJv = p.nEJv + ejv;
                     tally_ejv2(ty, c, p.nEJv * p.nEJv, scnt);
                     p.nEJv = 0.0;
-                    tally_ejv(ty, c, ejv);
+                    tally_ejv(ty, c, ejv, scnt);
                     p.onEdge = true;
                     p.tau = p.tau + tau1;
                     tautot += tau1;
@@ -428,9 +510,8 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                 p.U = 0.0;
                 p.V = 0.0;
                 double Ex = (p.E * (KabsL - 0.0)) / KabsL;
-                double qd = rint(Ex * A.invE);
-                unsigned long long q = (unsigned long long)qd;
-                unsigned long long old = atomicAdd(&ty.Eabs[c], q);
+                unsigned long long q;
+                unsigned long long old = add_eabs(ty, c, Ex, A.invE, q, scnt);
                 const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
                 double mass = md.mass[c];
                 double Thint = __longlong_as_double((long long)__hip_atomic_load(
@@ -637,6 +718,63 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                 p.tau0 = -log(g.next());
                 p.tau = 0.0;
                 p.ntrace = 0;
+                // trace2d starts with the random walk in thick cells (MCRadiation.f:916)
+                if (A.rw.on && A.rw.thick[c]) st = S_RW;
+            }
+        }
+
+        // ---- modified random walk (MCRadiation.f:916-929, RandomWalk.f:47-147)
+        if (st == S_RW) {
+            const DevRW &rw = A.rw;
+            const int c = p.i * nt + p.jj;
+            // MinDist (RandomWalk.f:15-43)
+            double rho = p.x * p.x + p.y * p.y;
+            const double r = sqrt(rho + p.z * p.z);
+            rho = sqrt(rho);
+            const double az = fabs(p.z);
+            const double vR1 = fabs(r - rw.R[p.i]), vR2 = fabs(rw.R[p.i + 1] - r);
+            const double vT1 = fabs(sTh[p.jj] * rho - rw.sinT[p.jj] * az);
+            const double vT2 = fabs(sTh[p.jj + 1] * rho - rw.sinT[p.jj + 1] * az);
+            double dmin = vR1;
+            if (vR2 < dmin) dmin = vR2;
+            if (vT1 < dmin) dmin = vT1;
+            if (vT2 < dmin) dmin = vT2;
+            const double Thint = __longlong_as_double(
+                (long long)__hip_atomic_load(&ty.Tcell[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            int iT = (int)(Thint / 1.0);
+            if (iT == 0) iT = 1;
+            if (iT > TMAXD - 1) iT = TMAXD - 1;
+            const double2 kd = rw.KD[(size_t)rw.cls[c] * (TMAXD + 1) + iT];
+            const double dens = md.dens[c];
+            const double lr = 1.0 / ((AU * kd.x) * dens);
+            if (dmin <= rw.factRW * lr) {
+                st = S_STEP;
+            } else {
+                CNT(C_NRW);
+                const int iy = hunt_desc(rw.phi, NY_RW, g.next());
+                const double v = -((3.0 * log(rw.y[iy - 1])) * (dmin * dmin)) / (lr * (MCX_PI * MCX_PI));
+                random_direction(g, p.vx, p.vy, p.vz);
+                p.x = p.x + dmin * p.vx;
+                p.y = p.y + dmin * p.vy;
+                p.z = p.z + dmin * p.vz;
+                const double ejv = ((p.E * v) * AU) * kd.y;
+                unsigned long long q;
+                const unsigned long long old = add_eabs(ty, c, ejv * dens, A.invE, q, scnt);
+                tally_ejv(ty, c, ejv, scnt);
+                const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
+                const double T1 = dev_increaseT(kpc, ((double)(old + q) * A.unitE) / md.mass[c], Thint);
+                atomicMax(&ty.Tcell[c], (unsigned long long)__double_as_longlong(T1));
+                tautot += v / dmin;
+                // leave in a direction of the forward hemisphere (MCRadiation.f:919-925)
+                double vx, vy, vz;
+                for (;;) {
+                    random_direction(g, vx, vy, vz);
+                    const double inp = p.vx * vx + p.vy * vy + p.vz * vz;
+                    if (!(inp < 0.0)) break;
+                }
+                p.vx = vx;
+                p.vy = vy;
+                p.vz = vz;
             }
         }
 
@@ -708,6 +846,75 @@ __global__ void k_cell_units(int nr, int nt, int nlam, const double *__restrict_
     if (!(B > 0.0)) B = 1e-300;
     invU1[c] = ldexp(1.0, EJV_BITS) / B;
     invU2[c] = ldexp(1.0, EJV2_BITS) / (B * B);
+}
+
+// DiffCoeffCell (Dtable.f:97-182) for every species-mix class and integer T:
+// KDext = int dB/dT dnu / int dB/dT / (Kext - g^2 Ksca) dnu,
+// KDabs = int dB/dT Kabs dnu / int dB/dT dnu, trapezoids over nu.
+// The species sums are those the MC pass sees: KabsTot/KscaTot, except at the
+// four temperatures InitRandomWalk evaluates first (Kext - Ksca from the
+// species tables, kept by the one-species cache of Dtable.f:107-112, 175-179).
+__global__ void k_diffcoeff(int nclass, int nlam, int ng, int scattering, const double *__restrict__ Kabs,
+                            const double *__restrict__ Ksca, const double *__restrict__ Kext,
+                            const double *__restrict__ gsym, const double *__restrict__ wcls,
+                            const double *__restrict__ BB, const double *__restrict__ nu, double2 *KD) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)nclass * (TMAXD + 1)) return;
+    const int k = (int)(idx / (TMAXD + 1));
+    int iT = (int)(idx % (TMAXD + 1));
+    if (iT < 1) iT = 1;
+    if (iT > TMAXD - 1) iT = TMAXD - 1;
+    const bool kabstot = !(iT == 100 || iT == 600 || iT == 1100 || iT == 1600);
+    const double *w = wcls + (size_t)k * ng;
+    const double *B0 = BB + (size_t)iT * nlam, *B1 = BB + (size_t)(iT + 1) * nlam;
+    double i1 = 0.0, i2 = 0.0, i3 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+    for (int l = 0; l < nlam; l++) {
+        double ka = 0.0, ks = 0.0, ke = 0.0, gg = 0.0;
+        if (kabstot) {
+            for (int s = 0; s < ng; s++) {
+                ka = ka + (Kabs[(size_t)s * nlam + l] * w[s]) * 1.0;
+                ks = ks + (Ksca[(size_t)s * nlam + l] * w[s]) * 1.0;
+            }
+            ke = ka + ks;
+        } else {
+            for (int s = 0; s < ng; s++) {
+                ke = ke + (Kext[(size_t)s * nlam + l] * w[s]) * 1.0;
+                ks = ks + (Ksca[(size_t)s * nlam + l] * w[s]) * 1.0;
+            }
+            ka = ke - ks;
+        }
+        for (int s = 0; s < ng; s++) gg = gg + ((gsym[(size_t)s * nlam + l] * Ksca[(size_t)s * nlam + l]) * w[s]) * 1.0;
+        const double gl = scattering ? gg / ks : 0.0;
+        const double dB = B1[l] - B0[l];
+        const double s1 = dB / (ke - (gl * gl) * ks), s2 = dB, s3 = dB * (ka - 0.0);
+        if (l > 0) {
+            const double h = fabs(nu[l - 1] - nu[l]) * 0.5;
+            i1 = i1 + h * (p1 + s1);
+            i2 = i2 + h * (p2 + s2);
+            i3 = i3 + h * (p3 + s3);
+        }
+        p1 = s1;
+        p2 = s2;
+        p3 = s3;
+    }
+    KD[idx] = make_double2(i2 / i1, i3 / i2);
+}
+
+// C%thick (RandomWalk.f:174-189): the shell is wider than factRW Rosseland
+// lengths at any of T = 100, 600, 1100, 1600 K.
+__global__ void k_thick(int nr, int nt, const double *__restrict__ R, const double *__restrict__ dens,
+                        const int *__restrict__ cls, const double2 *__restrict__ KD, double factRW, int *thick) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nr * nt) return;
+    const int i = c / nt;
+    int th = 0;
+    if (i > 0) {
+        for (int iT = 100; iT <= 1600; iT += 500) {
+            const double lr = 1.0 / ((MCX_AU * KD[(size_t)cls[c] * (TMAXD + 1) + iT].x) * dens[c]);
+            if ((R[i + 1] - R[i]) > factRW * lr) th = 1;
+        }
+    }
+    thick[c] = th;
 }
 
 __global__ void k_popcount(int ncell, int nwords, const unsigned *__restrict__ bitmap, int *Ni) {
@@ -790,6 +997,12 @@ struct mcx_ctx {
     double Lstar = 0.0;
     std::vector<double> h_Kp, h_nu;
     double *d_Kabs = nullptr, *d_Ksca = nullptr;
+    double *d_Kext = nullptr, *d_gsym = nullptr, *d_BB = nullptr; // random-walk tables (may stay null)
+    const double *d_rwy = nullptr, *d_rwphi = nullptr;
+    // random walk, per model
+    DevRW rw{};
+    int nclass = 0;
+    int *d_thick = nullptr;
     // model
     DevModel md{};
     double *d_V = nullptr, *d_R2 = nullptr;
@@ -839,6 +1052,35 @@ static int dupload(mcx_ctx *ctx, std::vector<void *> &list, T **p, const T *h, s
 static void free_list(std::vector<void *> &list) {
     for (void *q : list) (void)hipFree(q);
     list.clear();
+}
+
+// y^k for integer k by right-to-left binary powering (Fortran's real**integer)
+static double powi_rl(double x, unsigned k) {
+    double r = (k & 1u) ? x : 1.0;
+    while (k >>= 1) {
+        x = x * x;
+        if (k & 1u) r = r * x;
+    }
+    return r;
+}
+
+// InitRandomWalk's tables (RandomWalk.f:160-169): y(i) and the survival
+// probability phi(y) = 2 sum_n (-1)^(n+1) y^(n^2), 1000 terms.
+static void random_walk_tables(std::vector<double> &y, std::vector<double> &phi) {
+    const int NY = NY_RW, nmax = 1000;
+    y.assign(NY, 0.0);
+    phi.assign(NY, 0.0);
+    y[0] = 1.0;
+    phi[0] = 1.0;
+    for (int i = 2; i <= NY; i++) {
+        y[i - 1] = (double)((float)(NY - i + 1) / (float)(NY - 1)) * 0.75;
+        double sum = 0.0;
+        for (int n = 1; n <= nmax; n++) {
+            double sg = ((n + 1) % 2 == 0) ? 1.0 : -1.0;
+            sum = sum + sg * powi_rl(y[i - 1], (unsigned)(n * n));
+        }
+        phi[i - 1] = sum * 2.0;
+    }
 }
 
 extern "C" int mcx_create(mcx_ctx **out, int device) {
@@ -974,6 +1216,18 @@ extern "C" int mcx_set_tables(mcx_ctx *ctx, const mcx_tables *t) {
     }
     UP(ctx->d_Kabs, t->Kabs, (size_t)ng * nlam);
     ctx->d_Ksca = (double *)tb.Ksca;
+    ctx->d_Kext = ctx->d_gsym = ctx->d_BB = nullptr;
+    if (t->Kext && t->gsym) {
+        UP(ctx->d_Kext, t->Kext, (size_t)ng * nlam);
+        UP(ctx->d_gsym, t->gsym, (size_t)ng * nlam);
+        UP(ctx->d_BB, t->BB, (size_t)(TMAXD + 1) * nlam);
+    }
+    {
+        std::vector<double> y, phi;
+        random_walk_tables(y, phi);
+        UP(ctx->d_rwy, y.data(), NY_RW);
+        UP(ctx->d_rwphi, phi.data(), NY_RW);
+    }
 #undef UP
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->have_tables = true;
@@ -1034,6 +1288,52 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
     }
     md.KT = KT;
     md.KpCell = KpCell;
+    // random walk: species-mix classes, diffusion-coefficient table, edges
+    ctx->rw = DevRW{};
+    ctx->nclass = 0;
+    ctx->d_thick = nullptr;
+    if (ctx->d_Kext && gr->thet) {
+        std::map<std::vector<uint64_t>, int> seen;
+        std::vector<int> cls(ncell);
+        std::vector<double> wcls;
+        for (int c = 0; c < ncell; c++) {
+            std::vector<uint64_t> key(ng);
+            memcpy(key.data(), gr->w + (size_t)c * ng, sizeof(double) * ng);
+            auto it = seen.find(key);
+            if (it == seen.end()) {
+                it = seen.emplace(key, (int)seen.size()).first;
+                wcls.insert(wcls.end(), gr->w + (size_t)c * ng, gr->w + (size_t)(c + 1) * ng);
+            }
+            cls[c] = it->second;
+        }
+        const int nclass = (int)seen.size();
+        if ((size_t)nclass * (TMAXD + 1) * sizeof(double2) <= ((size_t)4 << 30)) {
+            std::vector<double> sinT(nt + 1);
+            for (int j = 0; j <= nt; j++) sinT[j] = sin(gr->thet[j]);
+            const double *dw, *dR, *dsin;
+            int *dcls;
+            double2 *KD;
+            if ((r = dupload(ctx, ctx->mod_alloc, (double **)&dw, wcls.data(), wcls.size()))) return r;
+            if ((r = dupload(ctx, ctx->mod_alloc, (double **)&dR, gr->R, (size_t)nr + 1))) return r;
+            if ((r = dupload(ctx, ctx->mod_alloc, (double **)&dsin, sinT.data(), (size_t)nt + 1))) return r;
+            if ((r = dupload(ctx, ctx->mod_alloc, &dcls, cls.data(), (size_t)ncell))) return r;
+            if ((r = dalloc(ctx, ctx->mod_alloc, &KD, (size_t)nclass * (TMAXD + 1)))) return r;
+            if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_thick, (size_t)ncell))) return r;
+            size_t n = (size_t)nclass * (TMAXD + 1);
+            hipLaunchKernelGGL(k_diffcoeff, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, nclass, nlam,
+                               ng, ctx->tb.scattering, ctx->d_Kabs, ctx->d_Ksca, ctx->d_Kext, ctx->d_gsym, dw,
+                               ctx->d_BB, ctx->tb.nu, KD);
+            HIPCHK(ctx, hipGetLastError());
+            ctx->nclass = nclass;
+            ctx->rw.cls = dcls;
+            ctx->rw.KD = KD;
+            ctx->rw.R = dR;
+            ctx->rw.sinT = dsin;
+            ctx->rw.y = ctx->d_rwy;
+            ctx->rw.phi = ctx->d_rwphi;
+            ctx->rw.thick = ctx->d_thick;
+        }
+    }
     {
         // Main.f:546
         const double AU = MCX_AU;
@@ -1097,6 +1397,22 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
         if (r) return r;
         ctx->bitmap_bytes = need;
     }
+    DevRW rw = ctx->rw;
+    rw.on = 0;
+    if (run->randomwalk) {
+        if (!rw.KD)
+            return fail(ctx, MCX_EINVAL,
+                        "random walk needs Kext and gsym (mcx_set_tables), thet (mcx_set_model) and at most "
+                        "4 GiB of diffusion-coefficient tables");
+        if (!(run->factRW > 0.0)) return fail(ctx, MCX_EINVAL, "factRW must be positive");
+        rw.on = 1;
+        rw.factRW = run->factRW;
+        hipLaunchKernelGGL(k_thick, dim3((ncell + 255) / 256), dim3(256), 0, s, md.nr, md.nt, rw.R, md.dens, rw.cls,
+                           rw.KD, run->factRW, ctx->d_thick);
+        HIPCHK(ctx, hipGetLastError());
+    } else if (ctx->d_thick) {
+        HIPCHK(ctx, hipMemsetAsync(ctx->d_thick, 0, sizeof(int) * ncell, s));
+    }
     const double E0 = ctx->Lstar / ((double)(float)run->nphot * ((1.0 - 0.0) - 0.0));
     ctx->E0 = E0;
     ctx->unitE = ldexp(E0, -EABS_BITS);
@@ -1136,6 +1452,7 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     A.tb = ctx->tb;
     A.md = md;
     A.ty = ty;
+    A.rw = rw;
     A.next_pid = ctx->d_next;
     A.nphot = run->nphot;
     A.seed = run->seed;
@@ -1190,6 +1507,10 @@ extern "C" int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *o) {
     if (o->Eabs) HIPCHK(ctx, hipMemcpy(o->Eabs, ctx->d_Eabs, sizeof(double) * ncell, hipMemcpyDeviceToHost));
     if (o->Tbw) HIPCHK(ctx, hipMemcpy(o->Tbw, ctx->d_Tbw, sizeof(double) * ncell, hipMemcpyDeviceToHost));
     if (o->Ni) HIPCHK(ctx, hipMemcpy(o->Ni, ctx->d_Ni, sizeof(int) * ncell, hipMemcpyDeviceToHost));
+    if (o->thick) {
+        if (ctx->d_thick) HIPCHK(ctx, hipMemcpy(o->thick, ctx->d_thick, sizeof(int) * ncell, hipMemcpyDeviceToHost));
+        else memset(o->thick, 0, sizeof(int32_t) * ncell);
+    }
     const size_t nb = (size_t)(NANGD + 1) * nlam;
     std::vector<unsigned long long> sE(2 * nb);
     std::vector<unsigned> sc(2 * nb);
@@ -1230,6 +1551,8 @@ extern "C" int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *o) {
     o->nraar = (int64_t)cnt[C_NRAAR];
     o->nmaxinteract = (int64_t)cnt[C_NMAXINT];
     o->ngeomfail = (int64_t)cnt[C_NGEOM];
+    o->nrw = (int64_t)cnt[C_NRW];
+    o->noverflow = (int64_t)cnt[C_NOVERFLOW];
     o->tautot = tt;
     o->Eescape = Eesc;
     return MCX_OK;

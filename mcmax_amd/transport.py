@@ -56,22 +56,29 @@ class Transport:
             lam=_c(m.lam), nu=_c(m.nu), BB=_c(m.BB), Kabs=_c(m.Kabs), Ksca=_c(m.Ksca),
             F=_c(m.F), IF11=_c(m.IF11), IF12=_c(m.IF12), Kp=_c(m.Kp), Fstar=_c(m.Fstar),
             c2=_c(m.cos2phi), s2=_c(m.sin2phi), R=_c(m.R), Th=_c(m.Theta), dens=_c(m.dens),
-            V=_c(m.V), w=_c(m.w))
+            V=_c(m.V), w=_c(m.w), Kext=_c(m.Kext), gsym=_c(m.gsym), thet=_c(m.thet))
         t = _lib.Tables(nlam=m.nlam, ng=m.ng, tmax=TMAX, lam=ptr(keep["lam"]), nu=ptr(keep["nu"]),
                         BB=ptr(keep["BB"]), Kabs=ptr(keep["Kabs"]), Ksca=ptr(keep["Ksca"]),
                         F=ptr(keep["F"]), IF11=ptr(keep["IF11"]), IF12=ptr(keep["IF12"]),
                         Kp=ptr(keep["Kp"]), Fstar=ptr(keep["Fstar"]), Lstar=float(m.Lstar),
-                        cos2phi=ptr(keep["c2"]), sin2phi=ptr(keep["s2"]), scat_how=int(m.scat_how))
+                        cos2phi=ptr(keep["c2"]), sin2phi=ptr(keep["s2"]), scat_how=int(m.scat_how),
+                        Kext=ptr(keep["Kext"]), gsym=ptr(keep["gsym"]))
         self._check(self.lib.mcx_set_tables(self.ctx, C.byref(t)))
         g = _lib.Grid(nr=m.nr, nt=m.nt, R=ptr(keep["R"]), Theta=ptr(keep["Th"]), dens=ptr(keep["dens"]),
-                      V=ptr(keep["V"]), w=ptr(keep["w"]), Rstar_cm=float(m.Rstar_cm), Tstar=float(m.Tstar))
+                      V=ptr(keep["V"]), w=ptr(keep["w"]), Rstar_cm=float(m.Rstar_cm), Tstar=float(m.Tstar),
+                      thet=ptr(keep["thet"]))
         self._check(self.lib.mcx_set_model(self.ctx, C.byref(g)))
         self.model = m
 
-    def transport(self, nphot, seed=1, nlanes=0, maxinteract=0, bitmap_bytes_max=0):
-        """One MC pass (mcx_transport); tallies stay on the device."""
+    def transport(self, nphot, seed=1, nlanes=0, maxinteract=0, bitmap_bytes_max=0, randomwalk=None,
+                  factRW=None):
+        """One MC pass (mcx_transport); tallies stay on the device.  The
+        random-walk switches default to the model's keywords."""
+        m = self.model
+        rw = bool(getattr(m, "randomwalk", False)) if randomwalk is None else bool(randomwalk)
+        f = float(getattr(m, "factRW", 10.0)) if factRW is None else float(factRW)
         r = _lib.Run(nphot=int(nphot), seed=int(seed), maxinteract=int(maxinteract), nlanes=int(nlanes),
-                     bitmap_bytes_max=int(bitmap_bytes_max))
+                     bitmap_bytes_max=int(bitmap_bytes_max), randomwalk=int(rw), factRW=f)
         rc = self.lib.mcx_transport(self.ctx, C.byref(r))
         if rc not in (0, _lib.MCX_EGEOM):
             self._check(rc)
@@ -83,6 +90,7 @@ class Transport:
         nb = (NANGLE + 1) * m.nlam
         out = {k: np.zeros(n) for k in ("EJv", "EJv2", "Eabs", "Tbw")}
         out["Ni"] = np.zeros(n, dtype=np.int32)
+        out["thick"] = np.zeros(n, dtype=np.int32)
         for k in ("spec", "scatspec"):
             out[k] = np.zeros(nb)
         for k in ("ispec", "iscatspec"):
@@ -91,10 +99,11 @@ class Transport:
         self._check(self.lib.mcx_get_tallies(self.ctx, C.byref(t)))
         for k in ("EJv", "EJv2", "Eabs", "Tbw", "Ni"):
             out[k] = out[k].reshape(m.nr, m.nt)
+        out["thick"] = out["thick"].reshape(m.nr, m.nt).astype(bool)
         for k in ("spec", "scatspec", "ispec", "iscatspec"):
             out[k] = out[k].reshape(NANGLE + 1, m.nlam)
         for k in ("nemit", "ninteract", "nabs", "nscat", "nescape", "nstarhit", "nsteps", "nraar",
-                  "nmaxinteract", "ngeomfail", "tautot", "Eescape"):
+                  "nmaxinteract", "ngeomfail", "tautot", "Eescape", "nrw", "noverflow"):
             out[k] = getattr(t, k)
         return out
 

@@ -600,6 +600,28 @@ void orc_diffcoeff(const orc_model *m, int c, int iT, int opacity_set, double *K
     free(Ksca); free(Kext); free(Kabs); free(g); free(dBB); free(spec);
 }
 
+/* y**k with an integer exponent: Fortran evaluates it by repeated squaring
+ * (right-to-left binary method), not through the real-exponent pow. */
+static double powi_rl(double x, unsigned k)
+{
+    double r = (k & 1u) ? x : 1.0;
+    while (k >>= 1) {
+        x = x * x;
+        if (k & 1u) r = r * x;
+    }
+    return r;
+}
+
+/* Which opacity sums DiffCoeffCell sees during the MC pass.  InitRandomWalk
+ * evaluates it at iT = 100, 600, 1100, 1600 before any KabsTot exists
+ * (species sums, Kabs = Kext - Ksca), and for one species those results are
+ * kept in the global iTD/KDext/KDabs cache (Dtable.f:107-112, 175-179); every
+ * other temperature is evaluated inside MCRadiation on KabsTot/KscaTot. */
+int orc_rw_opacity_set(int iT)
+{
+    return !(iT == 100 || iT == 600 || iT == 1100 || iT == 1600);
+}
+
 void orc_init_random_walk(const orc_model *m, double *y, double *phi, int32_t *thick)
 {
     /* RandomWalk.f:152-194 */
@@ -611,7 +633,7 @@ void orc_init_random_walk(const orc_model *m, double *y, double *phi, int32_t *t
         double s = 0.0;
         for (int n = 1; n <= nmax; n++) {
             double sg = ((n + 1) % 2 == 0) ? 1.0 : -1.0;
-            s = s + sg * pow(y[i - 1], (double)(n * n));
+            s = s + sg * powi_rl(y[i - 1], (unsigned)(n * n));
         }
         phi[i - 1] = s * 2.0;
     }
@@ -663,15 +685,15 @@ static int hunt_desc(const double *xx, int n, double x)
 }
 
 /* RandomWalk.f:15-43 */
-static double min_dist(const orc_model *m, const phot_t *p)
+double orc_min_dist(const orc_model *m, double x, double y, double z, int i, int jj)
 {
-    double rho = p->x * p->x + p->y * p->y;
-    double r = sqrt(rho + p->z * p->z);
+    double rho = x * x + y * y;
+    double r = sqrt(rho + z * z);
     rho = sqrt(rho);
-    double z = fabs(p->z);
-    double R1 = m->R[p->i], R2 = m->R[p->i + 1];
-    double T1 = m->Theta[p->jj], T2 = m->Theta[p->jj + 1];
-    double ST1 = sin(m->thet[p->jj]), ST2 = sin(m->thet[p->jj + 1]);
+    z = fabs(z);
+    double R1 = m->R[i], R2 = m->R[i + 1];
+    double T1 = m->Theta[jj], T2 = m->Theta[jj + 1];
+    double ST1 = sin(m->thet[jj]), ST2 = sin(m->thet[jj + 1]);
     double vR1 = fabs(r - R1), vR2 = fabs(R2 - r);
     double vT1 = fabs(T1 * rho - ST1 * z), vT2 = fabs(T2 * rho - ST2 * z);
     double d = vR1;
@@ -679,6 +701,11 @@ static double min_dist(const orc_model *m, const phot_t *p)
     if (vT1 < d) d = vT1;
     if (vT2 < d) d = vT2;
     return d;
+}
+
+static double min_dist(const orc_model *m, const phot_t *p)
+{
+    return orc_min_dist(m, p->x, p->y, p->z, p->i, p->jj);
 }
 
 /* RandomWalk.f:47-147 (tcontact, no QHP/LRF) */
@@ -692,12 +719,13 @@ static int random_walk(ctx_t *C, phot_t *p, double dmin)
     if (iT > TMAX) iT = TMAX;
     if (iT > TMAX - 1) iT = TMAX - 1; /* DiffCoeffCell clamps its argument */
     if (iT != C->iTD[c]) {
-        orc_diffcoeff(m, c, iT, 1, &C->KDext[c], &C->KDabs[c]);
+        orc_diffcoeff(m, c, iT, orc_rw_opacity_set(iT), &C->KDext[c], &C->KDabs[c]);
         C->iTD[c] = iT;
     }
     double Kext = C->KDext[c];
     double lr = 1.0 / ((AU_C * Kext) * m->dens[c]);
     if (dmin <= m->factRW * lr) return 0;
+    t->nrw++;
     double x = ran2(&C->g);
     int iy = hunt_desc(C->rphi, 1000, x);
     double v = -((3.0 * log(C->ry[iy - 1])) * (dmin * dmin)) / (lr * (PI * PI));
@@ -1187,6 +1215,7 @@ int orc_mcradiation(const orc_model *m, const orc_run *run, orc_ran1_state *rs, 
     }
     t->nemit = t->ninteract = t->nabs = t->nscat = t->nescape = t->nstarhit = t->nsteps = 0;
     t->nraar = t->nmaxinteract = 0;
+    t->nrw = 0;
     t->tautot = 0.0;
     t->Eescape = 0.0;
     int err = 0;

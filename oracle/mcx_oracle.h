@@ -66,8 +66,15 @@ typedef struct {
  * absorption of cell c at integer temperature iT; opacity_set selects the
  * per-cell KabsTot/KscaTot path (inside MCRadiation) or the species sums. */
 void orc_diffcoeff(const orc_model *m, int c, int iT, int opacity_set, double *KDext, double *KDabs);
-/* InitRandomWalk (RandomWalk.f:152-194): y/phi tables and the thick flags */
+/* InitRandomWalk (RandomWalk.f:152-194): y/phi tables and the thick flags
+ * (thick may be NULL for the tables alone) */
 void orc_init_random_walk(const orc_model *m, double *y, double *phi, int32_t *thick);
+/* opacity_set argument DiffCoeffCell effectively sees at temperature iT in
+ * the MC pass (1: KabsTot/KscaTot, 0: species sums) */
+int orc_rw_opacity_set(int iT);
+/* MinDist (RandomWalk.f:15-43): distance from (x,y,z) in cell (i,jj) to the
+ * nearest of its two spheres and two cones */
+double orc_min_dist(const orc_model *m, double x, double y, double z, int i, int jj);
 
 typedef struct {
     int64_t nphot;
@@ -87,6 +94,7 @@ typedef struct {
     double tautot, Eescape;
     int32_t *nev1, *nev2;          /* optional [nr*nt]: number of additions
                                       to EJv and to EJv2 (test diagnostics) */
+    int64_t nrw;                   /* modified-random-walk steps taken      */
 } orc_tally;
 
 /* NR ran1 state (iv, iy SAVEd and THREADPRIVATE in the reference). */

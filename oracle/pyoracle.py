@@ -40,7 +40,7 @@ class OrcTally(C.Structure):
         ("nemit", C.c_int64), ("ninteract", C.c_int64), ("nabs", C.c_int64), ("nscat", C.c_int64),
         ("nescape", C.c_int64), ("nstarhit", C.c_int64), ("nsteps", C.c_int64), ("nraar", C.c_int64),
         ("nmaxinteract", C.c_int64), ("tautot", C.c_double), ("Eescape", C.c_double),
-        ("nev1", _ip), ("nev2", _ip),
+        ("nev1", _ip), ("nev2", _ip), ("nrw", C.c_int64),
     ]
 
 
@@ -124,6 +124,9 @@ def load():
     lib.orc_arrinterpol_log.restype = C.c_double
     lib.orc_diffcoeff.argtypes = [C.POINTER(OrcModel), C.c_int, C.c_int, C.c_int, _dp, _dp]
     lib.orc_init_random_walk.argtypes = [C.POINTER(OrcModel), _dp, _dp, _ip]
+    lib.orc_rw_opacity_set.argtypes = [C.c_int]
+    lib.orc_min_dist.argtypes = [C.POINTER(OrcModel), C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
+    lib.orc_min_dist.restype = C.c_double
     _lib = lib
     return lib
 
@@ -156,6 +159,22 @@ class Oracle:
                            thet=p(k["thet"]))
         self.ran1 = Ran1State(idum=-42, iy=0)
 
+    def random_walk_tables(self):
+        """(y, phi, thick) of InitRandomWalk for this model."""
+        m = self.m
+        y, ph = np.zeros(1000), np.zeros(1000)
+        th = np.zeros(m.nr * m.nt, dtype=np.int32)
+        self.lib.orc_init_random_walk(C.byref(self.om), p(y), p(ph), p(th))
+        return y, ph, th.reshape(m.nr, m.nt).astype(bool)
+
+    def diffcoeff(self, c, iT, opacity_set):
+        a, b = C.c_double(), C.c_double()
+        self.lib.orc_diffcoeff(C.byref(self.om), int(c), int(iT), int(opacity_set), C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def min_dist(self, x, y, z, i, jj):
+        return self.lib.orc_min_dist(C.byref(self.om), x, y, z, int(i), int(jj))
+
     def mcradiation(self, nphot, rng="philox", seed=1, maxinteract=0):
         m = self.m
         n = m.nr * m.nt
@@ -171,7 +190,7 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"oracle transport failed: {rc}")
         for k in ("nemit", "ninteract", "nabs", "nscat", "nescape", "nstarhit", "nsteps", "nraar",
-                  "nmaxinteract", "tautot", "Eescape"):
+                  "nmaxinteract", "tautot", "Eescape", "nrw"):
             out[k] = getattr(t, k)
         out["Tbw"] = out.pop("T")
         for k in ("EJv", "EJv2", "Eabs", "Tbw", "Ni", "nev1", "nev2"):
