@@ -761,6 +761,7 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                 unsigned long long q;
                 const unsigned long long old = add_eabs(ty, c, ejv * dens, A.invE, q, scnt);
                 tally_ejv(ty, c, ejv, scnt);
+                p.nEJv = p.nEJv + ejv; // the module-level nEJv of RandomWalk.f:115
                 const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
                 const double T1 = dev_increaseT(kpc, ((double)(old + q) * A.unitE) / md.mass[c], Thint);
                 atomicMax(&ty.Tcell[c], (unsigned long long)__double_as_longlong(T1));

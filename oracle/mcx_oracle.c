@@ -1225,6 +1225,7 @@ int orc_mcradiation(const orc_model *m, const orc_run *run, orc_ran1_state *rs, 
     for (int64_t ip = 1; ip <= Nphot && !err; ip++) {
         p.nr = ip;
         if (run->rng == 1) { C.g.pid = (uint64_t)(ip - 1); C.g.k = 0; }
+        if (t->pkt_steps) t->pkt_steps[ip - 1] = -t->nsteps;
     label3:
         C.nEJv = 0.0;
         emit_star(&C, &p, Nphot);
@@ -1257,6 +1258,7 @@ int orc_mcradiation(const orc_model *m, const orc_run *run, orc_ran1_state *rs, 
             }
             tau = -log(ran2(&C.g));
         }
+        if (t->pkt_steps) t->pkt_steps[ip - 1] += t->nsteps;
     }
     free(C.KabsTot); free(C.KscaTot); free(C.mass); free(C.KabsL); free(C.KextL);
     free(C.spec); free(C.lastphotnr);

@@ -95,6 +95,8 @@ typedef struct {
     int32_t *nev1, *nev2;          /* optional [nr*nt]: number of additions
                                       to EJv and to EJv2 (test diagnostics) */
     int64_t nrw;                   /* modified-random-walk steps taken      */
+    int64_t *pkt_steps;            /* optional [nphot]: cell steps of each
+                                      packet (test diagnostics)             */
 } orc_tally;
 
 /* NR ran1 state (iv, iy SAVEd and THREADPRIVATE in the reference). */

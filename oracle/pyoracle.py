@@ -40,7 +40,7 @@ class OrcTally(C.Structure):
         ("nemit", C.c_int64), ("ninteract", C.c_int64), ("nabs", C.c_int64), ("nscat", C.c_int64),
         ("nescape", C.c_int64), ("nstarhit", C.c_int64), ("nsteps", C.c_int64), ("nraar", C.c_int64),
         ("nmaxinteract", C.c_int64), ("tautot", C.c_double), ("Eescape", C.c_double),
-        ("nev1", _ip), ("nev2", _ip), ("nrw", C.c_int64),
+        ("nev1", _ip), ("nev2", _ip), ("nrw", C.c_int64), ("pkt_steps", C.POINTER(C.c_int64)),
     ]
 
 
@@ -175,7 +175,7 @@ class Oracle:
     def min_dist(self, x, y, z, i, jj):
         return self.lib.orc_min_dist(C.byref(self.om), x, y, z, int(i), int(jj))
 
-    def mcradiation(self, nphot, rng="philox", seed=1, maxinteract=0):
+    def mcradiation(self, nphot, rng="philox", seed=1, maxinteract=0, per_packet=False):
         m = self.m
         n = m.nr * m.nt
         nb = (NANGLE + 1) * m.nlam
@@ -185,6 +185,9 @@ class Oracle:
         out["ispec"], out["iscatspec"] = np.zeros(nb, dtype=np.int32), np.zeros(nb, dtype=np.int32)
         out["nev1"], out["nev2"] = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.int32)
         t = OrcTally(**{k: p(v) for k, v in out.items()})
+        if per_packet:
+            out["pkt_steps"] = np.zeros(int(nphot), dtype=np.int64)
+            t.pkt_steps = out["pkt_steps"].ctypes.data_as(C.POINTER(C.c_int64))
         run = OrcRun(nphot=int(nphot), rng=0 if rng == "ran1" else 1, seed=int(seed), maxinteract=int(maxinteract))
         rc = self.lib.orc_mcradiation(C.byref(self.om), C.byref(run), C.byref(self.ran1), C.byref(t))
         if rc != 0:
