@@ -93,6 +93,11 @@ typedef struct {
     int32_t randomwalk;              /* keyword randomwalk (RNDW): modified
                                         random walk in thick cells          */
     double factRW;                   /* keyword factRW (Init.f:205: 10)     */
+    int32_t coop_lanes;              /* inverse-CDF lookups: a wavefront
+                                        serves them together when at most
+                                        this many lanes need one; -1 picks
+                                        the default, 0 never (results do not
+                                        depend on it)                       */
 } mcx_run;
 
 /* Pass tallies and statistics (MCRadiation.f:656-715). */

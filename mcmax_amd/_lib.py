@@ -42,7 +42,7 @@ class Run(C.Structure):
     _fields_ = [
         ("nphot", C.c_int64), ("seed", C.c_uint64), ("maxinteract", C.c_int32),
         ("nlanes", C.c_int32), ("bitmap_bytes_max", C.c_int64), ("randomwalk", C.c_int32),
-        ("factRW", C.c_double),
+        ("factRW", C.c_double), ("coop_lanes", C.c_int32),
     ]
 
 

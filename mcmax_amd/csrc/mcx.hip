@@ -113,11 +113,13 @@ struct TransportArgs {
     unsigned long long seed;
     int maxinteract;
     int nwords;
+    int serial;    // only lane 0 of the grid takes packets (in packet order)
+    int coop;      // wave-cooperative lookups when at most this many lanes need one
     double invE;   // 2^EABS_BITS / E0
     double unitE;  // E0 / 2^EABS_BITS
 };
 
-enum { S_NEED = 0, S_EMIT, S_STEP, S_INTERACT, S_ESCAPE, S_RW, S_DONE };
+enum { S_NEED = 0, S_EMIT, S_STEP, S_INTERACT, S_LOOKUP, S_AFTER, S_ESCAPE, S_RW, S_DONE };
 
 // ---------------------------------------------------------------------------
 // Temperature from absorbed energy per gram (Radiation.f:537-562, 629-702).
@@ -225,16 +227,6 @@ struct Packet {
     bool onEdge, scatt, pol;
 };
 
-__device__ __forceinline__ void tally_ejv(const DevTally &ty, int c, double x, unsigned long long *scnt) {
-    double q = rint(x * ty.invU1[c]);
-    if (!(q < 0x1p62)) {
-        if (!(q > 0.0)) return;
-        atomicAdd(&scnt[C_NOVERFLOW], 1ull);
-        q = 0x1p62;
-    }
-    if (q > 0.0) atomicAdd(&ty.EJv[c], (unsigned long long)q);
-}
-
 // Fixed-point Eabs add; returns the cell's total before the add.
 __device__ __forceinline__ unsigned long long add_eabs(const DevTally &ty, int c, double E, double invE,
                                                        unsigned long long &q, unsigned long long *scnt) {
@@ -292,19 +284,146 @@ __device__ int hunt_desc(const double *__restrict__ xx, int n, double x) {
     return jlo;
 }
 
-__device__ __forceinline__ void tally_ejv2(const DevTally &ty, int c, double x2, unsigned long long *scnt) {
-    double q = x2 * ty.invU2[c];
+// Per-lane integer partial sums of EJv and the three EJv2 limbs for one cell.
+// Integer addition is associative, so adding them to the cell when the lane
+// moves on gives the same totals as adding every contribution at once.
+struct TallyAcc {
+    int c;
+    unsigned long long e1, l0, l1, l2;
+};
+
+__device__ __forceinline__ void acc_flush(const DevTally &ty, TallyAcc &t) {
+    if (t.c >= 0) {
+        if (t.e1) atomicAdd(&ty.EJv[t.c], t.e1);
+        unsigned long long *b = ty.EJv2 + 3 * (size_t)t.c;
+        if (t.l0) atomicAdd(b, t.l0);
+        if (t.l1) atomicAdd(b + 1, t.l1);
+        if (t.l2) atomicAdd(b + 2, t.l2);
+    }
+    t.e1 = t.l0 = t.l1 = t.l2 = 0ull;
+}
+
+// one EJv contribution x of the current cell, in units 1/invU1
+__device__ __forceinline__ void acc_ejv(TallyAcc &t, double x, double invU1, unsigned long long *scnt) {
+    double q = rint(x * invU1);
+    if (!(q < 0x1p62)) {
+        if (!(q > 0.0)) return;
+        atomicAdd(&scnt[C_NOVERFLOW], 1ull);
+        q = 0x1p62;
+    }
+    if (q > 0.0) t.e1 += (unsigned long long)q;
+}
+
+// one EJv2 contribution x2 in units 1/invU2, split into 32-bit-weighted limbs
+__device__ __forceinline__ void acc_ejv2(TallyAcc &t, double x2, double invU2, unsigned long long *scnt) {
+    double q = x2 * invU2;
     if (!(q > 0.0)) return;
     double l2 = floor(q * 0x1p-64);
-    if (l2 > 0x1p31) { atomicAdd(&scnt[C_NOVERFLOW], 1ull); l2 = 0x1p31; }
+    if (l2 > 0x1p31) {
+        atomicAdd(&scnt[C_NOVERFLOW], 1ull);
+        l2 = 0x1p31;
+    }
     q = q - l2 * 0x1p64;
     double l1 = floor(q * 0x1p-32);
     double l0 = rint(q - l1 * 0x1p32);
-    unsigned long long *b = ty.EJv2 + 3 * (size_t)c;
-    if (l0 > 0.0) atomicAdd(b, (unsigned long long)l0);
-    if (l1 > 0.0) atomicAdd(b + 1, (unsigned long long)l1);
-    if (l2 > 0.0) atomicAdd(b + 2, (unsigned long long)l2);
+    if (l0 > 0.0) t.l0 += (unsigned long long)l0;
+    if (l1 > 0.0) t.l1 += (unsigned long long)l1;
+    if (l2 > 0.0) t.l2 += (unsigned long long)l2;
 }
+
+// An inverse-CDF lookup: the first bin b in [0, n) whose cumulative value
+// f(b) reaches the target t.  Kinds:
+//  L_SAME  f = ((R0[n+b] - R0[b]) ws) 1          reemission, T0 and T1 in one
+//                                                 integer bin (Radiation.f:187-213)
+//  L_DIFF  f = ((((e1 R1[n+b] + (1-e1) R1[b]) - e0 R0[n+b]) - (1-e0) R0[b]) ws) 1
+//                                                 reemission, four Planck rows
+//  L_SCAT  f = e1 R1[b] + e0 R0[b] > t             scattering angle, E F11 + Q F12
+//                                                 (Radiation.f:436-452)
+// R0, R1 point at cumulative rows; for L_SCAT R1 = C11, R0 = C12 of the
+// chosen Mueller matrix and e1 = E, e0 = Q.
+enum { L_SAME = 0, L_DIFF = 1, L_SCAT = 2 };
+
+struct Lookup {
+    const double *R0, *R1;
+    double e0, e1, ws, t;
+    int n, kind;
+};
+
+__device__ __forceinline__ double lk_eval(const Lookup &q, int b) {
+    if (q.kind == L_SCAT) return q.e1 * q.R1[b] + q.e0 * q.R0[b];
+    if (q.kind == L_SAME) return ((q.R0[q.n + b] - q.R0[b]) * q.ws) * 1.0;
+    const double a2 = 1.0 - q.e1, a4 = 1.0 - q.e0;
+    return ((((q.e1 * q.R1[q.n + b] + a2 * q.R1[b]) - q.e0 * q.R0[q.n + b]) - a4 * q.R0[b]) * q.ws) * 1.0;
+}
+
+__device__ __forceinline__ bool lk_hit(const Lookup &q, double v) { return (q.kind == L_SCAT) ? (v > q.t) : (v >= q.t); }
+
+// one lane alone: binary search (the CDFs are non-decreasing)
+__device__ __forceinline__ void lk_single(const Lookup &q, int &b, double &fb, double &fbm1) {
+    int lo = 0, hi = q.n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (lk_hit(q, lk_eval(q, mid))) hi = mid;
+        else lo = mid + 1;
+    }
+    b = lo;
+    fb = (lo < q.n) ? lk_eval(q, lo) : 0.0;
+    fbm1 = (lo > 0 && lo < q.n) ? lk_eval(q, lo - 1) : 0.0;
+}
+
+__device__ __forceinline__ const double *shfl_ptr(const double *p, int src) {
+    unsigned long long v = (unsigned long long)p;
+    unsigned lo = __shfl((unsigned)v, src), hi = __shfl((unsigned)(v >> 32), src);
+    return (const double *)(((unsigned long long)hi << 32) | lo);
+}
+
+// the whole wavefront serves lane src's lookup: every lane evaluates 4 bins
+// per round (all loads in flight together), a ballot finds the first hit.
+// Must be called by all 64 lanes.
+__device__ void lk_wave(const Lookup &mine, int src, int lane, int &b, double &fb, double &fbm1) {
+    Lookup q;
+    q.R0 = shfl_ptr(mine.R0, src);
+    q.R1 = shfl_ptr(mine.R1, src);
+    q.e0 = __shfl(mine.e0, src);
+    q.e1 = __shfl(mine.e1, src);
+    q.ws = __shfl(mine.ws, src);
+    q.t = __shfl(mine.t, src);
+    q.n = __shfl(mine.n, src);
+    q.kind = __shfl(mine.kind, src);
+    b = q.n;
+    fb = 0.0;
+    fbm1 = 0.0;
+    double prev_last = 0.0;
+    for (int base = 0; base < q.n; base += 256) {
+        double v[4];
+        bool h[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int bb = base + 64 * k + lane;
+            v[k] = (bb < q.n) ? lk_eval(q, bb) : 0.0;
+            h[k] = (bb < q.n) && lk_hit(q, v[k]);
+        }
+        bool done = false;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (done) break;
+            const unsigned long long m = __ballot(h[k]);
+            if (m) {
+                const int bl = __ffsll((long long)m) - 1;
+                b = base + 64 * k + bl;
+                fb = __shfl(v[k], bl);
+                double below = __shfl(v[k], (bl + 63) & 63);
+                double prevk = (k > 0) ? __shfl(v[(k + 3) & 3], 63) : prev_last;
+                fbm1 = (b == 0) ? 0.0 : ((bl > 0) ? below : prevk);
+                done = true;
+            }
+        }
+        if (done) break;
+        prev_last = __shfl(v[3], 63);
+    }
+}
+
+#define COOP_LANES 6 // default: a wave serves lookups together when at most this many lanes need one
 
 __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
     extern __shared__ double lds[];
@@ -345,8 +464,23 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
     g.init(A.seed, 0);
     long long pid = -1;
     int st = S_NEED;
+    if (A.serial && (blockIdx.x * blockDim.x + threadIdx.x) != 0) st = S_DONE;
     const long long raar_limit = (long long)nr * (long long)(nt + 1) * 2;
     const double AU = MCX_AU;
+
+    // cell cache: rho and the fixed-point units of cell cc, the opacity pair
+    // of (kc, kl); tallies of cell cc accumulate in registers
+    int cc = -1, kc = -1, kl = -1;
+    double c_dens = 0.0, c_u1 = 0.0, c_u2 = 0.0;
+    double2 k1 = make_double2(0.0, 0.0), k2 = make_double2(0.0, 0.0);
+    TallyAcc acc;
+    acc.c = -1;
+    acc.e1 = acc.l0 = acc.l1 = acc.l2 = 0ull;
+    // a pending inverse-CDF lookup and its result
+    Lookup lk;
+    memset(&lk, 0, sizeof(lk));
+    int lk_b = 0;
+    double lk_fb = 0.0, lk_fbm1 = 0.0;
 
     for (;;) {
         // ---- wave-aggregated packet acquisition -------------------------
@@ -375,7 +509,6 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
             }
         }
         if (__all(st == S_DONE)) break;
-        if (st == S_DONE) continue;
 
         // ---- stellar emission (MCRadiation.f:347-383, 500-513, 554) ------
         if (st == S_EMIT) {
@@ -426,13 +559,26 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
         // ---- one cell step of trace2d (MCRadiation.f:941-1138) ----------
         if (st == S_STEP) {
             const int c = p.i * nt + p.jj;
+            if (c != cc) {
+                const double nd = md.dens[c], n1 = ty.invU1[c], n2 = ty.invU2[c];
+                acc_flush(ty, acc);
+                cc = c;
+                acc.c = c;
+                c_dens = nd;
+                c_u1 = n1;
+                c_u2 = n2;
+            }
+            if (c != kc || p.l1 != kl) {
+                k1 = md.KT[(size_t)c * nlam + p.l1];
+                k2 = md.KT[(size_t)c * nlam + p.l1 + 1];
+                kc = c;
+                kl = p.l1;
+            }
             if (c != p.lastc) {
                 long long rel = pid - A.pid_base;
                 atomicOr(&ty.bitmap[(size_t)c * A.nwords + (size_t)(rel >> 5)], 1u << (rel & 31));
                 p.lastc = c;
             }
-            const double2 k1 = md.KT[(size_t)c * nlam + p.l1];
-            const double2 k2 = md.KT[(size_t)c * nlam + p.l1 + 1];
             const double wl2 = 1.0 - p.wl1;
             double Kabs = k1.x * p.wl1 + k2.x * wl2;
             double Kext = tb.scattering ? ((k1.x + k1.y) * p.wl1 + (k2.x + k2.y) * wl2) : Kabs;
@@ -445,7 +591,7 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                 st = S_NEED;
             } else {
                 CNT(C_NSTEPS);
-                double tau1 = ((Kext * v) * md.dens[c]) * AU;
+                double tau1 = ((Kext * v) * c_dens) * AU;
                 if ((p.tau + tau1) > p.tau0) {
                     v = (v * (p.tau0 - p.tau)) / tau1;
                     tautot += 2.0 * (p.tau0 - p.tau);
@@ -454,7 +600,7 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                     p.z = p.z + p.vz * v;
                     p.onEdge = false;
                     double ejv = ((p.E * v) * AU) * Kabs;
-                    tally_ejv(ty, c, ejv, scnt);
+                    acc_ejv(acc, ejv, c_u1, scnt);
                     p.nEJv = p.nEJv + ejv;
                     st = S_INTERACT;
                 } else {
@@ -463,9 +609,9 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                     p.z = p.z + p.vz * v;
                     double ejv = ((p.E * v) * AU) * Kabs;
                     p.nEJv = p.nEJv + ejv;
-                    tally_ejv2(ty, c, p.nEJv * p.nEJv, scnt);
+                    acc_ejv2(acc, p.nEJv * p.nEJv, c_u2, scnt);
                     p.nEJv = 0.0;
-                    tally_ejv(ty, c, ejv, scnt);
+                    acc_ejv(acc, ejv, c_u1, scnt);
                     p.onEdge = true;
                     p.tau = p.tau + tau1;
                     tautot += tau1;
@@ -487,13 +633,11 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
             }
         }
 
-        // ---- interaction (Radiation.f:41-355) ---------------------------
+        // ---- interaction up to its CDF lookup (Radiation.f:41-355) ------
         if (st == S_INTERACT) {
             CNT(C_NINTERACT);
             const int c = p.i * nt + p.jj;
             const int l1 = p.l1, l2 = p.l1 + 1;
-            const double2 k1 = md.KT[(size_t)c * nlam + l1];
-            const double2 k2 = md.KT[(size_t)c * nlam + l2];
             const double wl2 = 1.0 - p.wl1;
             double Ksca = k1.y * p.wl1 + k2.y * wl2;
             double Kext = Ksca + (k1.x * p.wl1 + k2.x * wl2);
@@ -510,12 +654,12 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                 p.U = 0.0;
                 p.V = 0.0;
                 double Ex = (p.E * (KabsL - 0.0)) / KabsL;
+                const double Thint = __longlong_as_double((long long)__hip_atomic_load(
+                    &ty.Tcell[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
                 unsigned long long q;
                 unsigned long long old = add_eabs(ty, c, Ex, A.invE, q, scnt);
                 const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
                 double mass = md.mass[c];
-                double Thint = __longlong_as_double((long long)__hip_atomic_load(
-                    &ty.Tcell[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
                 double T0 = 0.0;
                 if (old != 0ull) T0 = dev_increaseT(kpc, ((double)old * A.unitE) / mass, Thint);
                 double T1 = dev_increaseT(kpc, ((double)(old + q) * A.unitE) / mass, T0);
@@ -529,7 +673,7 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                 if (iT1 < 2) iT1 = 2;
                 const bool same = (iT0 == iT1);
                 const double *wc = md.w + (size_t)c * ng;
-                // coefficient of the four Planck rows of the difference spectrum
+                // coefficients of the four Planck rows of the difference spectrum
                 const double a1 = e1, a2 = 1.0 - e1, a3 = e0, a4 = 1.0 - e0;
                 // Planck-mean difference of species s, times its weight
                 auto kps = [&](int s) {
@@ -554,15 +698,15 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                 int s = 0;
                 double Lrs = Lr;
                 if (ng > 1) {
-                    double acc = 0.0;
+                    double accs = 0.0;
                     int last = -1;
                     s = -1;
                     for (int k = 0; k < ng; k++) {
                         double kk = kps(k);
                         if (kk > 0.0) last = k;
-                        double prev = acc;
-                        acc = acc + kk;
-                        if (kk > 0.0 && acc >= Lr) {
+                        double prev = accs;
+                        accs = accs + kk;
+                        if (kk > 0.0 && accs >= Lr) {
                             s = k;
                             Lrs = Lr - prev;
                             break;
@@ -573,24 +717,22 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                         Lrs = 2.0 * kps(s) + 1.0; // beyond the species total: top bin rule
                     }
                 }
-                const double ws = wc[s];
                 const double *Cs = tb.CUM + (size_t)s * (TMAXD + 1) * nbins;
-                const double *r0 = Cs + (size_t)iT0 * nbins, *r0p = Cs + (size_t)(iT0 + 1) * nbins;
-                const double *r1 = Cs + (size_t)iT1 * nbins, *r1p = Cs + (size_t)(iT1 + 1) * nbins;
-                if (same) {
-                    pick_lambda(nbins, Lrs, [&](int b) { return ((r0p[b] - r0[b]) * ws) * 1.0; }, p.l1, p.wl1);
-                } else {
-                    pick_lambda(nbins, Lrs,
-                                [&](int b) {
-                                    return ((((a1 * r1p[b] + a2 * r1[b]) - a3 * r0p[b]) - a4 * r0[b]) * ws) * 1.0;
-                                },
-                                p.l1, p.wl1);
-                }
+                lk.R0 = Cs + (size_t)iT0 * nbins;
+                lk.R1 = Cs + (size_t)iT1 * nbins;
+                lk.e0 = e0;
+                lk.e1 = e1;
+                lk.ws = wc[s];
+                lk.t = Lrs;
+                lk.n = nbins;
+                lk.kind = same ? L_SAME : L_DIFF;
+                st = S_LOOKUP;
             } else {
                 CNT(C_NSCAT);
                 p.scatt = true;
                 if (tb.scat_how == 1) {
                     random_direction(g, p.vx, p.vy, p.vz);
+                    st = S_AFTER;
                 } else {
                     // pick the (species, lambda) Mueller matrix (Radiation.f:273-290)
                     double KscaR = Ksca * g.next();
@@ -605,8 +747,6 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                         KscaR = KscaR - w1;
                         if (KscaR < 0.0) { moff = (size_t)s * nlam + l1; break; }
                     }
-                    const double *M = tb.F + moff * 6 * 180;
-                    const double *C11 = tb.C11 + moff * 180, *C12 = tb.C12 + moff * 180;
                     const double MIF11 = tb.IF11[moff], MIF12 = tb.IF12[moff];
                     // scatangle (Radiation.f:381-495)
                     double IF11 = MIF11 * 180.0, IF12 = MIF12 * 180.0;
@@ -655,56 +795,106 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                     p.Sz = sz;
                     double Fr = 180.0 * (p.E * MIF11 + p.Q * MIF12);
                     Fr = g.next() * Fr;
-                    // first angle bin whose cumulative exceeds Fr
-                    const double E = p.E, Qs = p.Q;
-                    int lo = 0, hi = 180;
-                    while (lo < hi) {
-                        int mid = (lo + hi) >> 1;
-                        if (E * C11[mid] + Qs * C12[mid] > Fr) hi = mid;
-                        else lo = mid + 1;
-                    }
-                    double theta;
-                    int ia;
-                    if (lo >= 180) {
-                        theta = MCX_PI;
-                        ia = 180;
-                    } else {
-                        double Fi = E * C11[lo] + Qs * C12[lo];
-                        double FiOld = (lo > 0) ? (E * C11[lo - 1] + Qs * C12[lo - 1]) : 0.0;
-                        ia = lo + 1;
-                        theta = (double)ia - (Fi - Fr) / (Fi - FiOld);
-                        theta = (theta * MCX_PI) / 180.0;
-                    }
-                    if (g.next() < 0.5) theta = -theta;
-                    const double *F11 = M, *F12 = M + 180, *F22 = M + 360, *F33 = M + 540, *F34 = M + 720,
-                                 *F44 = M + 900;
-                    double En = 0.0, Qn = 0.0, Un = 0.0, Vn = 0.0;
-                    En = En + F11[ia - 1] * p.E;
-                    En = En + F12[ia - 1] * p.Q;
-                    Qn = Qn + F12[ia - 1] * p.E;
-                    Qn = Qn + F22[ia - 1] * p.Q;
-                    Un = Un + F33[ia - 1] * p.U;
-                    double Fs = (theta >= 0.0) ? fabs(F34[ia - 1]) : -fabs(F34[ia - 1]);
-                    Un = Un + Fs * p.V;
-                    Vn = Vn - Fs * p.U;
-                    Vn = Vn + F44[ia - 1] * p.V;
-                    if (En != 0.0) {
-                        p.Q = (p.E * Qn) / En;
-                        p.U = (p.E * Un) / En;
-                        p.V = (p.E * Vn) / En;
-                    }
-                    rotate_about(p.vx, p.vy, p.vz, p.Sx, p.Sy, p.Sz, theta);
-                    double r = sqrt(p.vx * p.vx + p.vy * p.vy + p.vz * p.vz);
-                    p.vx /= r;
-                    p.vy /= r;
-                    p.vz /= r;
-                    r = sqrt(p.Sx * p.Sx + p.Sy * p.Sy + p.Sz * p.Sz);
-                    p.Sx /= r;
-                    p.Sy /= r;
-                    p.Sz /= r;
-                    if (ia != 1 && ia != 180) p.pol = true;
+                    // first angle bin whose cumulative E F11 + Q F12 exceeds Fr
+                    lk.R1 = tb.C11 + moff * 180;
+                    lk.R0 = tb.C12 + moff * 180;
+                    lk.e1 = p.E;
+                    lk.e0 = p.Q;
+                    lk.ws = 1.0;
+                    lk.t = Fr;
+                    lk.n = 180;
+                    lk.kind = L_SCAT;
+                    st = S_LOOKUP;
                 }
             }
+        }
+
+        // ---- inverse-CDF lookups: a wave whose few lanes need one serves
+        //      them together, one lookup at a time; otherwise each lane
+        //      searches alone.  Both give the first bin that reaches the
+        //      target.  All lanes of the wave reach this point.
+        {
+            const unsigned long long sm = __ballot(st == S_LOOKUP);
+            if (sm) {
+                if (__popcll(sm) <= A.coop) {
+                    unsigned long long rem = sm;
+                    while (rem) {
+                        const int src = __ffsll((long long)rem) - 1;
+                        rem &= rem - 1;
+                        int b;
+                        double fb, fbm1;
+                        lk_wave(lk, src, lane, b, fb, fbm1);
+                        if (lane == src) {
+                            lk_b = b;
+                            lk_fb = fb;
+                            lk_fbm1 = fbm1;
+                        }
+                    }
+                } else if (st == S_LOOKUP) {
+                    lk_single(lk, lk_b, lk_fb, lk_fbm1);
+                }
+            }
+        }
+
+        // ---- the interaction after its lookup ----------------------------
+        if (st == S_LOOKUP) {
+            if (lk.kind != L_SCAT) {
+                // emit()'s wavelength rule (Radiation.f:966-989)
+                if (lk_b >= nbins) {
+                    p.l1 = nbins - 1;
+                    p.wl1 = 0.0;
+                } else {
+                    p.wl1 = (lk.t - lk_fbm1) / (lk_fb - lk_fbm1);
+                    p.l1 = lk_b;
+                }
+            } else {
+                const double Fr = lk.t;
+                double theta;
+                int ia;
+                if (lk_b >= 180) {
+                    theta = MCX_PI;
+                    ia = 180;
+                } else {
+                    ia = lk_b + 1;
+                    theta = (double)ia - (lk_fb - Fr) / (lk_fb - lk_fbm1);
+                    theta = (theta * MCX_PI) / 180.0;
+                }
+                if (g.next() < 0.5) theta = -theta;
+                const size_t moff = (size_t)(lk.R1 - tb.C11) / 180;
+                const double *M = tb.F + moff * 6 * 180;
+                const double *F11 = M, *F12 = M + 180, *F22 = M + 360, *F33 = M + 540, *F34 = M + 720,
+                             *F44 = M + 900;
+                double En = 0.0, Qn = 0.0, Un = 0.0, Vn = 0.0;
+                En = En + F11[ia - 1] * p.E;
+                En = En + F12[ia - 1] * p.Q;
+                Qn = Qn + F12[ia - 1] * p.E;
+                Qn = Qn + F22[ia - 1] * p.Q;
+                Un = Un + F33[ia - 1] * p.U;
+                double Fs = (theta >= 0.0) ? fabs(F34[ia - 1]) : -fabs(F34[ia - 1]);
+                Un = Un + Fs * p.V;
+                Vn = Vn - Fs * p.U;
+                Vn = Vn + F44[ia - 1] * p.V;
+                if (En != 0.0) {
+                    p.Q = (p.E * Qn) / En;
+                    p.U = (p.E * Un) / En;
+                    p.V = (p.E * Vn) / En;
+                }
+                rotate_about(p.vx, p.vy, p.vz, p.Sx, p.Sy, p.Sz, theta);
+                double r = sqrt(p.vx * p.vx + p.vy * p.vy + p.vz * p.vz);
+                p.vx /= r;
+                p.vy /= r;
+                p.vz /= r;
+                r = sqrt(p.Sx * p.Sx + p.Sy * p.Sy + p.Sz * p.Sz);
+                p.Sx /= r;
+                p.Sy /= r;
+                p.Sz /= r;
+                if (ia != 1 && ia != 180) p.pol = true;
+            }
+            st = S_AFTER;
+        }
+
+        // ---- after every interaction (MCRadiation.f:567-573) -------------
+        if (st == S_AFTER) {
             st = S_STEP;
             if (A.maxinteract > 0) {
                 if (g.next() < (1.0 / (double)(float)A.maxinteract)) {
@@ -719,7 +909,7 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                 p.tau = 0.0;
                 p.ntrace = 0;
                 // trace2d starts with the random walk in thick cells (MCRadiation.f:916)
-                if (A.rw.on && A.rw.thick[c]) st = S_RW;
+                if (A.rw.on && A.rw.thick[p.i * nt + p.jj]) st = S_RW;
             }
         }
 
@@ -745,8 +935,7 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
             if (iT == 0) iT = 1;
             if (iT > TMAXD - 1) iT = TMAXD - 1;
             const double2 kd = rw.KD[(size_t)rw.cls[c] * (TMAXD + 1) + iT];
-            const double dens = md.dens[c];
-            const double lr = 1.0 / ((AU * kd.x) * dens);
+            const double lr = 1.0 / ((AU * kd.x) * c_dens);
             if (dmin <= rw.factRW * lr) {
                 st = S_STEP;
             } else {
@@ -759,8 +948,8 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                 p.z = p.z + dmin * p.vz;
                 const double ejv = ((p.E * v) * AU) * kd.y;
                 unsigned long long q;
-                const unsigned long long old = add_eabs(ty, c, ejv * dens, A.invE, q, scnt);
-                tally_ejv(ty, c, ejv, scnt);
+                const unsigned long long old = add_eabs(ty, c, ejv * c_dens, A.invE, q, scnt);
+                acc_ejv(acc, ejv, c_u1, scnt);
                 p.nEJv = p.nEJv + ejv; // the module-level nEJv of RandomWalk.f:115
                 const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
                 const double T1 = dev_increaseT(kpc, ((double)(old + q) * A.unitE) / md.mass[c], Thint);
@@ -797,6 +986,7 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
             st = S_NEED;
         }
     }
+    acc_flush(ty, acc);
 
     // ---- counters: one global atomic per block -------------------------
     if (tautot != 0.0) atomicAdd(&stau, tautot);
@@ -1434,7 +1624,7 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     int threads = 256, blocks = 1;
     size_t lds = sizeof(double) * ((size_t)(md.nr + 1) + 2 * (size_t)(md.nt + 1) + 2 * 361);
     if (run->nlanes == 1) {
-        threads = 1;
+        threads = 64; // one wavefront: lane 0 traces, the others serve its lookups
         blocks = 1;
     } else {
         int per_cu = 0;
@@ -1461,6 +1651,8 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     A.invE = ldexp(1.0, EABS_BITS) / E0;
     A.unitE = ctx->unitE;
     A.nwords = nwords;
+    A.serial = (run->nlanes == 1);
+    A.coop = (run->coop_lanes < 0) ? COOP_LANES : run->coop_lanes;
     float kms_total = 0.f;
     for (long long base = 0; base < run->nphot; base += batch) {
         long long end = base + batch;

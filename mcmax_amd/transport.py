@@ -71,14 +71,15 @@ class Transport:
         self.model = m
 
     def transport(self, nphot, seed=1, nlanes=0, maxinteract=0, bitmap_bytes_max=0, randomwalk=None,
-                  factRW=None):
+                  factRW=None, coop_lanes=-1):
         """One MC pass (mcx_transport); tallies stay on the device.  The
         random-walk switches default to the model's keywords."""
         m = self.model
         rw = bool(getattr(m, "randomwalk", False)) if randomwalk is None else bool(randomwalk)
         f = float(getattr(m, "factRW", 10.0)) if factRW is None else float(factRW)
         r = _lib.Run(nphot=int(nphot), seed=int(seed), maxinteract=int(maxinteract), nlanes=int(nlanes),
-                     bitmap_bytes_max=int(bitmap_bytes_max), randomwalk=int(rw), factRW=f)
+                     bitmap_bytes_max=int(bitmap_bytes_max), randomwalk=int(rw), factRW=f,
+                     coop_lanes=int(coop_lanes))
         rc = self.lib.mcx_transport(self.ctx, C.byref(r))
         if rc not in (0, _lib.MCX_EGEOM):
             self._check(rc)
@@ -121,10 +122,10 @@ class Transport:
         return k.value, p.value
 
 
-def mc_radiation(tr, nphot, seed=1, nlanes=0, maxinteract=0):
+def mc_radiation(tr, nphot, seed=1, nlanes=0, maxinteract=0, coop_lanes=-1):
     """MCRadiation + temperature inversion; returns a dict with the tallies
     and T, dT, dEJv."""
-    rc = tr.transport(nphot, seed=seed, nlanes=nlanes, maxinteract=maxinteract)
+    rc = tr.transport(nphot, seed=seed, nlanes=nlanes, maxinteract=maxinteract, coop_lanes=coop_lanes)
     out = tr.tallies()
     out["T"], out["dT"], out["dEJv"] = tr.temperature()
     out["rc"] = rc

@@ -29,9 +29,9 @@ def fixed_point_units(m, nphot):
     return B * 2.0 ** -31, B * B * 2.0 ** -40
 
 
-def _serial_compare(tr, m, oracle_lib, n, seed, rtol=1e-6):
+def _serial_compare(tr, m, oracle_lib, n, seed, rtol=1e-6, coop_lanes=-1):
     tr.set_model(m)
-    g = transport.mc_radiation(tr, n, seed=seed, nlanes=1)
+    g = transport.mc_radiation(tr, n, seed=seed, nlanes=1, coop_lanes=coop_lanes)
     orc = oracle_lib.Oracle(m)
     o = orc.mcradiation(n, rng="philox", seed=seed)
     assert g["noverflow"] == 0 and g["ngeomfail"] == 0
@@ -59,6 +59,10 @@ def _serial_compare(tr, m, oracle_lib, n, seed, rtol=1e-6):
     off[:, -1] = False
     bound2 = 0.5 * u2 * o["nev2"] + 1e-9 * np.abs(o["EJv2"]) + 1e-300
     d2 = np.abs(g["EJv2"] - o["EJv2"])
+    bad = np.argwhere(off & (d2 > bound2))
+    for i, j in bad[:8]:
+        print("EJv2 cell", i, j, "gpu", g["EJv2"][i, j], "oracle", o["EJv2"][i, j], "nev2", o["nev2"][i, j],
+              "EJv", g["EJv"][i, j], o["EJv"][i, j], "u2", u2[i, j])
     assert np.all(d2[off] <= bound2[off]), np.max(d2[off] / bound2[off])
     mid = o["EJv2"][:, -1] > 0
     ratio = g["EJv2"][:, -1][mid] / o["EJv2"][:, -1][mid]
@@ -93,6 +97,15 @@ def test_serial_parity_tiny(gpu, tiny_model, oracle_lib):
 
 def test_serial_parity_small(gpu, small_model, oracle_lib):
     _serial_compare(gpu, small_model, oracle_lib, 150, seed=17)
+
+
+def test_serial_parity_lane_lookups(gpu, small_model, oracle_lib):
+    """The per-lane binary-search lookups (coop_lanes=0) follow the same
+    histories as the wavefront-cooperative ones used by a lone lane."""
+    g, _ = _serial_compare(gpu, small_model, oracle_lib, 150, seed=17, coop_lanes=0)
+    h = transport.mc_radiation(gpu, 150, seed=17, nlanes=1)
+    for k in ("EJv", "EJv2", "Eabs", "Tbw", "Ni", "ispec"):
+        assert np.array_equal(g[k], h[k]), k
 
 
 @pytest.fixture(scope="module")
