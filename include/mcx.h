@@ -85,7 +85,12 @@ typedef struct {
     uint64_t seed;                   /* Philox4x32-10 key; packet p uses the
                                         counter stream (block, p)           */
     int32_t maxinteract;             /* >0: roulette, MCRadiation.f:568-571 */
-    int32_t nlanes;                  /* 0: fill the device; 1: a single lane
+    int32_t nlanes;                  /* packets in flight at once.  0: the
+                                        device's resident lanes, at most
+                                        nphot/64 (Bjorkman-Wood slices are
+                                        taken in arrival order; below ~2% in
+                                        flight the temperatures keep the
+                                        serial statistics); 1: a single lane
                                         (packets in order, as the serial
                                         reference loop MCRadiation.f:320)   */
     int64_t bitmap_bytes_max;        /* cap for the distinct-packet bitmap

@@ -176,6 +176,36 @@ __device__ double dev_increaseT(const double *__restrict__ kpc, double E1, doubl
     return dev_determineT(kpc, E1, Tstart);
 }
 
+// increaseT with entries jw..jw+2 of the cell's table already in registers
+// (w0, w1, w2): the same walk and result, without the loads it can skip.
+__device__ __forceinline__ double win_at(const double *__restrict__ kpc, int jw, double w0, double w1, double w2,
+                                         int k) {
+    if (k == jw) return w0;
+    if (k == jw + 1) return w1;
+    if (k == jw + 2) return w2;
+    return kpc[k];
+}
+
+__device__ double dev_increaseT_w(const double *__restrict__ kpc, int jw, double w0, double w1, double w2, double E1,
+                                  double Tstart) {
+    int j = (int)(Tstart / 1.0);
+    if (j > TMAXD) j = TMAXD;
+    if (j < 1) j = 1;
+    double kp0 = win_at(kpc, jw, w0, w1, w2, j);
+    if (kp0 <= E1) {
+        for (int i = j; i <= TMAXD - 1; i++) {
+            double kp1 = win_at(kpc, jw, w0, w1, w2, i + 1);
+            if (kp0 <= E1 && kp1 >= E1) {
+                float a4 = fpow4(i), b4 = fpow4(i + 1);
+                float d4 = b4 - a4;
+                return pow((double)a4 + ((double)d4 * (E1 - kp0)) / (kp1 - kp0), 0.25);
+            }
+            kp0 = kp1;
+        }
+    }
+    return dev_determineT(kpc, E1, Tstart);
+}
+
 // Radiation.f:1007-1024
 __device__ __forceinline__ void random_direction(PhiloxStream &g, double &x, double &y, double &z) {
     double r;
@@ -220,8 +250,11 @@ __device__ __forceinline__ void pick_lambda(int nbins, double Lr, F cum, int &l1
     l1 = b;
 }
 
+// Packet state in registers.  The Stokes vector (Q, U, V) and the
+// polarisation reference direction S, which only emission and scattering
+// touch, live in LDS (field-major, one slot per thread; see P_Q .. P_SZ).
 struct Packet {
-    double x, y, z, vx, vy, vz, E, wl1, Q, U, V, Sx, Sy, Sz;
+    double x, y, z, vx, vy, vz, E, wl1;
     double tau0, tau, nEJv;
     int i, jj, l1, edgeNr, ntrace, lastc;
     bool onEdge, scatt, pol;
@@ -423,9 +456,10 @@ __device__ void lk_wave(const Lookup &mine, int src, int lane, int &b, double &f
     }
 }
 
+#define INFLIGHT_DIV 64 // default cap on packets in flight: nphot / 64
 #define COOP_LANES 6 // default: a wave serves lookups together when at most this many lanes need one
 
-__global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
+__global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     extern __shared__ double lds[];
     const DevTables &tb = A.tb;
     const DevModel &md = A.md;
@@ -437,6 +471,15 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
     double *sTh = sT2 + (nt + 1);
     double *sC2 = sTh + (nt + 1);
     double *sS2 = sC2 + 361;
+    // per-thread Stokes vector and S direction, field-major
+    double *pl = sS2 + 361 + threadIdx.x;
+    const int PB = blockDim.x;
+#define P_Q pl[0 * PB]
+#define P_U pl[1 * PB]
+#define P_V pl[2 * PB]
+#define P_SX pl[3 * PB]
+#define P_SY pl[4 * PB]
+#define P_SZ pl[5 * PB]
     for (int k = threadIdx.x; k <= nr; k += blockDim.x) sR2[k] = md.R2[k];
     for (int k = threadIdx.x; k <= nt; k += blockDim.x) {
         sT2[k] = md.T2[k];
@@ -472,6 +515,7 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
     // of (kc, kl); tallies of cell cc accumulate in registers
     int cc = -1, kc = -1, kl = -1;
     double c_dens = 0.0, c_u1 = 0.0, c_u2 = 0.0;
+    double c_Tl = 0.0; // BW T after this lane's last absorption in cell cc (0: none yet)
     double2 k1 = make_double2(0.0, 0.0), k2 = make_double2(0.0, 0.0);
     TallyAcc acc;
     acc.c = -1;
@@ -539,15 +583,15 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                 }
             p.pol = false;
             p.scatt = false;
-            p.Q = 0.0;
-            p.U = 0.0;
-            p.V = 0.0;
+            P_Q = 0.0;
+            P_U = 0.0;
+            P_V = 0.0;
             {
                 double sx = -p.vy, sy = p.vx, sz = 0.0;
                 double r = sqrt(sx * sx + sy * sy + sz * sz);
-                p.Sx = sx / r;
-                p.Sy = sy / r;
-                p.Sz = sz / r;
+                P_SX = sx / r;
+                P_SY = sy / r;
+                P_SZ = sz / r;
             }
             p.E = p.E * 1.0;
             p.tau0 = -log(g.next());
@@ -559,18 +603,36 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
         // ---- one cell step of trace2d (MCRadiation.f:941-1138) ----------
         if (st == S_STEP) {
             const int c = p.i * nt + p.jj;
-            if (c != cc) {
-                const double nd = md.dens[c], n1 = ty.invU1[c], n2 = ty.invU2[c];
+            // issue the loads of a new cell first; atomics only after them, so
+            // that the loads do not wait behind the atomics' completion
+            const bool newc = (c != cc), newk = (c != kc || p.l1 != kl);
+            double nd = 0.0, n1 = 0.0, n2 = 0.0;
+            double2 nk1 = k1, nk2 = k2;
+            if (newc) {
+                nd = md.dens[c];
+                n1 = ty.invU1[c];
+                n2 = ty.invU2[c];
+            }
+            if (newk) {
+                nk1 = md.KT[(size_t)c * nlam + p.l1];
+                nk2 = md.KT[(size_t)c * nlam + p.l1 + 1];
+            }
+            double v;
+            int inext, jnext;
+            bool ok = exit_face(sR2, sT2, nt, p.x, p.y, p.z, p.vx, p.vy, p.vz, p.i, p.jj, p.onEdge, p.edgeNr, v,
+                                inext, jnext);
+            if (newc) {
                 acc_flush(ty, acc);
                 cc = c;
                 acc.c = c;
+                c_Tl = 0.0;
                 c_dens = nd;
                 c_u1 = n1;
                 c_u2 = n2;
             }
-            if (c != kc || p.l1 != kl) {
-                k1 = md.KT[(size_t)c * nlam + p.l1];
-                k2 = md.KT[(size_t)c * nlam + p.l1 + 1];
+            if (newk) {
+                k1 = nk1;
+                k2 = nk2;
                 kc = c;
                 kl = p.l1;
             }
@@ -582,10 +644,6 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
             const double wl2 = 1.0 - p.wl1;
             double Kabs = k1.x * p.wl1 + k2.x * wl2;
             double Kext = tb.scattering ? ((k1.x + k1.y) * p.wl1 + (k2.x + k2.y) * wl2) : Kabs;
-            double v;
-            int inext, jnext;
-            bool ok = exit_face(sR2, sT2, nt, p.x, p.y, p.z, p.vx, p.vy, p.vz, p.i, p.jj, p.onEdge, p.edgeNr, v,
-                                inext, jnext);
             if (!ok) {
                 CNT(C_NGEOM);
                 st = S_NEED;
@@ -650,19 +708,50 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                 CNT(C_NABS);
                 p.scatt = false;
                 p.pol = false;
-                p.Q = 0.0;
-                p.U = 0.0;
-                p.V = 0.0;
+                P_Q = 0.0;
+                P_U = 0.0;
+                P_V = 0.0;
                 double Ex = (p.E * (KabsL - 0.0)) / KabsL;
+                const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
+                const double *Kp0 = tb.Kp; // species 0
+                // The cell's T after this lane's previous absorption in it is
+                // the likely start: fetch that window of the tables together
+                // with the cell's energy (one round trip instead of several).
+                int jw = (c_Tl >= 1.0) ? (int)(c_Tl / 1.0) : -1;
+                if (jw > TMAXD - 2) jw = TMAXD - 2;
+                double w0 = 0.0, w1 = 0.0, w2 = 0.0, q0 = 0.0, q1 = 0.0, q2 = 0.0;
+                if (jw >= 1) {
+                    w0 = kpc[jw];
+                    w1 = kpc[jw + 1];
+                    w2 = kpc[jw + 2];
+                    if (ng == 1) {
+                        q0 = Kp0[jw];
+                        q1 = Kp0[jw + 1];
+                        q2 = Kp0[jw + 2];
+                    }
+                }
+                const double mass = md.mass[c];
                 const double Thint = __longlong_as_double((long long)__hip_atomic_load(
                     &ty.Tcell[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
                 unsigned long long q;
                 unsigned long long old = add_eabs(ty, c, Ex, A.invE, q, scnt);
-                const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
-                double mass = md.mass[c];
+                if (jw < 1) {
+                    jw = (int)(Thint / 1.0);
+                    if (jw < 1) jw = 1;
+                    if (jw > TMAXD - 2) jw = TMAXD - 2;
+                    w0 = kpc[jw];
+                    w1 = kpc[jw + 1];
+                    w2 = kpc[jw + 2];
+                    if (ng == 1) {
+                        q0 = Kp0[jw];
+                        q1 = Kp0[jw + 1];
+                        q2 = Kp0[jw + 2];
+                    }
+                }
                 double T0 = 0.0;
-                if (old != 0ull) T0 = dev_increaseT(kpc, ((double)old * A.unitE) / mass, Thint);
-                double T1 = dev_increaseT(kpc, ((double)(old + q) * A.unitE) / mass, T0);
+                if (old != 0ull) T0 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)old * A.unitE) / mass, Thint);
+                double T1 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)(old + q) * A.unitE) / mass, T0);
+                c_Tl = T1;
                 atomicMax(&ty.Tcell[c], (unsigned long long)__double_as_longlong(T1));
                 int iT0 = (int)(T0 / 1.0), iT1 = (int)(T1 / 1.0);
                 double e0 = T0 - (double)(float)iT0;
@@ -678,8 +767,19 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                 // Planck-mean difference of species s, times its weight
                 auto kps = [&](int s) {
                     const double *Kp = tb.Kp + (size_t)s * (TMAXD + 1);
-                    double d = same ? (Kp[iT0 + 1] - Kp[iT0])
-                                    : (((a1 * Kp[iT1 + 1] + a2 * Kp[iT1]) - a3 * Kp[iT0 + 1]) - a4 * Kp[iT0]);
+                    double k00, k01, k10, k11;
+                    if (ng == 1) {
+                        k00 = win_at(Kp, jw, q0, q1, q2, iT0);
+                        k01 = win_at(Kp, jw, q0, q1, q2, iT0 + 1);
+                        k10 = win_at(Kp, jw, q0, q1, q2, iT1);
+                        k11 = win_at(Kp, jw, q0, q1, q2, iT1 + 1);
+                    } else {
+                        k00 = Kp[iT0];
+                        k01 = Kp[iT0 + 1];
+                        k10 = Kp[iT1];
+                        k11 = Kp[iT1 + 1];
+                    }
+                    double d = same ? (k01 - k00) : (((a1 * k11 + a2 * k10) - a3 * k01) - a4 * k00);
                     return (d * wc[s]) * 1.0;
                 };
                 double kp = 0.0;
@@ -690,9 +790,9 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                 {
                     double sx = -p.vy, sy = p.vx, sz = 0.0;
                     double r = sqrt(sx * sx + sy * sy + sz * sz);
-                    p.Sx = sx / r;
-                    p.Sy = sy / r;
-                    p.Sz = sz / r;
+                    P_SX = sx / r;
+                    P_SY = sy / r;
+                    P_SZ = sz / r;
                 }
                 double Lr = g.next() * kp;
                 int s = 0;
@@ -754,12 +854,12 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                     if (p.pol) {
                         double Itot = 0.0;
                         for (int k = 1; k <= 180; k++)
-                            Itot = Itot + IF11 * p.E + IF12 * (p.Q * sC2[k] + p.U * sS2[k]);
+                            Itot = Itot + IF11 * p.E + IF12 * (P_Q * sC2[k] + P_U * sS2[k]);
                         Itot = g.next() * Itot;
                         double I1 = 0.0, I0 = 0.0;
                         phi = MCX_PI;
                         for (int k = 1; k <= 180; k++) {
-                            I0 = I0 + IF11 * p.E + IF12 * (p.Q * sC2[k] + p.U * sS2[k]);
+                            I0 = I0 + IF11 * p.E + IF12 * (P_Q * sC2[k] + P_U * sS2[k]);
                             if (I0 > Itot) {
                                 double wp1 = (I0 - Itot) / (I0 - I1);
                                 phi = (MCX_PI * ((double)k - wp1)) / 180.0;
@@ -770,36 +870,36 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                     } else {
                         phi = g.next() * MCX_PI;
                     }
-                    double sx = p.Sx, sy = p.Sy, sz = p.Sz;
+                    double sx = P_SX, sy = P_SY, sz = P_SZ;
                     rotate_about(sx, sy, sz, p.vx, p.vy, p.vz, phi);
                     if (p.pol) {
                         // RotateStokes (Radiation.f:506-534)
-                        double P = (p.U / p.E) * (p.U / p.E) + (p.Q / p.E) * (p.Q / p.E);
+                        double P = (P_U / p.E) * (P_U / p.E) + (P_Q / p.E) * (P_Q / p.E);
                         if (P != 0.0) {
-                            double cost = p.Sx * sx + p.Sy * sy + p.Sz * sz;
-                            double sint = (sx - (p.Sx * (p.vy * p.vy + p.vz * p.vz) -
-                                                 p.vx * (p.vy * p.Sy + p.vz * p.Sz)) * cost) /
-                                          (p.vy * p.Sz - p.vz * p.Sy);
+                            double cost = P_SX * sx + P_SY * sy + P_SZ * sz;
+                            double sint = (sx - (P_SX * (p.vy * p.vy + p.vz * p.vz) -
+                                                 p.vx * (p.vy * P_SY + p.vz * P_SZ)) * cost) /
+                                          (p.vy * P_SZ - p.vz * P_SY);
                             double cos2t = cost * cost - sint * sint;
                             double sin2t = (2.0 * sint) * cost;
-                            double Qn = p.Q * cos2t + p.U * sin2t;
-                            double Un = -p.Q * sin2t + p.U * cos2t;
+                            double Qn = P_Q * cos2t + P_U * sin2t;
+                            double Un = -P_Q * sin2t + P_U * cos2t;
                             double P2 = (Un / p.E) * (Un / p.E) + (Qn / p.E) * (Qn / p.E);
                             double sPP2 = sqrt(P / P2);
-                            p.U = sPP2 * Un;
-                            p.Q = sPP2 * Qn;
+                            P_U = sPP2 * Un;
+                            P_Q = sPP2 * Qn;
                         }
                     }
-                    p.Sx = sx;
-                    p.Sy = sy;
-                    p.Sz = sz;
-                    double Fr = 180.0 * (p.E * MIF11 + p.Q * MIF12);
+                    P_SX = sx;
+                    P_SY = sy;
+                    P_SZ = sz;
+                    double Fr = 180.0 * (p.E * MIF11 + P_Q * MIF12);
                     Fr = g.next() * Fr;
                     // first angle bin whose cumulative E F11 + Q F12 exceeds Fr
                     lk.R1 = tb.C11 + moff * 180;
                     lk.R0 = tb.C12 + moff * 180;
                     lk.e1 = p.E;
-                    lk.e0 = p.Q;
+                    lk.e0 = P_Q;
                     lk.ws = 1.0;
                     lk.t = Fr;
                     lk.n = 180;
@@ -866,28 +966,28 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
                              *F44 = M + 900;
                 double En = 0.0, Qn = 0.0, Un = 0.0, Vn = 0.0;
                 En = En + F11[ia - 1] * p.E;
-                En = En + F12[ia - 1] * p.Q;
+                En = En + F12[ia - 1] * P_Q;
                 Qn = Qn + F12[ia - 1] * p.E;
-                Qn = Qn + F22[ia - 1] * p.Q;
-                Un = Un + F33[ia - 1] * p.U;
+                Qn = Qn + F22[ia - 1] * P_Q;
+                Un = Un + F33[ia - 1] * P_U;
                 double Fs = (theta >= 0.0) ? fabs(F34[ia - 1]) : -fabs(F34[ia - 1]);
-                Un = Un + Fs * p.V;
-                Vn = Vn - Fs * p.U;
-                Vn = Vn + F44[ia - 1] * p.V;
+                Un = Un + Fs * P_V;
+                Vn = Vn - Fs * P_U;
+                Vn = Vn + F44[ia - 1] * P_V;
                 if (En != 0.0) {
-                    p.Q = (p.E * Qn) / En;
-                    p.U = (p.E * Un) / En;
-                    p.V = (p.E * Vn) / En;
+                    P_Q = (p.E * Qn) / En;
+                    P_U = (p.E * Un) / En;
+                    P_V = (p.E * Vn) / En;
                 }
-                rotate_about(p.vx, p.vy, p.vz, p.Sx, p.Sy, p.Sz, theta);
+                rotate_about(p.vx, p.vy, p.vz, P_SX, P_SY, P_SZ, theta);
                 double r = sqrt(p.vx * p.vx + p.vy * p.vy + p.vz * p.vz);
                 p.vx /= r;
                 p.vy /= r;
                 p.vz /= r;
-                r = sqrt(p.Sx * p.Sx + p.Sy * p.Sy + p.Sz * p.Sz);
-                p.Sx /= r;
-                p.Sy /= r;
-                p.Sz /= r;
+                r = sqrt(P_SX * P_SX + P_SY * P_SY + P_SZ * P_SZ);
+                P_SX /= r;
+                P_SY /= r;
+                P_SZ /= r;
                 if (ia != 1 && ia != 180) p.pol = true;
             }
             st = S_AFTER;
@@ -995,6 +1095,12 @@ __global__ void __launch_bounds__(256) k_transport(TransportArgs A) {
         if (scnt[k]) atomicAdd(&ty.counters[k], scnt[k]);
     if (threadIdx.x == 0 && stau != 0.0) atomicAdd(ty.tautot, stau);
 #undef CNT
+#undef P_Q
+#undef P_U
+#undef P_V
+#undef P_SX
+#undef P_SY
+#undef P_SZ
 }
 
 // ---------------------------------------------------------------------------
@@ -1622,21 +1728,23 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     HIPCHK(ctx, hipGetLastError());
     // launch shape: a full device of resident wavefronts, or one lane
     int threads = 256, blocks = 1;
-    size_t lds = sizeof(double) * ((size_t)(md.nr + 1) + 2 * (size_t)(md.nt + 1) + 2 * 361);
+    if (run->nlanes == 1) threads = 64; // one wavefront: lane 0 traces, the others serve its lookups
+    size_t lds = sizeof(double) * ((size_t)(md.nr + 1) + 2 * (size_t)(md.nt + 1) + 2 * 361 + 6 * (size_t)threads);
     if (run->nlanes == 1) {
-        threads = 64; // one wavefront: lane 0 traces, the others serve its lookups
         blocks = 1;
     } else {
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport, threads, lds) != hipSuccess || per_cu < 1)
             per_cu = 1;
         blocks = per_cu * ctx->sm_count;
-        long long need_blocks = (run->nphot + threads - 1) / threads;
-        if (blocks > need_blocks) blocks = (int)need_blocks;
-        if (run->nlanes > 1) {
-            long long b = (run->nlanes + threads - 1) / threads;
-            if (b < blocks) blocks = (int)b;
-        }
+        // Packets in flight at once take their Bjorkman-Wood slices in
+        // arrival order rather than packet order.  With at most nphot/64 of
+        // them in flight the temperatures keep the serial loop's statistics;
+        // more concurrency biases thick unresolved cells.
+        long long cap = (run->nlanes > 1) ? run->nlanes : run->nphot / INFLIGHT_DIV;
+        long long cap_blocks = (cap + threads - 1) / threads;
+        if (cap_blocks < 1) cap_blocks = 1;
+        if (cap_blocks < blocks) blocks = (int)cap_blocks;
         if (blocks < 1) blocks = 1;
     }
     TransportArgs A;

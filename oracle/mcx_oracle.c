@@ -1156,6 +1156,92 @@ static void emit_star(ctx_t *C, phot_t *p, int64_t Nphot)
     }
 }
 
+/* MCRadiation.f:576-646 (multiwav off): the MC spectrum of an escaping packet */
+static void tally_escape(ctx_t *C, const phot_t *p, const double *wangle, const double *wlam)
+{
+    orc_tally *t = C->t;
+    const int nlam = C->nlam;
+    int ia = (p->vz > 0.0) ? (int)(p->vz * (double)(float)NANG + 1) : (int)(-p->vz * (double)(float)NANG + 1);
+    if (ia > NANG) ia = NANG;
+    int il = (p->wl1 > p->wl2) ? p->ilam1 : p->ilam2;
+    double add = ((wangle[ia] * p->E) * wlam[il - 1]) / 1.0;
+    t->spec[(size_t)ia * nlam + il - 1] += add;
+    t->ispec[(size_t)ia * nlam + il - 1] += 1;
+    if (p->scatt) {
+        t->scatspec[(size_t)ia * nlam + il - 1] += add;
+        t->iscatspec[(size_t)ia * nlam + il - 1] += 1;
+    }
+    t->nescape++;
+    t->Eescape += p->E;
+}
+
+/* The same pass with `inflight` packets advanced in turn, one trace2d call
+ * (up to the next interaction, escape or star hit) per turn: the schedule of
+ * a device that keeps many packets in flight, each drawing from its own
+ * Philox stream.  Not a reference code path: a diagnostic for how the
+ * Bjorkman-Wood sequence depends on interleaving. */
+typedef struct { phot_t p; rng_t g; double nEJv, tau; int active; } slot_t;
+
+static int interleaved_loop(ctx_t *C, const orc_run *run, const double *wangle, const double *wlam)
+{
+    orc_tally *t = C->t;
+    const int64_t Nphot = run->nphot, K = run->inflight;
+    slot_t *S = (slot_t *)calloc((size_t)K, sizeof(slot_t));
+    if (!S) return -3;
+    int64_t next = 1;
+    int err = 0;
+    for (;;) {
+        int any = 0;
+        for (int64_t s = 0; s < K && !err; s++) {
+            slot_t *sl = &S[s];
+            if (!sl->active) {
+                if (next > Nphot) continue;
+                memset(&sl->p, 0, sizeof(phot_t));
+                sl->p.nr = next;
+                C->g.pid = (uint64_t)(next - 1);
+                C->g.k = 0;
+                C->nEJv = 0.0;
+                emit_star(C, &sl->p, Nphot);
+                sl->tau = -log(ran2(&C->g));
+                sl->g = C->g;
+                sl->nEJv = C->nEJv;
+                sl->active = 1;
+                next++;
+            }
+            any = 1;
+            C->g = sl->g;
+            C->nEJv = sl->nEJv;
+            phot_t *p = &sl->p;
+            int escape, hitstar;
+            if (trace2d(C, p, sl->tau, &escape, &hitstar) != 0) { err = -1; break; }
+            if (hitstar) {
+                t->nstarhit++;
+                C->nEJv = 0.0;
+                emit_star(C, p, Nphot);
+                sl->tau = -log(ran2(&C->g));
+            } else if (escape) {
+                tally_escape(C, p, wangle, wlam);
+                sl->active = 0;
+            } else {
+                t->ninteract++;
+                interact(C, p);
+                int killed = 0;
+                if (run->maxinteract > 0) {
+                    if (ran2(&C->g) < (1.0 / (double)(float)run->maxinteract)) { t->nmaxinteract++; killed = 1; }
+                    else p->E = p->E * (double)((float)run->maxinteract / (float)(run->maxinteract - 1));
+                }
+                if (killed) sl->active = 0;
+                else sl->tau = -log(ran2(&C->g));
+            }
+            sl->g = C->g;
+            sl->nEJv = C->nEJv;
+        }
+        if (!any || err) break;
+    }
+    free(S);
+    return err;
+}
+
 int orc_mcradiation(const orc_model *m, const orc_run *run, orc_ran1_state *rs, orc_tally *t)
 {
     ctx_t C;
@@ -1220,45 +1306,38 @@ int orc_mcradiation(const orc_model *m, const orc_run *run, orc_ran1_state *rs, 
     t->Eescape = 0.0;
     int err = 0;
     int64_t Nphot = run->nphot;
-    phot_t p;
-    memset(&p, 0, sizeof(p));
-    for (int64_t ip = 1; ip <= Nphot && !err; ip++) {
-        p.nr = ip;
-        if (run->rng == 1) { C.g.pid = (uint64_t)(ip - 1); C.g.k = 0; }
-        if (t->pkt_steps) t->pkt_steps[ip - 1] = -t->nsteps;
-    label3:
-        C.nEJv = 0.0;
-        emit_star(&C, &p, Nphot);
-        double tau = -log(ran2(&C.g));
-        for (;;) {
-            int escape, hitstar;
-            if (trace2d(&C, &p, tau, &escape, &hitstar) != 0) { err = -1; break; }
-            if (hitstar) { t->nstarhit++; goto label3; }
-            if (escape) {
-                /* MCRadiation.f:576-646 (multiwav off) */
-                int ia = (p.vz > 0.0) ? (int)(p.vz * (double)(float)NANG + 1) : (int)(-p.vz * (double)(float)NANG + 1);
-                if (ia > NANG) ia = NANG;
-                int il = (p.wl1 > p.wl2) ? p.ilam1 : p.ilam2;
-                double add = ((wangle[ia] * p.E) * wlam[il - 1]) / 1.0;
-                t->spec[(size_t)ia * nlam + il - 1] += add;
-                t->ispec[(size_t)ia * nlam + il - 1] += 1;
-                if (p.scatt) {
-                    t->scatspec[(size_t)ia * nlam + il - 1] += add;
-                    t->iscatspec[(size_t)ia * nlam + il - 1] += 1;
+    if (run->inflight > 1) {
+        if (run->rng != 1) return -4;
+        err = interleaved_loop(&C, run, wangle, wlam);
+    } else {
+        phot_t p;
+        memset(&p, 0, sizeof(p));
+        for (int64_t ip = 1; ip <= Nphot && !err; ip++) {
+            p.nr = ip;
+            if (run->rng == 1) { C.g.pid = (uint64_t)(ip - 1); C.g.k = 0; }
+            if (t->pkt_steps) t->pkt_steps[ip - 1] = -t->nsteps;
+        label3:
+            C.nEJv = 0.0;
+            emit_star(&C, &p, Nphot);
+            double tau = -log(ran2(&C.g));
+            for (;;) {
+                int escape, hitstar;
+                if (trace2d(&C, &p, tau, &escape, &hitstar) != 0) { err = -1; break; }
+                if (hitstar) { t->nstarhit++; goto label3; }
+                if (escape) {
+                    tally_escape(&C, &p, wangle, wlam);
+                    break;
                 }
-                t->nescape++;
-                t->Eescape += p.E;
-                break;
+                t->ninteract++;
+                interact(&C, &p);
+                if (run->maxinteract > 0) {
+                    if (ran2(&C.g) < (1.0 / (double)(float)run->maxinteract)) { t->nmaxinteract++; break; }
+                    p.E = p.E * (double)((float)run->maxinteract / (float)(run->maxinteract - 1));
+                }
+                tau = -log(ran2(&C.g));
             }
-            t->ninteract++;
-            interact(&C, &p);
-            if (run->maxinteract > 0) {
-                if (ran2(&C.g) < (1.0 / (double)(float)run->maxinteract)) { t->nmaxinteract++; break; }
-                p.E = p.E * (double)((float)run->maxinteract / (float)(run->maxinteract - 1));
-            }
-            tau = -log(ran2(&C.g));
+            if (t->pkt_steps) t->pkt_steps[ip - 1] += t->nsteps;
         }
-        if (t->pkt_steps) t->pkt_steps[ip - 1] += t->nsteps;
     }
     free(C.KabsTot); free(C.KscaTot); free(C.mass); free(C.KabsL); free(C.KextL);
     free(C.spec); free(C.lastphotnr);

@@ -82,6 +82,9 @@ typedef struct {
                                1: Philox4x32-10, one stream per packet    */
     uint64_t seed;          /* Philox key                                 */
     int maxinteract;        /* > 0 enables the overflow roulette          */
+    int64_t inflight;       /* > 1 (Philox only): advance this many packets
+                               in turn, one trace2d call each (diagnostic
+                               of interleaved schedules; 0/1: serial)    */
 } orc_run;
 
 typedef struct {

@@ -30,7 +30,8 @@ class OrcModel(C.Structure):
 
 
 class OrcRun(C.Structure):
-    _fields_ = [("nphot", C.c_int64), ("rng", C.c_int), ("seed", C.c_uint64), ("maxinteract", C.c_int)]
+    _fields_ = [("nphot", C.c_int64), ("rng", C.c_int), ("seed", C.c_uint64), ("maxinteract", C.c_int),
+                ("inflight", C.c_int64)]
 
 
 class OrcTally(C.Structure):
@@ -175,7 +176,7 @@ class Oracle:
     def min_dist(self, x, y, z, i, jj):
         return self.lib.orc_min_dist(C.byref(self.om), x, y, z, int(i), int(jj))
 
-    def mcradiation(self, nphot, rng="philox", seed=1, maxinteract=0, per_packet=False):
+    def mcradiation(self, nphot, rng="philox", seed=1, maxinteract=0, per_packet=False, inflight=0):
         m = self.m
         n = m.nr * m.nt
         nb = (NANGLE + 1) * m.nlam
@@ -188,7 +189,8 @@ class Oracle:
         if per_packet:
             out["pkt_steps"] = np.zeros(int(nphot), dtype=np.int64)
             t.pkt_steps = out["pkt_steps"].ctypes.data_as(C.POINTER(C.c_int64))
-        run = OrcRun(nphot=int(nphot), rng=0 if rng == "ran1" else 1, seed=int(seed), maxinteract=int(maxinteract))
+        run = OrcRun(nphot=int(nphot), rng=0 if rng == "ran1" else 1, seed=int(seed), maxinteract=int(maxinteract),
+                     inflight=int(inflight))
         rc = self.lib.orc_mcradiation(C.byref(self.om), C.byref(run), C.byref(self.ran1), C.byref(t))
         if rc != 0:
             raise RuntimeError(f"oracle transport failed: {rc}")
