@@ -167,7 +167,8 @@ __device__ __forceinline__ bool exit_face(const double *__restrict__ R2, const d
 // Subroutines.f:1648-1672: rotate (x,y,z) about the unit axis (u,v,w).
 __device__ __forceinline__ void rotate_about(double &x, double &y, double &z, double u, double v, double w,
                                              double theta) {
-    double cost = cos(theta), sint = sin(theta);
+    double sint, cost;
+    sincos(theta, &sint, &cost);
     double u2 = u * u, v2 = v * v, w2 = w * w;
     double X = x, Y = y, Z = z;
     double inp = X * u + Y * v + Z * w;
