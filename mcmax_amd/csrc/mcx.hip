@@ -340,6 +340,12 @@ __device__ __forceinline__ void acc_flush(const DevTally &ty, TallyAcc &t) {
     t.e1 = t.l0 = t.l1 = t.l2 = 0ull;
 }
 
+// non-negative integral double to u64; the common case below 2^32 takes the
+// short native conversion
+__device__ __forceinline__ unsigned long long d2u64(double q) {
+    return (q < 0x1p32) ? (unsigned long long)(unsigned)q : (unsigned long long)q;
+}
+
 // one EJv contribution x of the current cell, in units 1/invU1
 __device__ __forceinline__ void acc_ejv(TallyAcc &t, double x, double invU1, unsigned long long *scnt) {
     double q = rint(x * invU1);
@@ -348,13 +354,17 @@ __device__ __forceinline__ void acc_ejv(TallyAcc &t, double x, double invU1, uns
         atomicAdd(&scnt[C_NOVERFLOW], 1ull);
         q = 0x1p62;
     }
-    if (q > 0.0) t.e1 += (unsigned long long)q;
+    if (q > 0.0) t.e1 += d2u64(q);
 }
 
 // one EJv2 contribution x2 in units 1/invU2, split into 32-bit-weighted limbs
 __device__ __forceinline__ void acc_ejv2(TallyAcc &t, double x2, double invU2, unsigned long long *scnt) {
     double q = x2 * invU2;
     if (!(q > 0.0)) return;
+    if (q < 0x1p31) { // one limb: the value the general split below gives
+        t.l0 += (unsigned long long)(unsigned)rint(q);
+        return;
+    }
     double l2 = floor(q * 0x1p-64);
     if (l2 > 0x1p31) {
         atomicAdd(&scnt[C_NOVERFLOW], 1ull);
@@ -363,9 +373,9 @@ __device__ __forceinline__ void acc_ejv2(TallyAcc &t, double x2, double invU2, u
     q = q - l2 * 0x1p64;
     double l1 = floor(q * 0x1p-32);
     double l0 = rint(q - l1 * 0x1p32);
-    if (l0 > 0.0) t.l0 += (unsigned long long)l0;
-    if (l1 > 0.0) t.l1 += (unsigned long long)l1;
-    if (l2 > 0.0) t.l2 += (unsigned long long)l2;
+    if (l0 > 0.0) t.l0 += d2u64(l0);
+    if (l1 > 0.0) t.l1 += d2u64(l1);
+    if (l2 > 0.0) t.l2 += d2u64(l2);
 }
 
 // An inverse-CDF lookup: the first bin b in [0, n) whose cumulative value
