@@ -37,6 +37,20 @@ def algorithmic_bytes(t, ng, nlam):
     return total, (Bx, Ba, Bs, Brw)
 
 
+def measured_traffic(config, nphot):
+    """HBM bytes per k_transport launch for this workload, from the rocprofv3
+    FETCH_SIZE/WRITE_SIZE runs summarised in profiles/ (scaled by a plain copy
+    of known size); None when no such measurement exists."""
+    path = os.path.join(ROOT, "profiles", "k_transport_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("config") != config or int(t.get("nphot", -1)) != int(nphot):
+        return None
+    return t.get("bytes_per_launch")
+
+
 def cpu_baseline(m, seconds=12.0):
     """Oracle (FP64 C restatement of the reference loop) on one host core, on
     a bounded sample of the same workload."""
@@ -58,7 +72,7 @@ def cpu_baseline(m, seconds=12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nphot", type=int, default=10_000_000)
     ap.add_argument("--config", default="production")
@@ -117,7 +131,7 @@ def main():
                    "nphot": a.nphot, "nr": m.nr, "nt": m.nt, "nlam": m.nlam, "ng": m.ng,
                    "randomwalk": not a.plain},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic(a.config, a.nphot),
                      "kernel": "k_transport", "kernel_ms": kern_s * 1e3,
                      "bytes_per_event": {"step": Bx, "absorption": Ba, "scattering": Bs, "rw_step": Brw,
                                          "escape": 8},

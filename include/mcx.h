@@ -157,6 +157,31 @@ int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *out);
  * the host when the pointers are non-NULL. */
 int mcx_temperature(mcx_ctx *ctx, mcx_temps *out);
 
+/* Observation of the SED (Telescope.f:590-657 defaults in brackets): the
+ * reference's Observe with type SPECTRUM, one inclination. */
+typedef struct {
+    double angle;                    /* inclination [rad]                   */
+    int32_t nphi, nj0;               /* image azimuths [45], rays per radial
+                                        cell [2]                            */
+    int32_t nphot, nphotstar;        /* scattering packets per wavelength
+                                        from the disk / the star [1000]     */
+    int32_t nexits;                  /* escape directions per cell [5]      */
+    int32_t usepol;                  /* [1]                                 */
+    int32_t tracestar, traceemis, tracescat; /* [1, 1, 1]                   */
+    double pfstop;                   /* survival exponent [0.25]            */
+    uint64_t seed;                   /* Philox key of wavelength k: seed+k  */
+} mcx_obs;
+
+/* The ray-traced SED at nlam_obs wavelengths: TracePath + TraceFlux
+ * (TraceMono.f:8-578, 2032-2293) on the device.  T: cell temperatures
+ * [nr*nt] (NULL: those of the last mcx_temperature); theta_av: the theta
+ * cell centres [nt] (Init.f:2791-2815) that place the image radii.  Writes
+ * flux, scatflux and fluxQ [nlam_obs] (spectrum before the distance
+ * scaling, Telescope.f:160).  Needs Kext (mcx_set_tables) and thet
+ * (mcx_set_model). */
+int mcx_sed(mcx_ctx *ctx, const mcx_obs *obs, const double *T, const double *theta_av, int32_t nlam_obs,
+            const double *lam_obs, double *flux, double *scatflux, double *fluxQ);
+
 /* Duration in milliseconds of the last mcx_transport's packet kernel and of
  * the whole pass, both measured with HIP events on the context's stream. */
 int mcx_last_timing(const mcx_ctx *ctx, double *kernel_ms, double *pass_ms);

@@ -62,9 +62,17 @@ class Temps(C.Structure):
     _fields_ = [("T", _dp), ("dT", _dp), ("dEJv", _dp)]
 
 
+class Obs(C.Structure):
+    _fields_ = [
+        ("angle", C.c_double), ("nphi", C.c_int32), ("nj0", C.c_int32), ("nphot", C.c_int32),
+        ("nphotstar", C.c_int32), ("nexits", C.c_int32), ("usepol", C.c_int32), ("tracestar", C.c_int32),
+        ("traceemis", C.c_int32), ("tracescat", C.c_int32), ("pfstop", C.c_double), ("seed", C.c_uint64),
+    ]
+
+
 EXPORTS = (
     "mcx_create", "mcx_destroy", "mcx_last_error", "mcx_set_tables", "mcx_set_model",
-    "mcx_transport", "mcx_get_tallies", "mcx_temperature", "mcx_last_timing",
+    "mcx_transport", "mcx_get_tallies", "mcx_temperature", "mcx_last_timing", "mcx_sed",
 )
 
 _lib = None
@@ -92,8 +100,9 @@ def load():
     lib.mcx_get_tallies.argtypes = [vp, C.POINTER(Tallies)]
     lib.mcx_temperature.argtypes = [vp, C.POINTER(Temps)]
     lib.mcx_last_timing.argtypes = [vp, _dp, _dp]
+    lib.mcx_sed.argtypes = [vp, C.POINTER(Obs), _dp, _dp, C.c_int32, _dp, _dp, _dp, _dp]
     for n in ("mcx_set_tables", "mcx_set_model", "mcx_transport", "mcx_get_tallies",
-              "mcx_temperature", "mcx_last_timing"):
+              "mcx_temperature", "mcx_last_timing", "mcx_sed"):
         getattr(lib, n).restype = C.c_int
     _lib = lib
     return lib

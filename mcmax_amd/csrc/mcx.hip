@@ -26,80 +26,7 @@
 //    coefficients from a table per distinct species mix and integer T, built
 //    once per model on the device, instead of recomputing DiffCoeffCell
 //    (Dtable.f:97-182) whenever a cell's temperature moves.
-#include <hip/hip_runtime.h>
-#include <math.h>
-#include <string.h>
-#include <stdio.h>
-#include <stdlib.h>
-#include <map>
-#include <string>
-#include <vector>
-
-#include "../../include/mcx.h"
-#include "mcx_device.h"
-
-#define TMAXD MCX_DTMAX
-#define NANGD MCX_DNANG
-#define MAXNG 16
-
-// fixed-point resolutions
-#define EJV_BITS 31   // one EJv event of the cell's bound maps to 2^31
-#define EJV2_BITS 40  // one squared bound maps to 2^40 (3 x 32-bit limbs)
-#define EABS_BITS 24  // one nominal packet energy maps to 2^24
-
-enum { C_NEMIT = 0, C_NINTERACT, C_NABS, C_NSCAT, C_NESCAPE, C_NSTARHIT, C_NSTEPS, C_NRAAR, C_NMAXINT,
-       C_NGEOM, C_NOVERFLOW, C_NRW, C_NCOUNT };
-
-#define NY_RW 1000 // survival-table length (RandomWalk.f:6-7)
-
-struct DevTables {
-    int nlam, ng, scat_how, scattering;
-    const double *nu;        // [nlam]
-    const double *cumStar;   // [nlam-1]
-    double Lstar;
-    const double *CUM;       // [ng][TMAX+1][nlam-1]
-    const double *Kp;        // [ng][TMAX+1]
-    const double *Ksca;      // [ng][nlam]
-    const double *F;         // [ng][nlam][6][180]
-    const double *C11, *C12; // [ng][nlam][180]
-    const double *IF11, *IF12;
-    const double *cos2phi, *sin2phi; // [361]
-};
-
-struct DevModel {
-    int nr, nt, ncell;
-    const double *R2, *T2, *Theta; // [nr+1], [nt+1], [nt+1]
-    double R0;                     // R[0] in AU
-    const double *dens, *mass;     // [ncell]
-    const double *w;               // [ncell*ng]
-    const double2 *KT;             // [ncell*nlam] {KabsTot, KscaTot}
-    const double *KpCell;          // [ncell*(TMAX+1)]
-};
-
-struct DevTally {
-    unsigned long long *EJv;    // [ncell]
-    unsigned long long *EJv2;   // [ncell*3] limbs of weight 2^0, 2^32, 2^64
-    unsigned long long *Eabs;   // [ncell]
-    const double *invU1, *invU2;
-    unsigned long long *Tcell;  // [ncell] bits of the latest BW T (monotone)
-    unsigned *bitmap;           // [ncell*nwords]
-    unsigned long long *specE;  // [2][(NANG+1)*nlam]
-    unsigned *ispec;            // [2][(NANG+1)*nlam]
-    unsigned long long *counters;
-    double *tautot;
-};
-
-// modified random walk (RandomWalk.f)
-struct DevRW {
-    int on;
-    double factRW;
-    const int *thick;     // [ncell]
-    const int *cls;       // [ncell] species-mix class of the cell
-    const double2 *KD;    // [nclass*(TMAX+1)] {KDext, KDabs} (DiffCoeffCell)
-    const double *R;      // [nr+1] radial edges (AU)
-    const double *sinT;   // [nt+1] sin(theta) of the cone edges
-    const double *y, *phi; // [NY_RW] (InitRandomWalk)
-};
+#include "mcx_internal.h"
 
 struct TransportArgs {
     DevTables tb;
@@ -210,21 +137,6 @@ __device__ double dev_increaseT_w(const double *__restrict__ kpc, int jw, double
     return dev_determineT(kpc, E1, Tstart);
 }
 
-// Radiation.f:1007-1024
-__device__ __forceinline__ void random_direction(PhiloxStream &g, double &x, double &y, double &z) {
-    double r;
-    for (;;) {
-        x = 2.0 * g.next() - 1.0;
-        y = 2.0 * g.next() - 1.0;
-        z = 2.0 * g.next() - 1.0;
-        r = x * x + y * y + z * z;
-        if (!(r > 1.0)) break;
-    }
-    r = sqrt(r);
-    x = x / r;
-    y = y / r;
-    z = z / r;
-}
 
 // First bin b in [0, n) with cum(b) >= Lr; n when none.
 template <typename F>
@@ -1308,77 +1220,6 @@ __global__ void k_temperature(int nr, int nt, int ng, const double *__restrict__
 
 // ===========================================================================
 // host side
-struct mcx_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::string err;
-    std::vector<void *> tab_alloc, mod_alloc, run_alloc;
-    bool have_tables = false, have_model = false, have_pass = false;
-    // tables
-    DevTables tb{};
-    int nlam = 0, ng = 0;
-    double Lstar = 0.0;
-    std::vector<double> h_Kp, h_nu;
-    double *d_Kabs = nullptr, *d_Ksca = nullptr;
-    double *d_Kext = nullptr, *d_gsym = nullptr, *d_BB = nullptr; // random-walk tables (may stay null)
-    const double *d_rwy = nullptr, *d_rwphi = nullptr;
-    // random walk, per model
-    DevRW rw{};
-    int nclass = 0;
-    int *d_thick = nullptr;
-    // model
-    DevModel md{};
-    double *d_V = nullptr, *d_R2 = nullptr;
-    double Tvoid = 1.0;
-    // run
-    DevTally ty{};
-    size_t bitmap_bytes = 0;
-    int *d_Ni = nullptr;
-    double *d_EJv = nullptr, *d_EJv2 = nullptr, *d_Eabs = nullptr, *d_Tbw = nullptr;
-    double *d_T = nullptr, *d_dT = nullptr, *d_dEJv = nullptr;
-    unsigned long long *d_next = nullptr;
-    double unitE = 0.0, E0 = 0.0;
-    double kernel_ms = 0.0, pass_ms = 0.0;
-    int sm_count = 0;
-};
-
-static int fail(mcx_ctx *ctx, int code, const std::string &msg) {
-    if (ctx) ctx->err = msg;
-    return code;
-}
-
-#define HIPCHK(ctx, call)                                                                                     \
-    do {                                                                                                       \
-        hipError_t e_ = (call);                                                                                \
-        if (e_ != hipSuccess) return fail(ctx, MCX_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-template <typename T>
-static int dalloc(mcx_ctx *ctx, std::vector<void *> &list, T **p, size_t n) {
-    void *q = nullptr;
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc(&q, n * sizeof(T));
-    if (e != hipSuccess) return fail(ctx, MCX_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
-    list.push_back(q);
-    *p = (T *)q;
-    return 0;
-}
-
-template <typename T>
-static int dupload(mcx_ctx *ctx, std::vector<void *> &list, T **p, const T *h, size_t n) {
-    int r = dalloc(ctx, list, p, n);
-    if (r) return r;
-    HIPCHK(ctx, hipMemcpyAsync(*p, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
-    return 0;
-}
-
-static void free_list(std::vector<void *> &list) {
-    for (void *q : list) (void)hipFree(q);
-    list.clear();
-}
-
-// y^k for integer k by right-to-left binary powering (Fortran's real**integer)
 static double powi_rl(double x, unsigned k) {
     double r = (k & 1u) ? x : 1.0;
     while (k >>= 1) {
@@ -1541,11 +1382,13 @@ extern "C" int mcx_set_tables(mcx_ctx *ctx, const mcx_tables *t) {
     UP(ctx->d_Kabs, t->Kabs, (size_t)ng * nlam);
     ctx->d_Ksca = (double *)tb.Ksca;
     ctx->d_Kext = ctx->d_gsym = ctx->d_BB = nullptr;
-    if (t->Kext && t->gsym) {
-        UP(ctx->d_Kext, t->Kext, (size_t)ng * nlam);
-        UP(ctx->d_gsym, t->gsym, (size_t)ng * nlam);
-        UP(ctx->d_BB, t->BB, (size_t)(TMAXD + 1) * nlam);
-    }
+    UP(ctx->d_BB, t->BB, (size_t)(TMAXD + 1) * nlam);
+    if (t->Kext) UP(ctx->d_Kext, t->Kext, (size_t)ng * nlam);
+    if (t->gsym) UP(ctx->d_gsym, t->gsym, (size_t)ng * nlam);
+    ctx->h_Kext.clear();
+    if (t->Kext) ctx->h_Kext.assign(t->Kext, t->Kext + (size_t)ng * nlam);
+    ctx->h_lam.assign(t->lam ? t->lam : t->nu, (t->lam ? t->lam : t->nu) + nlam);
+    ctx->h_Fstar.assign(t->Fstar, t->Fstar + nlam);
     {
         std::vector<double> y, phi;
         random_walk_tables(y, phi);
@@ -1612,11 +1455,13 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
     }
     md.KT = KT;
     md.KpCell = KpCell;
-    // random walk: species-mix classes, diffusion-coefficient table, edges
+    // species-mix classes (cells with bitwise equal weights), radial edges;
+    // the random walk's diffusion-coefficient table when Kext and g are known
     ctx->rw = DevRW{};
     ctx->nclass = 0;
     ctx->d_thick = nullptr;
-    if (ctx->d_Kext && gr->thet) {
+    ctx->d_wcls = nullptr;
+    {
         std::map<std::vector<uint64_t>, int> seen;
         std::vector<int> cls(ncell);
         std::vector<double> wcls;
@@ -1631,33 +1476,43 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
             cls[c] = it->second;
         }
         const int nclass = (int)seen.size();
-        if ((size_t)nclass * (TMAXD + 1) * sizeof(double2) <= ((size_t)4 << 30)) {
+        const double *dR;
+        int *dcls;
+        if ((r = dupload(ctx, ctx->mod_alloc, &ctx->d_wcls, wcls.data(), wcls.size()))) return r;
+        if ((r = dupload(ctx, ctx->mod_alloc, (double **)&dR, gr->R, (size_t)nr + 1))) return r;
+        if ((r = dupload(ctx, ctx->mod_alloc, &dcls, cls.data(), (size_t)ncell))) return r;
+        ctx->nclass = nclass;
+        ctx->rw.cls = dcls;
+        ctx->rw.R = dR;
+        if (ctx->d_Kext && ctx->d_gsym && gr->thet &&
+            (size_t)nclass * (TMAXD + 1) * sizeof(double2) <= ((size_t)4 << 30)) {
             std::vector<double> sinT(nt + 1);
             for (int j = 0; j <= nt; j++) sinT[j] = sin(gr->thet[j]);
-            const double *dw, *dR, *dsin;
-            int *dcls;
+            const double *dsin;
             double2 *KD;
-            if ((r = dupload(ctx, ctx->mod_alloc, (double **)&dw, wcls.data(), wcls.size()))) return r;
-            if ((r = dupload(ctx, ctx->mod_alloc, (double **)&dR, gr->R, (size_t)nr + 1))) return r;
             if ((r = dupload(ctx, ctx->mod_alloc, (double **)&dsin, sinT.data(), (size_t)nt + 1))) return r;
-            if ((r = dupload(ctx, ctx->mod_alloc, &dcls, cls.data(), (size_t)ncell))) return r;
             if ((r = dalloc(ctx, ctx->mod_alloc, &KD, (size_t)nclass * (TMAXD + 1)))) return r;
             if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_thick, (size_t)ncell))) return r;
             size_t n = (size_t)nclass * (TMAXD + 1);
             hipLaunchKernelGGL(k_diffcoeff, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, nclass, nlam,
-                               ng, ctx->tb.scattering, ctx->d_Kabs, ctx->d_Ksca, ctx->d_Kext, ctx->d_gsym, dw,
-                               ctx->d_BB, ctx->tb.nu, KD);
+                               ng, ctx->tb.scattering, ctx->d_Kabs, ctx->d_Ksca, ctx->d_Kext, ctx->d_gsym,
+                               ctx->d_wcls, ctx->d_BB, ctx->tb.nu, KD);
             HIPCHK(ctx, hipGetLastError());
-            ctx->nclass = nclass;
-            ctx->rw.cls = dcls;
             ctx->rw.KD = KD;
-            ctx->rw.R = dR;
             ctx->rw.sinT = dsin;
             ctx->rw.y = ctx->d_rwy;
             ctx->rw.phi = ctx->d_rwphi;
             ctx->rw.thick = ctx->d_thick;
         }
     }
+    // host copies for the observation geometry (mcx_sed)
+    ctx->h_R.assign(gr->R, gr->R + nr + 1);
+    ctx->h_Theta.assign(gr->Theta, gr->Theta + nt + 1);
+    ctx->h_thet.clear();
+    if (gr->thet) ctx->h_thet.assign(gr->thet, gr->thet + nt + 1);
+    ctx->h_dens.assign(gr->dens, gr->dens + ncell);
+    ctx->h_w.assign(gr->w, gr->w + (size_t)ncell * ng);
+    ctx->have_temps = false;
     {
         // Main.f:546
         const double AU = MCX_AU;
@@ -1898,6 +1753,7 @@ extern "C" int mcx_temperature(mcx_ctx *ctx, mcx_temps *o) {
                        ctx->Tvoid, ctx->d_T, ctx->d_dT, ctx->d_dEJv);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->have_temps = true;
     if (o) {
         if (o->T) HIPCHK(ctx, hipMemcpy(o->T, ctx->d_T, sizeof(double) * ncell, hipMemcpyDeviceToHost));
         if (o->dT) HIPCHK(ctx, hipMemcpy(o->dT, ctx->d_dT, sizeof(double) * ncell, hipMemcpyDeviceToHost));

@@ -60,10 +60,26 @@ struct PhiloxStream {
     }
 };
 
+// Radiation.f:1007-1024
+__device__ __forceinline__ void random_direction(PhiloxStream &g, double &x, double &y, double &z) {
+    double r;
+    for (;;) {
+        x = 2.0 * g.next() - 1.0;
+        y = 2.0 * g.next() - 1.0;
+        z = 2.0 * g.next() - 1.0;
+        r = x * x + y * y + z * z;
+        if (!(r > 1.0)) break;
+    }
+    r = sqrt(r);
+    x = x / r;
+    y = y / r;
+    z = z / r;
+}
+
 // ---------------------------------------------------------------------------
 // Ray-sphere and ray-cone crossings, in the numerically stable q-form of the
 // reference (MCRadiation.f:1274-1355).  Thet is cos^2 of the cone edge.
-__device__ __forceinline__ bool hit_sphere(double Rad2, double r, double b, double &v) {
+__host__ __device__ __forceinline__ bool hit_sphere(double Rad2, double r, double b, double &v) {
     bool hit = false;
     v = 1e200;
     double cc = r - Rad2;
@@ -78,7 +94,7 @@ __device__ __forceinline__ bool hit_sphere(double Rad2, double r, double b, doub
     return hit;
 }
 
-__device__ __forceinline__ bool hit_cone(double Thet, double r, double b, double z, double vz, double &v) {
+__host__ __device__ __forceinline__ bool hit_cone(double Thet, double r, double b, double z, double vz, double &v) {
     bool hit = false;
     v = 1e200;
     double at = Thet - vz * vz;
@@ -95,7 +111,7 @@ __device__ __forceinline__ bool hit_cone(double Thet, double r, double b, double
     return hit;
 }
 
-__device__ __forceinline__ bool hit_cone_same(double Thet, double b, double z, double vz, double &v) {
+__host__ __device__ __forceinline__ bool hit_cone_same(double Thet, double b, double z, double vz, double &v) {
     double bt = Thet * b - (2.0 * z) * vz;
     double at = Thet - vz * vz;
     v = -bt / at;
@@ -105,7 +121,7 @@ __device__ __forceinline__ bool hit_cone_same(double Thet, double b, double z, d
 // Distance to the exit face of cell (i, j) (MCRadiation.f:1150-1272).
 // R2/T2 are squared radial / cos-theta edges; ntheta_edges = nt + 1.
 // Returns false on "nothing to hit".  edgeNr is updated; jnext is 0-based.
-__device__ __forceinline__ bool exit_face(const double *__restrict__ R2, const double *__restrict__ T2,
+__host__ __device__ __forceinline__ bool exit_face(const double *__restrict__ R2, const double *__restrict__ T2,
                                           int nt, double x, double y, double z, double vx, double vy,
                                           double vz, int i, int jj, bool onEdge, int &edgeNr, double &v,
                                           int &inext, int &jnext) {

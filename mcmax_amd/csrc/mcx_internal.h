@@ -1,0 +1,155 @@
+// mcx_internal.h -- definitions shared by the translation units of libmcx.so
+// (transport: mcx.hip; ray-traced SED: sed.hip): device-resident tables and
+// model, tallies, and the host context behind the C-ABI of include/mcx.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mcx.h"
+#include "mcx_device.h"
+
+#define TMAXD MCX_DTMAX
+#define NANGD MCX_DNANG
+#define MAXNG 16
+
+// fixed-point resolutions
+#define EJV_BITS 31   // one EJv event of the cell's bound maps to 2^31
+#define EJV2_BITS 40  // one squared bound maps to 2^40 (3 x 32-bit limbs)
+#define EABS_BITS 24  // one nominal packet energy maps to 2^24
+
+enum { C_NEMIT = 0, C_NINTERACT, C_NABS, C_NSCAT, C_NESCAPE, C_NSTARHIT, C_NSTEPS, C_NRAAR, C_NMAXINT,
+       C_NGEOM, C_NOVERFLOW, C_NRW, C_NCOUNT };
+
+#define NY_RW 1000 // survival-table length (RandomWalk.f:6-7)
+
+struct DevTables {
+    int nlam, ng, scat_how, scattering;
+    const double *nu;        // [nlam]
+    const double *cumStar;   // [nlam-1]
+    double Lstar;
+    const double *CUM;       // [ng][TMAX+1][nlam-1]
+    const double *Kp;        // [ng][TMAX+1]
+    const double *Ksca;      // [ng][nlam]
+    const double *F;         // [ng][nlam][6][180]
+    const double *C11, *C12; // [ng][nlam][180]
+    const double *IF11, *IF12;
+    const double *cos2phi, *sin2phi; // [361]
+};
+
+struct DevModel {
+    int nr, nt, ncell;
+    const double *R2, *T2, *Theta; // [nr+1], [nt+1], [nt+1]
+    double R0;                     // R[0] in AU
+    const double *dens, *mass;     // [ncell]
+    const double *w;               // [ncell*ng]
+    const double2 *KT;             // [ncell*nlam] {KabsTot, KscaTot}
+    const double *KpCell;          // [ncell*(TMAX+1)]
+};
+
+struct DevTally {
+    unsigned long long *EJv;    // [ncell]
+    unsigned long long *EJv2;   // [ncell*3] limbs of weight 2^0, 2^32, 2^64
+    unsigned long long *Eabs;   // [ncell]
+    const double *invU1, *invU2;
+    unsigned long long *Tcell;  // [ncell] bits of the latest BW T (monotone)
+    unsigned *bitmap;           // [ncell*nwords]
+    unsigned long long *specE;  // [2][(NANG+1)*nlam]
+    unsigned *ispec;            // [2][(NANG+1)*nlam]
+    unsigned long long *counters;
+    double *tautot;
+};
+
+// modified random walk (RandomWalk.f)
+struct DevRW {
+    int on;
+    double factRW;
+    const int *thick;     // [ncell]
+    const int *cls;       // [ncell] species-mix class of the cell
+    const double2 *KD;    // [nclass*(TMAX+1)] {KDext, KDabs} (DiffCoeffCell)
+    const double *R;      // [nr+1] radial edges (AU)
+    const double *sinT;   // [nt+1] sin(theta) of the cone edges
+    const double *y, *phi; // [NY_RW] (InitRandomWalk)
+};
+
+struct mcx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+    std::vector<void *> tab_alloc, mod_alloc, run_alloc;
+    bool have_tables = false, have_model = false, have_pass = false;
+    // tables
+    DevTables tb{};
+    int nlam = 0, ng = 0;
+    double Lstar = 0.0;
+    std::vector<double> h_Kp, h_nu;
+    double *d_Kabs = nullptr, *d_Ksca = nullptr;
+    double *d_Kext = nullptr, *d_gsym = nullptr, *d_BB = nullptr; // Kext, g may stay null
+    std::vector<double> h_lam, h_Fstar, h_Kext;                      // host copies for mcx_sed
+    const double *d_rwy = nullptr, *d_rwphi = nullptr;
+    // random walk, per model
+    DevRW rw{};
+    int nclass = 0;
+    int *d_thick = nullptr;
+    double *d_wcls = nullptr; // [nclass][ng] weights of each species-mix class
+    std::vector<double> h_R, h_Theta, h_thet, h_dens, h_w;
+    bool have_temps = false;
+    // model
+    DevModel md{};
+    double *d_V = nullptr, *d_R2 = nullptr;
+    double Tvoid = 1.0;
+    // run
+    DevTally ty{};
+    size_t bitmap_bytes = 0;
+    int *d_Ni = nullptr;
+    double *d_EJv = nullptr, *d_EJv2 = nullptr, *d_Eabs = nullptr, *d_Tbw = nullptr;
+    double *d_T = nullptr, *d_dT = nullptr, *d_dEJv = nullptr;
+    unsigned long long *d_next = nullptr;
+    double unitE = 0.0, E0 = 0.0;
+    double kernel_ms = 0.0, pass_ms = 0.0;
+    int sm_count = 0;
+};
+
+static inline int fail(mcx_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                                     \
+    do {                                                                                                       \
+        hipError_t e_ = (call);                                                                                \
+        if (e_ != hipSuccess) return fail(ctx, MCX_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+static int dalloc(mcx_ctx *ctx, std::vector<void *> &list, T **p, size_t n) {
+    void *q = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess) return fail(ctx, MCX_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+    list.push_back(q);
+    *p = (T *)q;
+    return 0;
+}
+
+template <typename T>
+static int dupload(mcx_ctx *ctx, std::vector<void *> &list, T **p, const T *h, size_t n) {
+    int r = dalloc(ctx, list, p, n);
+    if (r) return r;
+    HIPCHK(ctx, hipMemcpyAsync(*p, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return 0;
+}
+
+static inline void free_list(std::vector<void *> &list) {
+    for (void *q : list) (void)hipFree(q);
+    list.clear();
+}
+
+// y^k for integer k by right-to-left binary powering (Fortran's real**integer)

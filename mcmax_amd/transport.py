@@ -8,7 +8,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import NANGLE, TMAX, ptr
+from ._lib import NANGLE, TMAX, Obs, ptr
 
 
 class MCXError(RuntimeError):
@@ -116,10 +116,36 @@ class Transport:
         self._check(self.lib.mcx_temperature(self.ctx, C.byref(o)))
         return T.reshape(m.nr, m.nt), dT.reshape(m.nr, m.nt), dE.reshape(m.nr, m.nt)
 
+    def sed(self, obs, lams, T=None):
+        """Ray-traced spectrum at wavelengths lams (mcx_sed): returns an
+        array [len(lams), 3] of flux, scattered flux and Q flux.  T: cell
+        temperatures [nr, nt], or None for those of the last temperature()."""
+        m = self.model
+        lams = np.ascontiguousarray(lams, dtype=np.float64)
+        out = np.zeros((3, len(lams)))
+        tav = _c(m.theta_av)
+        Tp = None
+        if T is not None:
+            Tc = _c(np.asarray(T).ravel())
+            Tp = ptr(Tc)
+        self._check(self.lib.mcx_sed(self.ctx, C.byref(obs), Tp, ptr(tav), len(lams), ptr(lams), ptr(out[0]),
+                                     ptr(out[1]), ptr(out[2])))
+        return out.T.copy()
+
     def timing(self):
         k, p = C.c_double(), C.c_double()
         self._check(self.lib.mcx_last_timing(self.ctx, C.byref(k), C.byref(p)))
         return k.value, p.value
+
+
+def make_obs(angle_deg=45.0, nphi=45, nj0=2, nphot=1000, nphotstar=1000, nexits=5, usepol=True, tracestar=True,
+             traceemis=True, tracescat=True, pfstop=0.25, seed=1):
+    """An observation with the reference's defaults (Telescope.f:590-657,
+    pfstop Init.f:378)."""
+    import math
+    return Obs(angle=math.pi * angle_deg / 180.0, nphi=nphi, nj0=nj0, nphot=nphot, nphotstar=nphotstar,
+               nexits=nexits, usepol=int(usepol), tracestar=int(tracestar), traceemis=int(traceemis),
+               tracescat=int(tracescat), pfstop=pfstop, seed=seed)
 
 
 def mc_radiation(tr, nphot, seed=1, nlanes=0, maxinteract=0, coop_lanes=-1):
