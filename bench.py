@@ -51,17 +51,18 @@ def measured_traffic(config, nphot):
     return t.get("bytes_per_launch")
 
 
-def cpu_baseline(m, seconds=12.0):
+def cpu_baseline(m, seconds=15.0):
     """Oracle (FP64 C restatement of the reference loop) on one host core, on
-    a bounded sample of the same workload."""
+    a bounded sample of the same workload: a pilot of 5000 packets sizes a
+    sample of about `seconds` of CPU time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     o = pyoracle.Oracle(m)
-    n = 500
+    n = 5000
     t0 = time.perf_counter()
     o.mcradiation(n, rng="philox", seed=99)
     dt = time.perf_counter() - t0
-    n2 = max(1000, int(n * seconds / max(dt, 1e-3)))
+    n2 = max(5000, int(n * seconds / max(dt, 1e-3)))
     t0 = time.perf_counter()
     o.mcradiation(n2, rng="philox", seed=98)
     dt = time.perf_counter() - t0

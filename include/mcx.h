@@ -182,6 +182,23 @@ typedef struct {
 int mcx_sed(mcx_ctx *ctx, const mcx_obs *obs, const double *T, const double *theta_av, int32_t nlam_obs,
             const double *lam_obs, double *flux, double *scatflux, double *fluxQ);
 
+/* Load a temperature state instead of computing one (the driver's
+ * restart from a structure file): T, and optionally dT and Ni [nr*nt]
+ * (NULL: zero), as mcx_temperature would leave them. */
+int mcx_set_temperature(mcx_ctx *ctx, const double *T, const double *dT, const int32_t *Ni);
+
+/* The diffusion approximation that follows the inversion in the driver
+ * (Diffuse, Diffusion.f:9-329, Main.f:561-563): thick cells (C%thick as
+ * InitRandomWalk sets it with factRW) seen by fewer than nphotmin packets or
+ * with dT/T > dTmax get T from the steady T^4 diffusion equation, solved
+ * densely (3 passes).  Works on the device T of the last mcx_temperature,
+ * which it updates; copies T, the thick flags and EJv (reference C%EJv: the
+ * solved cells' sum_s w_s Kp_s(T); other cells keep the values passed in,
+ * which may be NULL) back when the pointers are non-NULL.  R_av [nr],
+ * theta_av [nt]: cell centres.  ncells: number of cells that triggered it. */
+int mcx_diffuse(mcx_ctx *ctx, int32_t nphotmin, double dTmax, double factRW, const double *R_av,
+                const double *theta_av, double *T, int32_t *thick, double *EJv, int32_t *ncells);
+
 /* Duration in milliseconds of the last mcx_transport's packet kernel and of
  * the whole pass, both measured with HIP events on the context's stream. */
 int mcx_last_timing(const mcx_ctx *ctx, double *kernel_ms, double *pass_ms);

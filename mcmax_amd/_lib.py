@@ -73,6 +73,7 @@ class Obs(C.Structure):
 EXPORTS = (
     "mcx_create", "mcx_destroy", "mcx_last_error", "mcx_set_tables", "mcx_set_model",
     "mcx_transport", "mcx_get_tallies", "mcx_temperature", "mcx_last_timing", "mcx_sed",
+    "mcx_set_temperature", "mcx_diffuse",
 )
 
 _lib = None
@@ -101,8 +102,11 @@ def load():
     lib.mcx_temperature.argtypes = [vp, C.POINTER(Temps)]
     lib.mcx_last_timing.argtypes = [vp, _dp, _dp]
     lib.mcx_sed.argtypes = [vp, C.POINTER(Obs), _dp, _dp, C.c_int32, _dp, _dp, _dp, _dp]
+    lib.mcx_set_temperature.argtypes = [vp, _dp, _dp, _ip]
+    lib.mcx_diffuse.argtypes = [vp, C.c_int32, C.c_double, C.c_double, _dp, _dp, _dp, _ip, _dp,
+                                C.POINTER(C.c_int32)]
     for n in ("mcx_set_tables", "mcx_set_model", "mcx_transport", "mcx_get_tallies",
-              "mcx_temperature", "mcx_last_timing", "mcx_sed"):
+              "mcx_temperature", "mcx_last_timing", "mcx_sed", "mcx_set_temperature", "mcx_diffuse"):
         getattr(lib, n).restype = C.c_int
     _lib = lib
     return lib

@@ -5,7 +5,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRCS = [os.path.join(HERE, "csrc", "mcx.hip"), os.path.join(HERE, "csrc", "sed.hip")]
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("mcx.hip", "sed.hip", "diffuse.hip")]
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 OUT = os.path.join(HERE, "libmcx.so")
 
 HIPCC_FLAGS = [
@@ -19,7 +20,7 @@ HIPCC_FLAGS = [
 
 def build_lib(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", OUT + ".tmp"] + SRCS
+    cmd = [hipcc] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", OUT + ".tmp"] + SRCS + LIBS
     r = subprocess.run(cmd, capture_output=True, text=True)
     msgs = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
                      if "packed-fp32-ops" not in l and "is not a recognized feature" not in l)

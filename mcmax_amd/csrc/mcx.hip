@@ -1136,23 +1136,6 @@ __global__ void k_diffcoeff(int nclass, int nlam, int ng, int scattering, const 
     KD[idx] = make_double2(i2 / i1, i3 / i2);
 }
 
-// C%thick (RandomWalk.f:174-189): the shell is wider than factRW Rosseland
-// lengths at any of T = 100, 600, 1100, 1600 K.
-__global__ void k_thick(int nr, int nt, const double *__restrict__ R, const double *__restrict__ dens,
-                        const int *__restrict__ cls, const double2 *__restrict__ KD, double factRW, int *thick) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nr * nt) return;
-    const int i = c / nt;
-    int th = 0;
-    if (i > 0) {
-        for (int iT = 100; iT <= 1600; iT += 500) {
-            const double lr = 1.0 / ((MCX_AU * KD[(size_t)cls[c] * (TMAXD + 1) + iT].x) * dens[c]);
-            if ((R[i + 1] - R[i]) > factRW * lr) th = 1;
-        }
-    }
-    thick[c] = th;
-}
-
 __global__ void k_popcount(int ncell, int nwords, const unsigned *__restrict__ bitmap, int *Ni) {
     int c = blockIdx.x;
     if (c >= ncell) return;
@@ -1275,6 +1258,7 @@ extern "C" void mcx_destroy(mcx_ctx *ctx) {
     free_list(ctx->run_alloc);
     free_list(ctx->mod_alloc);
     free_list(ctx->tab_alloc);
+    release_blas(ctx);
     for (int k = 0; k < 4; k++)
         if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
     (void)hipStreamDestroy(ctx->stream);
@@ -1586,7 +1570,7 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
         if (!(run->factRW > 0.0)) return fail(ctx, MCX_EINVAL, "factRW must be positive");
         rw.on = 1;
         rw.factRW = run->factRW;
-        hipLaunchKernelGGL(k_thick, dim3((ncell + 255) / 256), dim3(256), 0, s, md.nr, md.nt, rw.R, md.dens, rw.cls,
+        hipLaunchKernelGGL(k_thick_cells, dim3((ncell + 255) / 256), dim3(256), 0, s, md.nr, md.nt, rw.R, md.dens, rw.cls,
                            rw.KD, run->factRW, ctx->d_thick);
         HIPCHK(ctx, hipGetLastError());
     } else if (ctx->d_thick) {
@@ -1759,6 +1743,22 @@ extern "C" int mcx_temperature(mcx_ctx *ctx, mcx_temps *o) {
         if (o->dT) HIPCHK(ctx, hipMemcpy(o->dT, ctx->d_dT, sizeof(double) * ncell, hipMemcpyDeviceToHost));
         if (o->dEJv) HIPCHK(ctx, hipMemcpy(o->dEJv, ctx->d_dEJv, sizeof(double) * ncell, hipMemcpyDeviceToHost));
     }
+    return MCX_OK;
+}
+
+extern "C" int mcx_set_temperature(mcx_ctx *ctx, const double *T, const double *dT, const int32_t *Ni) {
+    if (!ctx || !T) return MCX_EINVAL;
+    if (!ctx->have_model) return fail(ctx, MCX_ESTATE, "mcx_set_model first");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int ncell = ctx->md.ncell;
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_T, T, sizeof(double) * ncell, hipMemcpyHostToDevice, s));
+    if (dT) HIPCHK(ctx, hipMemcpyAsync(ctx->d_dT, dT, sizeof(double) * ncell, hipMemcpyHostToDevice, s));
+    else HIPCHK(ctx, hipMemsetAsync(ctx->d_dT, 0, sizeof(double) * ncell, s));
+    if (Ni) HIPCHK(ctx, hipMemcpyAsync(ctx->d_Ni, Ni, sizeof(int) * ncell, hipMemcpyHostToDevice, s));
+    else HIPCHK(ctx, hipMemsetAsync(ctx->d_Ni, 0, sizeof(int) * ncell, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    ctx->have_temps = true;
     return MCX_OK;
 }
 

@@ -101,6 +101,7 @@ struct mcx_ctx {
     double *d_wcls = nullptr; // [nclass][ng] weights of each species-mix class
     std::vector<double> h_R, h_Theta, h_thet, h_dens, h_w;
     bool have_temps = false;
+    void *blas = nullptr; // rocBLAS handle of the dense solves (mcx_diffuse)
     // model
     DevModel md{};
     double *d_V = nullptr, *d_R2 = nullptr;
@@ -153,3 +154,24 @@ static inline void free_list(std::vector<void *> &list) {
 }
 
 // y^k for integer k by right-to-left binary powering (Fortran's real**integer)
+
+// C%thick (RandomWalk.f:174-189): the shell is wider than factRW Rosseland
+// lengths at any of T = 100, 600, 1100, 1600 K.
+static __global__ void k_thick_cells(int nr, int nt, const double *__restrict__ R, const double *__restrict__ dens,
+                                     const int *__restrict__ cls, const double2 *__restrict__ KD, double factRW,
+                                     int *thick) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nr * nt) return;
+    const int i = c / nt;
+    int th = 0;
+    if (i > 0) {
+        for (int iT = 100; iT <= 1600; iT += 500) {
+            const double lr = 1.0 / ((MCX_AU * KD[(size_t)cls[c] * (TMAXD + 1) + iT].x) * dens[c]);
+            if ((R[i + 1] - R[i]) > factRW * lr) th = 1;
+        }
+    }
+    thick[c] = th;
+}
+
+// frees the context's rocBLAS handle (defined with the dense solves)
+void release_blas(mcx_ctx *ctx);

@@ -132,6 +132,30 @@ class Transport:
                                      ptr(out[1]), ptr(out[2])))
         return out.T.copy()
 
+    def set_temperature(self, T, dT=None, Ni=None):
+        """Load a temperature state (mcx_set_temperature)."""
+        Tc = _c(np.asarray(T).ravel())
+        dTc = _c(np.asarray(dT).ravel()) if dT is not None else None
+        Nic = np.ascontiguousarray(np.asarray(Ni).ravel(), dtype=np.int32) if Ni is not None else None
+        self._check(self.lib.mcx_set_temperature(self.ctx, ptr(Tc), ptr(dTc) if dTc is not None else None,
+                                                 ptr(Nic) if Nic is not None else None))
+
+    def diffuse(self, nphotmin=25, dtmax=1.0, factRW=None, EJv=None):
+        """Diffuse (mcx_diffuse) on the device temperatures; returns
+        (T, thick, EJv, ncells)."""
+        m = self.model
+        n = m.nr * m.nt
+        f = float(getattr(m, "factRW", 10.0)) if factRW is None else float(factRW)
+        T = np.zeros(n)
+        th = np.zeros(n, dtype=np.int32)
+        E = _c(np.asarray(EJv).ravel()).copy() if EJv is not None else np.zeros(n)
+        nc = C.c_int32()
+        Rav, tav = _c(m.R_av), _c(m.theta_av)
+        self._check(self.lib.mcx_diffuse(self.ctx, int(nphotmin), float(dtmax), f, ptr(Rav), ptr(tav), ptr(T),
+                                         ptr(th), ptr(E), C.byref(nc)))
+        sh = (m.nr, m.nt)
+        return T.reshape(sh), th.reshape(sh).astype(bool), E.reshape(sh), nc.value
+
     def timing(self):
         k, p = C.c_double(), C.c_double()
         self._check(self.lib.mcx_last_timing(self.ctx, C.byref(k), C.byref(p)))

@@ -2348,3 +2348,168 @@ int orc_trace_flux(const orc_model *m, const double *thet, const double *Tcell, 
     free(S.nrg); free(S.rgoff); free(S.thetrg);
     return err;
 }
+
+/* ---- diffusion approximation (Diffusion.f:9-329, FLD off) -------------- */
+
+/* Solve A x = b in place (A row-major n x n, destroyed; b -> x) by Gaussian
+ * elimination with partial pivoting, the pivot being the first entry of
+ * largest magnitude in its column (the LU factorisation DGESV computes).
+ * Returns 0, or k > 0 when U(k,k) is exactly zero. */
+int orc_lu_solve(int n, double *A, double *b)
+{
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        double amax = fabs(A[(size_t)k * n + k]);
+        for (int i = k + 1; i < n; i++) {
+            double a = fabs(A[(size_t)i * n + k]);
+            if (a > amax) { amax = a; p = i; }
+        }
+        if (amax == 0.0) return k + 1;
+        if (p != k) {
+            for (int j = 0; j < n; j++) {
+                double t = A[(size_t)k * n + j];
+                A[(size_t)k * n + j] = A[(size_t)p * n + j];
+                A[(size_t)p * n + j] = t;
+            }
+            double t = b[k]; b[k] = b[p]; b[p] = t;
+        }
+        double piv = A[(size_t)k * n + k];
+        for (int i = k + 1; i < n; i++) {
+            double f = A[(size_t)i * n + k] / piv;
+            if (f == 0.0) continue;
+            A[(size_t)i * n + k] = f;
+            for (int j = k + 1; j < n; j++) A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
+            b[i] -= f * b[k];
+        }
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double s = b[i];
+        for (int j = i + 1; j < n; j++) s -= A[(size_t)i * n + j] * b[j];
+        b[i] = s / A[(size_t)i * n + i];
+    }
+    return 0;
+}
+
+int orc_diffuse(const orc_model *m, const double *R_av, const double *theta_av, double *T, const double *dT,
+                const int32_t *Ni, int32_t *thick, double *EJv, int NphotMin, double dTMax, int *ncells)
+{
+    const int nr = m->nr, nt = m->nt, ng = m->ng, NTh = nt + 1;
+    /* reference indices: radial i = 0..nr-1, theta j = 1..nt (cell c = i*nt + j-1) */
+#define CI(i, j) ((i) * nt + (j) - 1)
+    int *number = (int *)calloc((size_t)(nr + 1) * (NTh + 1), sizeof(int));
+#define NUM(i, j) number[(size_t)(i) * (NTh + 1) + (j)]
+    int N = 0, ncor = 0;
+    for (int i = 2; i <= nr - 2; i++)
+        for (int j = 2; j <= NTh - 1; j++) {
+            int c = CI(i, j);
+            if (Ni[c] < NphotMin || (dT[c] / T[c]) > dTMax) {
+                if (thick[c] || Ni[c] <= 1) {
+                    ncor++;
+                    if (!NUM(i, j)) NUM(i, j) = ++N;
+                    if (!NUM(i - 1, j)) NUM(i - 1, j) = ++N;
+                    if (!NUM(i + 1, j)) NUM(i + 1, j) = ++N;
+                    if (!NUM(i, j - 1)) NUM(i, j - 1) = ++N;
+                    if (!NUM(i, j + 1) && j != NTh - 1) NUM(i, j + 1) = ++N;
+                }
+            }
+        }
+    if (ncells) *ncells = ncor;
+    if (N < 1) { free(number); return 0; }
+    int *celi = calloc(N + 1, sizeof(int)), *celj = calloc(N + 1, sizeof(int)), *ok = calloc(N + 1, sizeof(int));
+    double *r = calloc(N + 1, sizeof(double)), *t = calloc(N + 1, sizeof(double)), *Dc = calloc(N + 1, sizeof(double));
+    double *M = calloc((size_t)N * N, sizeof(double)), *T4 = calloc(N, sizeof(double));
+    for (int i = 1; i <= nr - 1; i++)
+        for (int j = 1; j <= NTh - 1; j++)
+            if (NUM(i, j)) {
+                int k = NUM(i, j);
+                celi[k] = i; celj[k] = j; r[k] = R_av[i]; t[k] = theta_av[j - 1];
+            }
+    int err = 0;
+    for (int iter = 1; iter <= 3 && !err; iter++) {
+        for (int k = 1; k <= N; k++) {
+            int i = celi[k], j = celj[k], c = CI(i, j);
+            int iT = (int)(T[c] / 1.0);
+            if (iT <= 1) iT = 1;
+            if (iT >= TMAX - 1) iT = TMAX - 1;
+            double KDext, KDabs;
+            orc_diffcoeff(m, c, iT, orc_rw_opacity_set(iT), &KDext, &KDabs);
+            double Kext = KDext - 0.0, dens = 0.0;
+            for (int s = 0; s < ng; s++) dens = dens + m->w[(size_t)c * ng + s] * m->dens[c];
+            Dc[k] = 1.0 / (Kext * dens);
+            if (((m->R[nr] - m->R[i]) * AU_C / Dc[k]) < 2.0) thick[c] = 0;
+            if (Ni[c] < NphotMin || (dT[c] / T[c]) > dTMax) {
+                ok[k] = 0;
+                if (i == 1 || i == nr - 1 || j == 1) ok[k] = 1;
+                if (!thick[c] && Ni[c] > 1) ok[k] = 1;
+            } else {
+                ok[k] = 1;
+            }
+        }
+        for (int k = 1; k <= N; k++) {
+            double *row = M + (size_t)(k - 1) * N;
+            for (int q = 0; q < N; q++) row[q] = 0.0;
+            int i = celi[k], j = celj[k], c = CI(i, j);
+            if (ok[k]) {
+                row[k - 1] = 1.0;
+                T4[k - 1] = powi_rl(T[c], 4u);
+                continue;
+            }
+            int j0 = NUM(i, j), jm = NUM(i - 1, j), jp = NUM(i + 1, j);
+            {
+                /* a row reaching an unnumbered neighbour would index outside the
+                 * system; every selected cell has all four numbered, so this
+                 * only guards invalid input */
+                int jt = (j != NTh - 1) ? NUM(i, j + 1) : j0;
+                if (jm < 1 || jp < 1 || NUM(i, j - 1) < 1 || jt < 1) {
+                    row[k - 1] = 1.0;
+                    T4[k - 1] = powi_rl(T[c], 4u);
+                    continue;
+                }
+            }
+            double dx0 = log(r[jp] / r[jm]), dx1 = log(r[jp] / r[j0]), dx2 = log(r[j0] / r[jm]);
+            double f1 = Dc[jp] * r[jp], f2 = Dc[jm] * r[jm];
+            double r3 = powi_rl(r[j0], 3u);
+            row[jm - 1] = row[jm - 1] + f2 / (((dx2 * dx0) * r3));
+            row[j0 - 1] = row[j0 - 1] - f2 / (((dx2 * dx0) * r3));
+            row[jp - 1] = row[jp - 1] + f1 / (((dx1 * dx0) * r3));
+            row[j0 - 1] = row[j0 - 1] - f1 / (((dx1 * dx0) * r3));
+            double tjm;
+            j0 = NUM(i, j);
+            if (j != NTh - 1) { jm = NUM(i, j + 1); tjm = t[jm]; }
+            else { jm = NUM(i, j); tjm = PI - t[jm]; }
+            jp = NUM(i, j - 1);
+            dx0 = t[jp] - tjm;
+            dx1 = t[jp] - t[j0];
+            dx2 = t[j0] - tjm;
+            f1 = Dc[jp] * sin(t[jp]);
+            f2 = Dc[jm] * sin(tjm);
+            double r2s = powi_rl(r[j0], 2u) * sin(t[j0]);
+            row[jm - 1] = row[jm - 1] + f2 / ((dx2 * dx0) * r2s);
+            row[j0 - 1] = row[j0 - 1] - f2 / ((dx2 * dx0) * r2s);
+            row[jp - 1] = row[jp - 1] + f1 / ((dx1 * dx0) * r2s);
+            row[j0 - 1] = row[j0 - 1] - f1 / ((dx1 * dx0) * r2s);
+            T4[k - 1] = 0.0;
+        }
+        if (orc_lu_solve(N, M, T4) != 0) err = -1;
+        for (int k = 1; k <= N && !err; k++) {
+            if (ok[k]) continue;
+            int i = celi[k], j = celj[k], c = CI(i, j);
+            double x = T4[k - 1];
+            if (x > 0.0) T[c] = pow(x, 0.25);
+            if (x < 0.0) T[c] = 1.0;
+            int iT = (int)(T[c] / 1.0);
+            if (iT <= 1) iT = 1;
+            if (iT >= TMAX - 1) iT = TMAX - 1;
+            double Etot = 0.0;
+            for (int s = 0; s < ng; s++) Etot = Etot + (m->w[(size_t)c * ng + s] * 1.0) * m->Kp[(size_t)s * (TMAX + 1) + iT];
+            EJv[c] = Etot;
+            if (T[c] > (double)((float)(TMAX - 1) * 1.0f)) T[c] = (double)((float)(TMAX - 1) * 1.0f);
+            if (T[c] < 1.0) T[c] = 1.0;
+            if (isnan(T[c]) || isinf(T[c])) T[c] = 1.0;
+        }
+    }
+    free(celi); free(celj); free(ok); free(r); free(t); free(Dc); free(M); free(T4); free(number);
+    return err;
+#undef NUM
+#undef CI
+}

@@ -171,6 +171,17 @@ int orc_trace_flux(const orc_model *m, const double *thet, const double *Tcell, 
 /* arrinterpol(x, xarr, yarr, n, log=1) (Subroutines.f:2506-2532) */
 double orc_arrinterpol_log(double x, const double *xarr, const double *yarr, int n);
 
+/* Diffusion approximation for thick cells with few packets (Diffuse,
+ * Diffusion.f:9-329, flux limiter off): T, thick and EJv are updated in
+ * place for the cells it solves; R_av [nr], theta_av [nt] are the cell
+ * centres.  ncells receives the number of cells that triggered it.
+ * Returns 0, or -1 when the linear system is singular. */
+int orc_diffuse(const orc_model *m, const double *R_av, const double *theta_av, double *T, const double *dT,
+                const int32_t *Ni, int32_t *thick, double *EJv, int NphotMin, double dTMax, int *ncells);
+/* Dense LU solve with partial pivoting (what DGESV computes); A row-major,
+ * overwritten; b becomes x.  Returns 0 or the 1-based index of a zero pivot. */
+int orc_lu_solve(int n, double *A, double *b);
+
 /* Straight-ray geometry through the grid (trace2d with tau0 = inf):
  * records up to maxseg (cell, length) pairs; returns number of segments,
  * -1 on a geometry failure.  onEdge/edgeNr as the reference photon.       */

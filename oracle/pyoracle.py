@@ -126,6 +126,9 @@ def load():
     lib.orc_diffcoeff.argtypes = [C.POINTER(OrcModel), C.c_int, C.c_int, C.c_int, _dp, _dp]
     lib.orc_init_random_walk.argtypes = [C.POINTER(OrcModel), _dp, _dp, _ip]
     lib.orc_rw_opacity_set.argtypes = [C.c_int]
+    lib.orc_lu_solve.argtypes = [C.c_int, _dp, _dp]
+    lib.orc_diffuse.argtypes = [C.POINTER(OrcModel), _dp, _dp, _dp, _dp, _ip, _ip, _dp, C.c_int, C.c_double,
+                                C.POINTER(C.c_int)]
     lib.orc_min_dist.argtypes = [C.POINTER(OrcModel), C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
     lib.orc_min_dist.restype = C.c_double
     _lib = lib
@@ -172,6 +175,25 @@ class Oracle:
         a, b = C.c_double(), C.c_double()
         self.lib.orc_diffcoeff(C.byref(self.om), int(c), int(iT), int(opacity_set), C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def diffuse(self, T, dT, Ni, thick, EJv, nphotmin=25, dtmax=1.0):
+        """Diffuse (Diffusion.f:9-329) on copies of T, thick, EJv; returns
+        (T, thick, EJv, ncells)."""
+        m = self.m
+        T = np.ascontiguousarray(T, dtype=np.float64).ravel().copy()
+        EJv = np.ascontiguousarray(EJv, dtype=np.float64).ravel().copy()
+        dT = np.ascontiguousarray(dT, dtype=np.float64).ravel()
+        Ni = np.ascontiguousarray(Ni, dtype=np.int32).ravel()
+        th = np.ascontiguousarray(thick, dtype=np.int32).ravel().copy()
+        Rav = np.ascontiguousarray(m.R_av, dtype=np.float64)
+        tav = np.ascontiguousarray(m.theta_av, dtype=np.float64)
+        nc = C.c_int()
+        rc = self.lib.orc_diffuse(C.byref(self.om), p(Rav), p(tav), p(T), p(dT), p(Ni), p(th), p(EJv),
+                                  int(nphotmin), float(dtmax), C.byref(nc))
+        if rc != 0:
+            raise RuntimeError(f"oracle diffuse failed: {rc}")
+        sh = (m.nr, m.nt)
+        return T.reshape(sh), th.reshape(sh).astype(bool), EJv.reshape(sh), nc.value
 
     def min_dist(self, x, y, z, i, jj):
         return self.lib.orc_min_dist(C.byref(self.om), x, y, z, int(i), int(jj))
