@@ -199,6 +199,65 @@ int mcx_set_temperature(mcx_ctx *ctx, const double *T, const double *dT, const i
 int mcx_diffuse(mcx_ctx *ctx, int32_t nphotmin, double dTmax, double factRW, const double *R_av,
                 const double *theta_av, double *T, int32_t *thick, double *EJv, int32_t *ncells);
 
+/* TempAverage(fw) (DiskStructure.f:25-79): the running average of the
+ * temperatures the vertical structure is solved for, kept on the device.
+ * first != 0 stores the current temperatures (the reference's first call,
+ * which allocates the average); otherwise Tav = (1-fw) Tav + fw max(T, 2.7).
+ * T is that of the last mcx_temperature / mcx_diffuse / mcx_set_temperature.
+ * The driver's weights (Main.f:683-690): 1 on the first iteration, f_weight
+ * [0.5] after, 1/(niter-niter0) beyond niter0.  Tav [nr*nt] may be NULL. */
+int mcx_temp_average(mcx_ctx *ctx, double fw, int32_t first, double *Tav);
+
+/* Vertical-structure settings (DiskStructure.f:137-169; Init.f defaults in
+ * brackets).  Every species has the scale-height type DISK. */
+typedef struct {
+    const double *R_av, *theta_av;   /* cell centres [nr] cm, [nt] rad      */
+    double Mstar;                    /* D%Mstar in g                        */
+    const double *shscale;           /* [nr] shscale(i) (Init.f:2754-2763)
+                                        [NULL: 1]; < 0 makes a wedge        */
+    const double *gshscale;          /* [ng*nr] Grain(ii)%shscale(i), used
+                                        where settle (Init.f:3413-3425)
+                                        [NULL: 1]                           */
+    const int32_t *settle;           /* [ng] Grain%settle [NULL: none]      */
+    const int32_t *gascoupled;       /* [ng] Grain%gascoupled [NULL: none]  */
+    int32_t mpstr;                   /* keyword mpstr: midplane T for the
+                                        whole column [0]                    */
+    const double *FradZ;             /* [nr*nt] C%FradZ (radiation pressure,
+                                        MCRadiation.f:835) [NULL: 0]        */
+} mcx_struct;
+
+/* The density state DiskStructure reads and rewrites: C%dens, C%dens0,
+ * C%gasdens [nr*nt] and C%w, C%w0 [nr*nt*ng] (Modules.f:85-121).  At set-up
+ * dens0 = gasdens = dens and w0 = w (Init.f:3899-3904). */
+typedef struct {
+    double *dens, *dens0, *gasdens;
+    double *w, *w0;
+} mcx_density;
+
+/* DiskStructure (DiskStructure.f:86-453): every column (radius, gas or
+ * species) in hydrostatic equilibrium at the temperatures Temp [nr*nt]
+ * (NULL: the device average of mcx_temp_average), integrated from the
+ * midplane up with the reference's adaptive Cash-Karp scheme and
+ * renormalised to the column's mass; then the abundances, the minimum
+ * densities (CheckMinimumDensity) and CheckCells.  d is read and
+ * rewritten.  The next transport pass sees the new structure after
+ * mcx_set_model with d->dens and d->w.  nsteps (may be NULL) receives the
+ * number of Runge-Kutta steps. */
+int mcx_disk_structure(mcx_ctx *ctx, const mcx_struct *st, const double *Temp, mcx_density *d, int64_t *nsteps);
+
+/* One state of the cells for compareCells: C%T, C%dT, C%dens, C%ddens,
+ * C%EJv (per volume), C%dEJv [nr*nt]. */
+typedef struct {
+    const double *T, *dT, *dens, *ddens, *EJv, *dEJv;
+} mcx_cellstate;
+
+/* compareCells (DiskStructure.f:553-584): the driver's convergence test
+ * (Main.f:734, errT < epsiter [3]).  Cells with dT <= 1e-200 in either
+ * state do not count; errRho divides by ddens, which the reference never
+ * sets (zero: errRho is then not finite, and the driver does not print it). */
+int mcx_compare_cells(mcx_ctx *ctx, const mcx_cellstate *prev, const mcx_cellstate *cur, double *errT,
+                      double *errRho, double *errE);
+
 /* Duration in milliseconds of the last mcx_transport's packet kernel and of
  * the whole pass, both measured with HIP events on the context's stream. */
 int mcx_last_timing(const mcx_ctx *ctx, double *kernel_ms, double *pass_ms);

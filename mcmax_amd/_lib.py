@@ -70,10 +70,25 @@ class Obs(C.Structure):
     ]
 
 
+class Struct(C.Structure):
+    _fields_ = [
+        ("R_av", _dp), ("theta_av", _dp), ("Mstar", C.c_double), ("shscale", _dp), ("gshscale", _dp),
+        ("settle", _ip), ("gascoupled", _ip), ("mpstr", C.c_int32), ("FradZ", _dp),
+    ]
+
+
+class Density(C.Structure):
+    _fields_ = [("dens", _dp), ("dens0", _dp), ("gasdens", _dp), ("w", _dp), ("w0", _dp)]
+
+
+class CellState(C.Structure):
+    _fields_ = [("T", _dp), ("dT", _dp), ("dens", _dp), ("ddens", _dp), ("EJv", _dp), ("dEJv", _dp)]
+
+
 EXPORTS = (
     "mcx_create", "mcx_destroy", "mcx_last_error", "mcx_set_tables", "mcx_set_model",
     "mcx_transport", "mcx_get_tallies", "mcx_temperature", "mcx_last_timing", "mcx_sed",
-    "mcx_set_temperature", "mcx_diffuse",
+    "mcx_set_temperature", "mcx_diffuse", "mcx_temp_average", "mcx_disk_structure", "mcx_compare_cells",
 )
 
 _lib = None
@@ -105,8 +120,12 @@ def load():
     lib.mcx_set_temperature.argtypes = [vp, _dp, _dp, _ip]
     lib.mcx_diffuse.argtypes = [vp, C.c_int32, C.c_double, C.c_double, _dp, _dp, _dp, _ip, _dp,
                                 C.POINTER(C.c_int32)]
+    lib.mcx_temp_average.argtypes = [vp, C.c_double, C.c_int32, _dp]
+    lib.mcx_disk_structure.argtypes = [vp, C.POINTER(Struct), _dp, C.POINTER(Density), C.POINTER(C.c_int64)]
+    lib.mcx_compare_cells.argtypes = [vp, C.POINTER(CellState), C.POINTER(CellState), _dp, _dp, _dp]
     for n in ("mcx_set_tables", "mcx_set_model", "mcx_transport", "mcx_get_tallies",
-              "mcx_temperature", "mcx_last_timing", "mcx_sed", "mcx_set_temperature", "mcx_diffuse"):
+              "mcx_temperature", "mcx_last_timing", "mcx_sed", "mcx_set_temperature", "mcx_diffuse",
+              "mcx_temp_average", "mcx_disk_structure", "mcx_compare_cells"):
         getattr(lib, n).restype = C.c_int
     _lib = lib
     return lib

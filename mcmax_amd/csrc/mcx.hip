@@ -1259,6 +1259,7 @@ extern "C" void mcx_destroy(mcx_ctx *ctx) {
     free_list(ctx->mod_alloc);
     free_list(ctx->tab_alloc);
     release_blas(ctx);
+    release_structure(ctx);
     for (int k = 0; k < 4; k++)
         if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
     (void)hipStreamDestroy(ctx->stream);

@@ -1,5 +1,6 @@
 // mcx_internal.h -- definitions shared by the translation units of libmcx.so
-// (transport: mcx.hip; ray-traced SED: sed.hip): device-resident tables and
+// (transport: mcx.hip; ray-traced SED: sed.hip; diffusion: diffuse.hip;
+// vertical structure: structure.hip): device-resident tables and
 // model, tallies, and the host context behind the C-ABI of include/mcx.h.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -102,6 +103,9 @@ struct mcx_ctx {
     std::vector<double> h_R, h_Theta, h_thet, h_dens, h_w;
     bool have_temps = false;
     void *blas = nullptr; // rocBLAS handle of the dense solves (mcx_diffuse)
+    double *d_Tav = nullptr; // TempAverage's running average (mcx_temp_average)
+    int tav_ncell = 0;
+    bool have_tav = false;
     // model
     DevModel md{};
     double *d_V = nullptr, *d_R2 = nullptr;
@@ -153,8 +157,6 @@ static inline void free_list(std::vector<void *> &list) {
     list.clear();
 }
 
-// y^k for integer k by right-to-left binary powering (Fortran's real**integer)
-
 // C%thick (RandomWalk.f:174-189): the shell is wider than factRW Rosseland
 // lengths at any of T = 100, 600, 1100, 1600 K.
 static __global__ void k_thick_cells(int nr, int nt, const double *__restrict__ R, const double *__restrict__ dens,
@@ -175,3 +177,5 @@ static __global__ void k_thick_cells(int nr, int nt, const double *__restrict__ 
 
 // frees the context's rocBLAS handle (defined with the dense solves)
 void release_blas(mcx_ctx *ctx);
+// frees the averaged temperatures (defined with the vertical structure)
+void release_structure(mcx_ctx *ctx);

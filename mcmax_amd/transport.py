@@ -49,9 +49,10 @@ class Transport:
             raise MCXError(rc, self.lib.mcx_last_error(self.ctx).decode())
         return rc
 
-    def set_model(self, m):
+    def set_model(self, m, tables=True):
         """Hand a model's tables and grid to the device (mcx_set_tables,
-        mcx_set_model)."""
+        mcx_set_model).  tables=False re-sends only the grid, densities and
+        abundances (after a new vertical structure)."""
         keep = dict(
             lam=_c(m.lam), nu=_c(m.nu), BB=_c(m.BB), Kabs=_c(m.Kabs), Ksca=_c(m.Ksca),
             F=_c(m.F), IF11=_c(m.IF11), IF12=_c(m.IF12), Kp=_c(m.Kp), Fstar=_c(m.Fstar),
@@ -63,7 +64,8 @@ class Transport:
                         Kp=ptr(keep["Kp"]), Fstar=ptr(keep["Fstar"]), Lstar=float(m.Lstar),
                         cos2phi=ptr(keep["c2"]), sin2phi=ptr(keep["s2"]), scat_how=int(m.scat_how),
                         Kext=ptr(keep["Kext"]), gsym=ptr(keep["gsym"]))
-        self._check(self.lib.mcx_set_tables(self.ctx, C.byref(t)))
+        if tables:
+            self._check(self.lib.mcx_set_tables(self.ctx, C.byref(t)))
         g = _lib.Grid(nr=m.nr, nt=m.nt, R=ptr(keep["R"]), Theta=ptr(keep["Th"]), dens=ptr(keep["dens"]),
                       V=ptr(keep["V"]), w=ptr(keep["w"]), Rstar_cm=float(m.Rstar_cm), Tstar=float(m.Tstar),
                       thet=ptr(keep["thet"]))
@@ -155,6 +157,50 @@ class Transport:
                                          ptr(th), ptr(E), C.byref(nc)))
         sh = (m.nr, m.nt)
         return T.reshape(sh), th.reshape(sh).astype(bool), E.reshape(sh), nc.value
+
+    def temp_average(self, fw, first=False):
+        """TempAverage (mcx_temp_average) of the device temperatures;
+        returns the averaged field [nr, nt]."""
+        m = self.model
+        out = np.zeros(m.nr * m.nt)
+        self._check(self.lib.mcx_temp_average(self.ctx, float(fw), int(bool(first)), ptr(out)))
+        return out.reshape(m.nr, m.nt)
+
+    def disk_structure(self, state, Temp=None, shscale=None, gshscale=None, settle=None, gascoupled=None,
+                       mpstr=False, FradZ=None):
+        """DiskStructure (mcx_disk_structure) on a copy of state = dict(dens,
+        dens0, gasdens [nr, nt], w, w0 [nr, nt, ng]); Temp None uses the
+        device's averaged temperatures.  Returns (new state, Runge-Kutta
+        steps)."""
+        m = self.model
+        sh = (m.nr, m.nt)
+        out = {k: _c(state[k]).copy() for k in ("dens", "dens0", "gasdens", "w", "w0")}
+        i32 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+        keep = dict(Rav=_c(m.R_av), tav=_c(m.theta_av),
+                    sh=None if shscale is None else _c(shscale), gsh=None if gshscale is None else _c(gshscale),
+                    st=i32(settle), gc=i32(gascoupled), fz=None if FradZ is None else _c(FradZ),
+                    T=None if Temp is None else _c(np.asarray(Temp).ravel()))
+        st = _lib.Struct(R_av=ptr(keep["Rav"]), theta_av=ptr(keep["tav"]), Mstar=float(m.Mstar_g),
+                         shscale=ptr(keep["sh"]), gshscale=ptr(keep["gsh"]), settle=ptr(keep["st"]),
+                         gascoupled=ptr(keep["gc"]), mpstr=int(bool(mpstr)), FradZ=ptr(keep["fz"]))
+        d = _lib.Density(**{k: ptr(v) for k, v in out.items()})
+        n = C.c_int64()
+        self._check(self.lib.mcx_disk_structure(self.ctx, C.byref(st), ptr(keep["T"]), C.byref(d), C.byref(n)))
+        for k in ("dens", "dens0", "gasdens"):
+            out[k] = out[k].reshape(sh)
+        return out, n.value
+
+    def compare_cells(self, prev, cur):
+        """compareCells (mcx_compare_cells) of two dicts of T, dT, dens,
+        ddens, EJv, dEJv [nr, nt]; returns (errT, errRho, errE)."""
+        keys = ("T", "dT", "dens", "ddens", "EJv", "dEJv")
+        kp = [_c(np.asarray(prev[k]).ravel()) for k in keys]
+        kc = [_c(np.asarray(cur[k]).ravel()) for k in keys]
+        a = _lib.CellState(*[ptr(x) for x in kp])
+        b = _lib.CellState(*[ptr(x) for x in kc])
+        e = [C.c_double() for _ in range(3)]
+        self._check(self.lib.mcx_compare_cells(self.ctx, C.byref(a), C.byref(b), *[C.byref(x) for x in e]))
+        return tuple(x.value for x in e)
 
     def timing(self):
         k, p = C.c_double(), C.c_double()

@@ -2513,3 +2513,287 @@ int orc_diffuse(const orc_model *m, const double *R_av, const double *theta_av, 
 #undef NUM
 #undef CI
 }
+
+/* ---------------------------------------------------------------------- */
+/* Vertical structure: TempAverage, DiskStructure, derivs, compareCells     */
+/* (DiskStructure.f:25-584) and the Numerical Recipes integrator it calls   */
+/* (odeint/rkqs/rkck, Subroutines.f:1933-2085).                             */
+
+void orc_temp_average(int ncell, const double *T, double *Temp, double fw, int first)
+{
+    /* DiskStructure.f:35-43: the first call stores C%T unchanged */
+    if (first) {
+        for (int c = 0; c < ncell; c++) Temp[c] = T[c];
+        return;
+    }
+    /* DiskStructure.f:45-76: floor 2.7 (a default-REAL literal), then the
+     * weighted running average */
+    const double Tfloor = (double)2.7f;
+    for (int c = 0; c < ncell; c++) {
+        double t = T[c];
+        if (t < Tfloor) t = Tfloor;
+        Temp[c] = (1.0 - fw) * Temp[c] + fw * t;
+    }
+}
+
+/* derivs (DiskStructure.f:469-493): mu = 2.3 m_p with 2.3 a default REAL,
+ * G in cgs; kb = 1.3806503d-16 (Modules.f:11).  T and beta are linear in z
+ * between the interval's end points; dy/dz does not depend on y. */
+typedef struct {
+    double z1, z2, T1, T2, F1, F2, scale, Mstar, R_av;
+} vseg;
+
+static double vs_derivs(const vseg *s, double x)
+{
+    const double mu = (double)2.3f * 1.67262158e-24, G = 6.67300e-8, kb = 1.3806503e-16;
+    double beta = s->F1 + (s->F2 - s->F1) * (x - s->z1) / (s->z2 - s->z1);
+    double T = s->T1 + (s->T2 - s->T1) * (x - s->z1) / (s->z2 - s->z1);
+    double dlnT = (log(s->T1) - log(s->T2)) / (s->z1 - s->z2);
+    double R3 = powi_rl(s->R_av, 3u);
+    return -((((mu * G) * s->Mstar) * (1.0 + beta)) * x) / ((((s->scale * s->scale) * kb) * T) * R3) - dlnT;
+}
+
+/* rkck (Subroutines.f:2042-2085): Cash-Karp tableau.  Every coefficient is
+ * a default-REAL constant expression (single precision), converted to
+ * real*8 when assigned; DCi = Ci - (single) b*_i in double. */
+#define RK_F(a) ((double)(float)(a))
+static void vs_rkck(const vseg *s, double y, double dydx, double x, double h, double *yout, double *yerr)
+{
+    const double A2 = RK_F(.2f), A3 = RK_F(.3f), A4 = RK_F(.6f), A5 = 1.0, A6 = RK_F(.875f);
+    const double C1 = RK_F(37.f / 378.f), C3 = RK_F(250.f / 621.f), C4 = RK_F(125.f / 594.f),
+                 C6 = RK_F(512.f / 1771.f);
+    const double DC1 = C1 - RK_F(2825.f / 27648.f), DC3 = C3 - RK_F(18575.f / 48384.f),
+                 DC4 = C4 - RK_F(13525.f / 55296.f), DC5 = RK_F(-277.f / 14336.f), DC6 = C6 - RK_F(.25f);
+    /* the intermediate y values (B coefficients) feed derivs only through
+     * y, which this right-hand side ignores; x + A*h is what matters */
+    double ak3, ak4, ak5, ak6;
+    (void)vs_derivs(s, x + A2 * h);
+    ak3 = vs_derivs(s, x + A3 * h);
+    ak4 = vs_derivs(s, x + A4 * h);
+    ak5 = vs_derivs(s, x + A5 * h);
+    ak6 = vs_derivs(s, x + A6 * h);
+    *yout = y + h * (C1 * dydx + C3 * ak3 + C4 * ak4 + C6 * ak6);
+    *yerr = h * (DC1 * dydx + DC3 * ak3 + DC4 * ak4 + DC5 * ak5 + DC6 * ak6);
+}
+
+/* rkqs (Subroutines.f:1998-2036) for one variable */
+static void vs_rkqs(const vseg *s, double *y, double dydx, double *x, double htry, double eps, double yscal,
+                    double *hdid, double *hnext)
+{
+    const double SAFETY = RK_F(0.9f), PGROW = RK_F(-.2f), PSHRNK = RK_F(-.25f), ERRCON = RK_F(1.89e-4f);
+    double h = htry, ytemp, yerr;
+    for (;;) {
+        vs_rkck(s, *y, dydx, *x, h, &ytemp, &yerr);
+        double errmax = 0.0;
+        errmax = fmax(errmax, fabs(yerr / yscal));
+        errmax = errmax / eps;
+        if (errmax > 1.0) {
+            double htemp = (SAFETY * h) * pow(errmax, PSHRNK);
+            h = copysign(fmax(fabs(htemp), RK_F(0.1f) * fabs(h)), h);
+            continue;
+        }
+        if (errmax > ERRCON) *hnext = (SAFETY * h) * pow(errmax, PGROW);
+        else *hnext = 5.0 * h;
+        *hdid = h;
+        *x = *x + h;
+        *y = ytemp;
+        return;
+    }
+}
+
+/* odeint (Subroutines.f:1933-1995) with kmax = 0 (no stored path).  On
+ * "too many steps" or a step below hmin ystart is left unchanged. */
+static double vs_odeint(const vseg *s, double ystart, double x1, double x2, double eps, double h1, double hmin,
+                        int *nsteps)
+{
+    const int MAXSTP = 10000;
+    const double TINY = 1.e-150;
+    double x = x1, h = copysign(h1, x2 - x1), y = ystart, hdid, hnext;
+    for (int nstp = 1; nstp <= MAXSTP; nstp++) {
+        double dydx = vs_derivs(s, x);
+        double yscal = fabs(y) + fabs(h * dydx) + TINY;
+        if ((x + h - x2) * (x + h - x1) > 0.0) h = x2 - x;
+        vs_rkqs(s, &y, dydx, &x, h, eps, yscal, &hdid, &hnext);
+        if (nsteps) (*nsteps)++;
+        if ((x - x2) * (x2 - x1) >= 0.0) return y;
+        if (fabs(hnext) < hmin) return ystart;
+        h = hnext;
+    }
+    return ystart;
+}
+
+double orc_vertical_step(double y0, double z1, double z2, double T1, double T2, double F1, double F2,
+                         double scale, double Mstar, double R_av, int *nsteps)
+{
+    vseg s = {z1, z2, T1, T2, F1, F2, scale, Mstar, R_av};
+    return vs_odeint(&s, y0, z1, z2, 1e-6, fabs(z2 - z1) * 1e-6, 0.0, nsteps);
+}
+
+/* CheckMinimumDensity (DiskStructure.f:2966-3012) of one cell */
+static void check_min_density(int ng, double *dens, double *dens0, double *gasdens, double *w, const double *w0)
+{
+    double tot[64];
+    if (*dens0 <= 1e-50) *dens0 = 1e-50;
+    if (*dens <= 1e-50) *dens = 1e-50;
+    if (*gasdens <= 1e-50) *gasdens = 1e-50;
+    for (int s = 0; s < ng; s++) tot[s] = *dens * w[s];
+    for (int s = 0; s < ng; s++) {
+        if (tot[s] < 1e-50) tot[s] = 1e-50;
+        if (tot[s] > *dens0 * w0[s]) tot[s] = *dens0 * w0[s];
+    }
+    double d = 0.0;
+    for (int s = 0; s < ng; s++) d = d + tot[s];
+    *dens = d;
+    if (*dens <= 1e-50) {
+        *dens = 1e-50;
+        for (int s = 0; s < ng; s++) w[s] = w0[s];
+    } else {
+        for (int s = 0; s < ng; s++) w[s] = tot[s] / *dens;
+    }
+}
+
+/* CheckCells (DiskStructure.f:2659-2702) of one cell */
+static void check_cell(int ng, double *dens, const double *dens0, double *w, const double *w0)
+{
+    double mass[64];
+    int rescale = 0;
+    for (int s = 0; s < ng; s++) {
+        mass[s] = *dens * w[s];
+        double mmax = *dens0 * w0[s];
+        if (mass[s] > mmax) {
+            mass[s] = mmax;
+            rescale = 1;
+        }
+    }
+    if (!rescale) return;
+    double d = 0.0;
+    for (int s = 0; s < ng; s++) d = d + mass[s];
+    *dens = d;
+    if (*dens > 1e-50) {
+        for (int s = 0; s < ng; s++) w[s] = mass[s] / *dens;
+    } else {
+        *dens = 1e-50;
+        for (int s = 0; s < ng; s++) w[s] = w0[s];
+    }
+}
+
+long orc_disk_structure(const orc_struct *s, const double *Temp, double *dens, double *dens0, double *gasdens,
+                        double *w, double *w0, double *rho_out)
+{
+    const int nr = s->nr, nt = s->nt, ng = s->ng;
+    if (ng > 63) return -1;
+    double *rho = (double *)malloc(sizeof(double) * (size_t)(ng + 1) * nt);
+    double *z = (double *)malloc(sizeof(double) * nt), *T2 = (double *)malloc(sizeof(double) * nt);
+    long nsteps = 0;
+#define RH(ii, j) rho[(size_t)(ii) * nt + (j)]
+    for (int i = 1; i < nr; i++) {
+        const double *Fz = s->FradZ ? s->FradZ + (size_t)i * nt : NULL;
+        for (int ii = 0; ii <= ng; ii++) {
+            /* DiskStructure.f:137-147: Temp2, or the midplane value */
+            for (int j = 0; j < nt; j++) T2[j] = s->mpstr ? Temp[(size_t)i * nt + nt - 1] : Temp[(size_t)i * nt + j];
+            /* 150-157: the scale factor of the column */
+            double scale;
+            if (ii != 0) {
+                scale = s->shscale[i];
+                if (s->settle[ii - 1]) scale = scale * s->gshscale[(size_t)(ii - 1) * nr + i];
+                if (s->gascoupled[ii - 1] && scale < 1.0) scale = 1.0;
+            } else {
+                scale = 1.0 * s->shscale[i];
+            }
+            /* 160-169: the column's mass */
+            double M0 = 0.0;
+            for (int j = 0; j < nt; j++) {
+                size_t c = (size_t)i * nt + j;
+                if (ii != 0) M0 = M0 + (dens0[c] * s->V[c]) * w0[c * ng + ii - 1];
+                else M0 = M0 + gasdens[c] * s->V[c];
+                z[j] = s->R_av[i] * cos(s->theta_av[j]);
+            }
+            /* 200-236: hydrostatic columns from the midplane up, or a wedge */
+            RH(ii, nt - 1) = 0.0;
+            if (scale > 0.0) {
+                for (int j = nt - 2; j >= 0; j--) {
+                    int j1 = j + 1, j2 = j, ns = 0;
+                    vseg sg = {z[j1], z[j2], T2[j1], T2[j2], Fz ? Fz[j1] : 0.0, Fz ? Fz[j2] : 0.0, scale,
+                               s->Mstar, s->R_av[i]};
+                    double h1 = fabs(z[j2] - z[j1]) * 1e-6;
+                    RH(ii, j2) = vs_odeint(&sg, RH(ii, j1), z[j1], z[j2], 1e-6, h1, 0.0, &ns);
+                    nsteps += ns;
+                }
+            } else {
+                for (int j = nt - 2; j >= 0; j--) RH(ii, j) = ((1.0 / tan(s->theta_av[j])) < -scale) ? 0.0 : -60.0;
+            }
+            /* 378-389: renormalise to the column's mass, floor 1e-50 */
+            double M1 = 0.0;
+            for (int j = 0; j < nt; j++) M1 = M1 + exp(RH(ii, j)) * s->V[(size_t)i * nt + j];
+            for (int j = 0; j < nt; j++) {
+                RH(ii, j) = (exp(RH(ii, j)) * M0) / M1;
+                if (RH(ii, j) < 1e-50) RH(ii, j) = 1e-50;
+            }
+            if (rho_out)
+                for (int j = 0; j < nt; j++) rho_out[((size_t)ii * nr + i) * nt + j] = RH(ii, j);
+        }
+        /* 399-447: densities and abundances of the cells of this radius */
+        for (int j = 0; j < nt; j++) {
+            size_t c = (size_t)i * nt + j;
+            double *wc = w + c * ng, *w0c = w0 + c * ng, gasf[64];
+            for (int q = 0; q < ng; q++) {
+                if (w0c[q] >= 1e-100) gasf[q] = 1.0 - (dens[c] * wc[q]) / (dens0[c] * w0c[q]);
+                else gasf[q] = 0.0;
+                if (gasf[q] < 0.0) gasf[q] = 0.0;
+                if (gasf[q] > 1.0) gasf[q] = 1.0;
+                if (isnan(gasf[q])) gasf[q] = 0.0;
+            }
+            gasdens[c] = RH(0, j);
+            double d0 = 0.0;
+            for (int q = 0; q < ng; q++) d0 = d0 + RH(q + 1, j);
+            dens0[c] = d0;
+            if (dens0[c] > 1e-50)
+                for (int q = 0; q < ng; q++) w0c[q] = RH(q + 1, j) / dens0[c];
+            double tot = 0.0;
+            for (int q = 0; q < ng; q++) tot = tot + w0c[q];
+            for (int q = 0; q < ng; q++) {
+                w0c[q] = w0c[q] / tot;
+                wc[q] = w0c[q] * (1.0 - gasf[q]);
+            }
+            tot = 0.0;
+            for (int q = 0; q < ng; q++) tot = tot + wc[q];
+            if (tot > 1e-50) {
+                for (int q = 0; q < ng; q++) wc[q] = wc[q] / tot;
+            } else {
+                for (int q = 0; q < ng; q++) wc[q] = w0c[q];
+            }
+            dens[c] = dens0[c] * tot;
+            check_min_density(ng, &dens[c], &dens0[c], &gasdens[c], wc, w0c);
+        }
+    }
+#undef RH
+    /* CheckCells over every radius, the void row included */
+    for (int c = 0; c < nr * nt; c++) check_cell(ng, &dens[c], &dens0[c], w + (size_t)c * ng, w0 + (size_t)c * ng);
+    free(rho); free(z); free(T2);
+    return nsteps;
+}
+
+void orc_compare_cells(int nr, int nt, const double *T1, const double *dT1, const double *dens1,
+                       const double *ddens1, const double *EJv1, const double *dEJv1, const double *T2,
+                       const double *dT2, const double *dens2, const double *ddens2, const double *EJv2,
+                       const double *dEJv2, double *errT, double *errRho, double *errE)
+{
+    double eT = 0.0, eR = 0.0, eE = 0.0;
+    for (int i = 1; i < nr; i++)
+        for (int j = 0; j < nt; j++) {
+            size_t c = (size_t)i * nt + j;
+            if (dT1[c] > 1e-200 && dT2[c] > 1e-200) {
+                double d = T1[c] - T2[c];
+                eT = eT + (d * d) / (dT1[c] * dT1[c] + dT2[c] * dT2[c]);
+                d = dens1[c] - dens2[c];
+                eR = eR + (d * d) / (ddens1[c] * ddens1[c] + ddens2[c] * ddens2[c]);
+                d = EJv1[c] - EJv2[c];
+                eE = eE + (d * d) / (dEJv1[c] * dEJv1[c] + dEJv2[c] * dEJv2[c]);
+            }
+        }
+    /* real(nR-1)*real(nTheta-1) is a single-precision product */
+    double N = (double)((float)(nr - 1) * (float)nt) - 1.0;
+    *errT = eT / N;
+    *errRho = sqrt(eR / N);
+    *errE = eE / N;
+}

@@ -182,6 +182,42 @@ int orc_diffuse(const orc_model *m, const double *R_av, const double *theta_av, 
  * overwritten; b becomes x.  Returns 0 or the 1-based index of a zero pivot. */
 int orc_lu_solve(int n, double *A, double *b);
 
+/* Vertical structure inputs (DiskStructure, DiskStructure.f:86-453).      */
+typedef struct {
+    int nr, nt, ng;
+    const double *R_av, *theta_av;   /* cell centres [nr] (cm), [nt] (rad)   */
+    const double *V;                 /* [nr*nt]                              */
+    double Mstar;                    /* D%Mstar, g                           */
+    const double *shscale;           /* [nr] (Init.f:2754-2763)              */
+    const double *gshscale;          /* [ng*nr] Grain%shscale (Init.f:3413)  */
+    const int32_t *settle;           /* [ng] Grain%settle                    */
+    const int32_t *gascoupled;       /* [ng] Grain%gascoupled                */
+    int mpstr;                       /* isothermal columns at midplane T     */
+    const double *FradZ;             /* [nr*nt] or NULL (zero)               */
+} orc_struct;
+
+/* TempAverage(fw) (DiskStructure.f:25-79) on the averaged field Temp[nr*nt]:
+ * first != 0 is the call that allocates it (a copy of T). */
+void orc_temp_average(int ncell, const double *T, double *Temp, double fw, int first);
+/* odeint/rkqs/rkck (Subroutines.f:1933-2085) on dy/dz of derivs
+ * (DiskStructure.f:469-493) over one theta interval; returns y(z2). */
+double orc_vertical_step(double y0, double z1, double z2, double T1, double T2, double F1, double F2,
+                         double scale, double Mstar, double R_av, int *nsteps);
+/* DiskStructure for every radius and species (DISK scale-height type):
+ * dens, dens0, gasdens [nr*nt] and w, w0 [nr*nt*ng] updated in place from
+ * the averaged temperatures Temp [nr*nt], then CheckCells (2659-2702).
+ * rho_out, when not NULL, receives the renormalised column densities
+ * rho[(ii*nr + i)*nt + j], ii = 0 gas, 1..ng species.  Returns the total
+ * number of Runge-Kutta steps taken. */
+long orc_disk_structure(const orc_struct *s, const double *Temp, double *dens, double *dens0, double *gasdens,
+                        double *w, double *w0, double *rho_out);
+/* compareCells (DiskStructure.f:553-584): convergence statistics of two
+ * states (arrays [nr*nt]). */
+void orc_compare_cells(int nr, int nt, const double *T1, const double *dT1, const double *dens1,
+                       const double *ddens1, const double *EJv1, const double *dEJv1, const double *T2,
+                       const double *dT2, const double *dens2, const double *ddens2, const double *EJv2,
+                       const double *dEJv2, double *errT, double *errRho, double *errE);
+
 /* Straight-ray geometry through the grid (trace2d with tau0 = inf):
  * records up to maxseg (cell, length) pairs; returns number of segments,
  * -1 on a geometry failure.  onEdge/edgeNr as the reference photon.       */

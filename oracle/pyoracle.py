@@ -49,10 +49,67 @@ class Ran1State(C.Structure):
     _fields_ = [("idum", C.c_int32), ("iy", C.c_int32), ("iv", C.c_int32 * 32)]
 
 
+class OrcStruct(C.Structure):
+    _fields_ = [("nr", C.c_int), ("nt", C.c_int), ("ng", C.c_int), ("R_av", _dp), ("theta_av", _dp), ("V", _dp),
+                ("Mstar", C.c_double), ("shscale", _dp), ("gshscale", _dp), ("settle", _ip), ("gascoupled", _ip),
+                ("mpstr", C.c_int), ("FradZ", _dp)]
+
+
 class OrcObs(C.Structure):
     _fields_ = [("angle", C.c_double), ("nphi", C.c_int), ("nj0", C.c_int), ("nphot", C.c_int),
                 ("nphotstar", C.c_int), ("nexits", C.c_int), ("usepol", C.c_int), ("tracestar", C.c_int),
                 ("traceemis", C.c_int), ("tracescat", C.c_int), ("pfstop", C.c_double)]
+
+
+def temp_average(T, Temp, fw, first=False):
+    """TempAverage (DiskStructure.f:25-79): returns the new averaged field."""
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    out = np.ascontiguousarray(Temp if Temp is not None else np.zeros_like(T), dtype=np.float64).copy()
+    load().orc_temp_average(T.size, p(T), p(out), float(fw), int(first))
+    return out
+
+
+def disk_structure(m, state, Temp, shscale=None, gshscale=None, settle=None, gascoupled=None, mpstr=False,
+                   FradZ=None):
+    """DiskStructure (DiskStructure.f:86-453) + CheckCells on a copy of
+    state for the grid of m (nr, nt, ng, R_av, theta_av, V, Mstar_g),
+    state = dict(dens, dens0, gasdens [nr,nt], w, w0 [nr,nt,ng]).
+    Returns (new state, rho [ng+1,nr,nt], Runge-Kutta steps)."""
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    out = {k: f(state[k]).copy() for k in ("dens", "dens0", "gasdens", "w", "w0")}
+    keep = dict(Rav=f(m.R_av), tav=f(m.theta_av), V=f(m.V),
+                sh=f(np.ones(m.nr) if shscale is None else shscale),
+                gsh=f(np.ones((m.ng, m.nr)) if gshscale is None else gshscale),
+                st=i32(np.zeros(m.ng) if settle is None else settle),
+                gc=i32(np.zeros(m.ng) if gascoupled is None else gascoupled),
+                T=f(Temp), rho=np.zeros((m.ng + 1, m.nr, m.nt)))
+    fz = None if FradZ is None else f(FradZ)
+    st = OrcStruct(nr=m.nr, nt=m.nt, ng=m.ng, R_av=p(keep["Rav"]), theta_av=p(keep["tav"]), V=p(keep["V"]),
+                   Mstar=m.Mstar_g, shscale=p(keep["sh"]), gshscale=p(keep["gsh"]), settle=p(keep["st"]),
+                   gascoupled=p(keep["gc"]), mpstr=int(mpstr), FradZ=p(fz) if fz is not None else None)
+    n = load().orc_disk_structure(C.byref(st), p(keep["T"]), p(out["dens"]), p(out["dens0"]),
+                                    p(out["gasdens"]), p(out["w"]), p(out["w0"]), p(keep["rho"]))
+    if n < 0:
+        raise RuntimeError("oracle disk structure failed")
+    return out, keep["rho"], n
+
+
+def vertical_step(y0, z1, z2, T1, T2, scale, Mstar, R_av, F1=0.0, F2=0.0):
+    """One odeint call of DiskStructure: returns (y(z2), Runge-Kutta steps)."""
+    n = C.c_int(0)
+    y = load().orc_vertical_step(y0, z1, z2, T1, T2, F1, F2, scale, Mstar, R_av, C.byref(n))
+    return y, n.value
+
+
+def compare_cells(nr, nt, a, b):
+    """compareCells (DiskStructure.f:553-584) of two states, each a dict of
+    T, dT, dens, ddens, EJv, dEJv [nr,nt]; returns (errT, errRho, errE)."""
+    keys = ("T", "dT", "dens", "ddens", "EJv", "dEJv")
+    arr = [np.ascontiguousarray(d[k], dtype=np.float64) for d in (a, b) for k in keys]
+    e = [C.c_double() for _ in range(3)]
+    load().orc_compare_cells(nr, nt, *[p(x) for x in arr], *[C.cast(C.byref(x), _dp) for x in e])
+    return tuple(x.value for x in e)
 
 
 def make_obs(angle_deg=45.0, nphi=45, nj0=2, nphot=1000, nphotstar=1000, nexits=5, usepol=True,
@@ -131,6 +188,12 @@ def load():
                                 C.POINTER(C.c_int)]
     lib.orc_min_dist.argtypes = [C.POINTER(OrcModel), C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
     lib.orc_min_dist.restype = C.c_double
+    lib.orc_temp_average.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int]
+    lib.orc_vertical_step.argtypes = [C.c_double] * 10 + [C.POINTER(C.c_int)]
+    lib.orc_vertical_step.restype = C.c_double
+    lib.orc_disk_structure.argtypes = [C.POINTER(OrcStruct), _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.orc_disk_structure.restype = C.c_long
+    lib.orc_compare_cells.argtypes = [C.c_int, C.c_int] + [_dp] * 12 + [_dp] * 3
     _lib = lib
     return lib
 
@@ -194,6 +257,9 @@ class Oracle:
             raise RuntimeError(f"oracle diffuse failed: {rc}")
         sh = (m.nr, m.nt)
         return T.reshape(sh), th.reshape(sh).astype(bool), EJv.reshape(sh), nc.value
+
+    def disk_structure(self, state, Temp, **kw):
+        return disk_structure(self.m, state, Temp, **kw)
 
     def min_dist(self, x, y, z, i, jj):
         return self.lib.orc_min_dist(C.byref(self.om), x, y, z, int(i), int(jj))
