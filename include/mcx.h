@@ -3,7 +3,10 @@
  * (libmcx.so).  It replaces one MCMax stage: the Monte Carlo pass
  *     subroutine MCRadiation(NphotTot,niter,dofastvisc)   MCRadiation.f:6
  * together with the Lucy temperature inversion that follows it in the
- * driver (Main.f:498-558).  The reference passes everything through module
+ * driver (Main.f:498-558), and the steps either side of it: the diffusion
+ * approximation (Diffuse), the vertical-structure iteration (TempAverage,
+ * DiskStructure, compareCells) and the ray-traced SED (Observe SPECTRUM,
+ * TraceMono).  The reference passes everything through module
  * globals (Modules.f:141-150); here the caller hands the same quantities over
  * once as plain host arrays (mcx_set_tables, mcx_set_model) and the library
  * keeps them resident in HBM.
@@ -122,6 +125,12 @@ typedef struct {
                                         in every valid pass)                */
     int32_t *thick;                  /* [nr*nt] C%thick of the last pass
                                         (InitRandomWalk, RandomWalk.f:152)  */
+    int64_t maxsteps;                /* cell steps of the longest packet
+                                        history                             */
+    double t_done[5];                /* ms after the packet kernel started
+                                        when 50, 90, 99, 99.9 and 100% of
+                                        the packets had finished (last
+                                        launch of the pass; -1: unknown)    */
 } mcx_tallies;
 
 /* Temperature products of Main.f:498-549 and the void cell (Main.f:546). */

@@ -55,6 +55,7 @@ class Tallies(C.Structure):
         ("nsteps", C.c_int64), ("nraar", C.c_int64), ("nmaxinteract", C.c_int64),
         ("ngeomfail", C.c_int64), ("tautot", C.c_double), ("Eescape", C.c_double),
         ("nrw", C.c_int64), ("noverflow", C.c_int64), ("thick", _ip),
+        ("maxsteps", C.c_int64), ("t_done", C.c_double * 5),
     ]
 
 

@@ -172,7 +172,7 @@ __device__ __forceinline__ void pick_lambda(int nbins, double Lr, F cum, int &l1
 struct Packet {
     double x, y, z, vx, vy, vz, E, wl1;
     double tau0, tau, nEJv;
-    int i, jj, l1, edgeNr, ntrace, lastc;
+    int i, jj, l1, edgeNr, ntrace, lastc, nstep;
     bool onEdge, scatt, pol;
 };
 
@@ -425,6 +425,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     __shared__ double stau;
     for (int k = threadIdx.x; k < C_NCOUNT; k += blockDim.x) scnt[k] = 0ull;
     if (threadIdx.x == 0) stau = 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ty.counters[G_TLINE] = wall_clock64();
     __syncthreads();
 #define CNT(k) atomicAdd(&scnt[k], 1ull)
     double tautot = 0.0;
@@ -458,6 +459,22 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
         // ---- wave-aggregated packet acquisition -------------------------
         {
             bool need = (st == S_NEED);
+            // packets that ended since the last acquisition: the pass's
+            // timeline (when 50/90/99/99.9/100% had finished) and the
+            // longest history
+            const bool fin = need && pid >= 0;
+            unsigned long long fmask = __ballot(fin);
+            if (fin) atomicMax(&ty.counters[G_MAXSTEPS], (unsigned long long)p.nstep);
+            if (fmask && lane == __ffsll((long long)fmask) - 1) {
+                const unsigned long long nf = (unsigned long long)__popcll(fmask);
+                const unsigned long long d0 = atomicAdd(&ty.counters[G_DONE], nf);
+                const unsigned long long n = (unsigned long long)(A.pid_end - A.pid_base);
+                const unsigned long long num[5] = {1, 9, 99, 999, 1}, den[5] = {2, 10, 100, 1000, 1};
+                for (int k = 0; k < 5; k++) {
+                    const unsigned long long mk = (n * num[k] + den[k] - 1) / den[k];
+                    if (d0 < mk && d0 + nf >= mk) ty.counters[G_TLINE + 1 + k] = wall_clock64();
+                }
+            }
             unsigned long long mask = __ballot(need);
             if (mask) {
                 int leader = __ffsll((long long)mask) - 1;
@@ -475,6 +492,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                         pid = mine;
                         g.init(A.seed, (uint64_t)pid);
                         p.lastc = -1;
+                        p.nstep = 0;
                         st = S_EMIT;
                     }
                 }
@@ -586,6 +604,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 st = S_NEED;
             } else {
                 CNT(C_NSTEPS);
+                p.nstep++;
                 double tau1 = ((Kext * v) * c_dens) * AU;
                 if ((p.tau + tau1) > p.tau0) {
                     v = (v * (p.tau0 - p.tau)) / tau1;
@@ -1247,6 +1266,8 @@ extern "C" int mcx_create(mcx_ctx **out, int device) {
     hipDeviceProp_t prop;
     (void)hipGetDeviceProperties(&prop, device);
     ctx->sm_count = prop.multiProcessorCount;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess) ctx->wclk_khz = khz;
     *out = ctx;
     return MCX_OK;
 }
@@ -1518,7 +1539,7 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
     if ((r = dalloc(ctx, ctx->mod_alloc, &ty.Tcell, ncell))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ty.specE, (size_t)2 * (NANGD + 1) * nlam))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ty.ispec, (size_t)2 * (NANGD + 1) * nlam))) return r;
-    if ((r = dalloc(ctx, ctx->mod_alloc, &ty.counters, 16))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ty.counters, G_NSLOTS))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ty.tautot, 1))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Ni, ncell))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_EJv, ncell))) return r;
@@ -1587,7 +1608,7 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     HIPCHK(ctx, hipMemsetAsync(ty.Tcell, 0, sizeof(unsigned long long) * ncell, s));
     HIPCHK(ctx, hipMemsetAsync(ty.specE, 0, sizeof(unsigned long long) * 2 * (NANGD + 1) * nlam, s));
     HIPCHK(ctx, hipMemsetAsync(ty.ispec, 0, sizeof(unsigned) * 2 * (NANGD + 1) * nlam, s));
-    HIPCHK(ctx, hipMemsetAsync(ty.counters, 0, sizeof(unsigned long long) * 16, s));
+    HIPCHK(ctx, hipMemsetAsync(ty.counters, 0, sizeof(unsigned long long) * G_NSLOTS, s));
     HIPCHK(ctx, hipMemsetAsync(ty.tautot, 0, sizeof(double), s));
     HIPCHK(ctx, hipMemsetAsync(ctx->d_Ni, 0, sizeof(int) * ncell, s));
     hipLaunchKernelGGL(k_cell_units, dim3((ncell + 255) / 256), dim3(256), 0, s, md.nr, md.nt, nlam, md.R2, md.KT, E0,
@@ -1638,6 +1659,8 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
         unsigned long long b0 = (unsigned long long)base;
         HIPCHK(ctx, hipMemcpyAsync(ctx->d_next, &b0, sizeof(b0), hipMemcpyHostToDevice, s));
         HIPCHK(ctx, hipMemsetAsync(ty.bitmap, 0, (size_t)ncell * nwords * sizeof(unsigned), s));
+        HIPCHK(ctx, hipMemsetAsync(ty.counters + G_DONE, 0, sizeof(unsigned long long), s));
+        HIPCHK(ctx, hipMemsetAsync(ty.counters + G_TLINE, 0, sizeof(unsigned long long) * 6, s));
         HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
         hipLaunchKernelGGL(k_transport, dim3(blocks), dim3(threads), lds, s, A);
         HIPCHK(ctx, hipGetLastError());
@@ -1659,7 +1682,7 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     ctx->kernel_ms = kms_total;
     ctx->pass_ms = pms;
     ctx->have_pass = true;
-    unsigned long long cnt[16];
+    unsigned long long cnt[G_NSLOTS];
     HIPCHK(ctx, hipMemcpy(cnt, ty.counters, sizeof(cnt), hipMemcpyDeviceToHost));
     if (cnt[C_NGEOM] > 0) return fail(ctx, MCX_EGEOM, std::to_string(cnt[C_NGEOM]) + " packets found nothing to hit");
     return MCX_OK;
@@ -1706,7 +1729,7 @@ extern "C" int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *o) {
             if (o->ispec) o->ispec[k] = (int32_t)sc[k];
             if (o->iscatspec) o->iscatspec[k] = (int32_t)sc[nb + k];
         }
-    unsigned long long cnt[16];
+    unsigned long long cnt[G_NSLOTS];
     double tt = 0.0;
     HIPCHK(ctx, hipMemcpy(cnt, ctx->ty.counters, sizeof(cnt), hipMemcpyDeviceToHost));
     HIPCHK(ctx, hipMemcpy(&tt, ctx->ty.tautot, sizeof(double), hipMemcpyDeviceToHost));
@@ -1724,6 +1747,11 @@ extern "C" int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *o) {
     o->noverflow = (int64_t)cnt[C_NOVERFLOW];
     o->tautot = tt;
     o->Eescape = Eesc;
+    o->maxsteps = (int64_t)cnt[G_MAXSTEPS];
+    for (int k = 0; k < 5; k++) {
+        const long long t0 = (long long)cnt[G_TLINE], t1 = (long long)cnt[G_TLINE + 1 + k];
+        o->t_done[k] = (t1 != 0 && ctx->wclk_khz > 0) ? (double)(t1 - t0) / (double)ctx->wclk_khz : -1.0;
+    }
     return MCX_OK;
 }
 

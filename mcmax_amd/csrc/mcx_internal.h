@@ -28,6 +28,14 @@
 enum { C_NEMIT = 0, C_NINTERACT, C_NABS, C_NSCAT, C_NESCAPE, C_NSTARHIT, C_NSTEPS, C_NRAAR, C_NMAXINT,
        C_NGEOM, C_NOVERFLOW, C_NRW, C_NCOUNT };
 
+// slots of DevTally::counters beyond the event counters: the longest packet
+// history (cell steps), packets finished in the current launch, and wall-clock
+// stamps of the launch's start and of 50/90/99/99.9/100% finished
+#define G_MAXSTEPS 12
+#define G_DONE 13
+#define G_TLINE 16
+#define G_NSLOTS 24
+
 #define NY_RW 1000 // survival-table length (RandomWalk.f:6-7)
 
 struct DevTables {
@@ -120,6 +128,7 @@ struct mcx_ctx {
     double unitE = 0.0, E0 = 0.0;
     double kernel_ms = 0.0, pass_ms = 0.0;
     int sm_count = 0;
+    int wclk_khz = 0; // rate of the device's constant wall clock (wall_clock64)
 };
 
 static inline int fail(mcx_ctx *ctx, int code, const std::string &msg) {

@@ -106,8 +106,9 @@ class Transport:
         for k in ("spec", "scatspec", "ispec", "iscatspec"):
             out[k] = out[k].reshape(NANGLE + 1, m.nlam)
         for k in ("nemit", "ninteract", "nabs", "nscat", "nescape", "nstarhit", "nsteps", "nraar",
-                  "nmaxinteract", "ngeomfail", "tautot", "Eescape", "nrw", "noverflow"):
+                  "nmaxinteract", "ngeomfail", "tautot", "Eescape", "nrw", "noverflow", "maxsteps"):
             out[k] = getattr(t, k)
+        out["t_done_ms"] = list(t.t_done)
         return out
 
     def temperature(self):
