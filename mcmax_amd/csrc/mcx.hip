@@ -42,6 +42,7 @@ struct TransportArgs {
     int nwords;
     int serial;    // only lane 0 of the grid takes packets (in packet order)
     int coop;      // wave-cooperative lookups when at most this many lanes need one
+    int rb_mask;   // draws per lane ring minus one (a power of two minus one)
     double invE;   // 2^EABS_BITS / E0
     double unitE;  // E0 / 2^EABS_BITS
 };
@@ -190,7 +191,7 @@ __device__ __forceinline__ unsigned long long add_eabs(const DevTally &ty, int c
 
 // hunt (Subroutines.f:2086-2130) in the decreasing survival table, from jlo = 1;
 // returns the 1-based jlo.
-__device__ int hunt_desc(const double *__restrict__ xx, int n, double x) {
+__device__ __forceinline__ int hunt_desc(const double *__restrict__ xx, int n, double x) {
     int jlo = 1, jhi, inc = 1;
     const bool ascnd = (xx[n - 1] >= xx[0]);
     if ((x >= xx[jlo - 1]) == ascnd) {
@@ -382,7 +383,38 @@ __device__ void lk_wave(const Lookup &mine, int src, int lane, int &b, double &f
     }
 }
 
+// A packet's Philox stream read through a ring of draws in LDS.  Draw d of
+// packet p is word d&1 of block d>>1 of the stream (p's counter), so the
+// draws are those of PhiloxStream; the blocks are computed ahead by the
+// whole wavefront at one point of the loop (rng_service), and here only
+// when the ring runs dry.
+__device__ __noinline__ double2 philox_pair(uint32_t k0, uint32_t k1, unsigned long long blk, unsigned long long pid) {
+    double2 r;
+    philox_block(k0, k1, blk, pid, r.x, r.y);
+    return r;
+}
+
+struct RingRng {
+    double *ring;
+    unsigned mask;
+    uint32_t k0, k1;
+    unsigned long long pid, head, fill; // next draw; draws in the ring up to fill
+    __device__ __forceinline__ double next() {
+        if (head >= fill) {
+            const unsigned long long b = head >> 1;
+            const double2 r = philox_pair(k0, k1, b, pid);
+            ring[(unsigned)(2 * b) & mask] = r.x;
+            ring[(unsigned)(2 * b + 1) & mask] = r.y;
+            fill = 2 * b + 2;
+        }
+        const double v = ring[(unsigned)head & mask];
+        head++;
+        return v;
+    }
+};
+
 #define INFLIGHT_DIV 64 // default cap on packets in flight: nphot / 64
+#define NPL 13          // per-thread LDS doubles (odd stride spreads the banks)
 #define COOP_LANES 6 // default: a wave serves lookups together when at most this many lanes need one
 
 __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
@@ -395,28 +427,49 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     double *sR2 = lds;
     double *sT2 = sR2 + (nr + 1);
     double *sTh = sT2 + (nt + 1);
-    double *sC2 = sTh + (nt + 1);
-    double *sS2 = sC2 + 361;
-    double *sCS = sS2 + 361; // stellar emission cumulative, nbins entries
-    // per-thread Stokes vector and S direction, field-major
-    double *pl = sCS + nbins + threadIdx.x;
-    const int PB = blockDim.x;
-#define P_Q pl[0 * PB]
-#define P_U pl[1 * PB]
-#define P_V pl[2 * PB]
-#define P_SX pl[3 * PB]
-#define P_SY pl[4 * PB]
-#define P_SZ pl[5 * PB]
+    // prefix sums over phi bins 1..k of cos(2 phi), sin(2 phi), k = 0..180
+    double *sPC2 = sTh + (nt + 1);
+    double *sPS2 = sPC2 + 181;
+    double *sCS = sPS2 + 181; // stellar emission cumulative, nbins entries
+    // per-thread state kept out of registers (Stokes vector, S direction,
+    // interaction cache): NPL doubles per thread, thread-major, so that every
+    // field is a constant offset from the thread's base
+    double *pl = sCS + nbins + (size_t)threadIdx.x * NPL;
+    // rings of Philox draws, RB per thread
+    const unsigned RB = (unsigned)A.rb_mask + 1u;
+    double *sRing = sCS + nbins + (size_t)blockDim.x * NPL;
+    // random walk: radial edges (AU), sin of the cone edges, survival table
+    double *sRau = sRing + (size_t)blockDim.x * RB;
+    double *sSinT = sRau + (nr + 1);
+    double *sPhi = sSinT + (nt + 1);
+#define P_Q pl[0]
+#define P_U pl[1]
+#define P_V pl[2]
+#define P_SX pl[3]
+#define P_SY pl[4]
+#define P_SZ pl[5]
+#define P_TRW pl[6]  // cell T the next random-walk check reads
+#define P_KDX pl[7]  // {KDext, KDabs} of the cell at kd_iT
+#define P_KDY pl[8]
+#define P_MASS pl[9] // mass of cell ic
+#define P_IC pl[10]  // ic: the cell these entries belong to (-1: none)
+#define P_TC pl[11]  // 2*class + C%thick of cell ic
+#define P_KDIT pl[12] // kd_iT
     for (int k = threadIdx.x; k <= nr; k += blockDim.x) sR2[k] = md.R2[k];
     for (int k = threadIdx.x; k <= nt; k += blockDim.x) {
         sT2[k] = md.T2[k];
         sTh[k] = md.Theta[k];
     }
-    for (int k = threadIdx.x; k < 361; k += blockDim.x) {
-        sC2[k] = tb.cos2phi[k];
-        sS2[k] = tb.sin2phi[k];
+    for (int k = threadIdx.x; k <= 180; k += blockDim.x) {
+        sPC2[k] = tb.pc2[k];
+        sPS2[k] = tb.ps2[k];
     }
     for (int k = threadIdx.x; k < nbins; k += blockDim.x) sCS[k] = tb.cumStar[k];
+    if (A.rw.on) {
+        for (int k = threadIdx.x; k <= nr; k += blockDim.x) sRau[k] = A.rw.R[k];
+        for (int k = threadIdx.x; k <= nt; k += blockDim.x) sSinT[k] = A.rw.sinT[k];
+        for (int k = threadIdx.x; k < NY_RW; k += blockDim.x) sPhi[k] = A.rw.phi[k];
+    }
     __syncthreads();
 
     const int lane = __lane_id();
@@ -432,8 +485,16 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
 
     Packet p;
     memset(&p, 0, sizeof(p));
-    PhiloxStream g;
-    g.init(A.seed, 0);
+    RingRng g;
+    g.ring = sRing + (size_t)threadIdx.x * RB;
+    g.mask = RB - 1u;
+    g.k0 = (uint32_t)A.seed;
+    g.k1 = (uint32_t)(A.seed >> 32);
+    g.pid = 0;
+    g.head = g.fill = 0;
+    // requests of the wave's lanes whose ring runs low (rng service)
+    __shared__ unsigned long long rq_pid[256], rq_blk[256];
+    __shared__ int rq_own[256];
     long long pid = -1;
     int st = S_NEED;
     if (A.serial && (blockIdx.x * blockDim.x + threadIdx.x) != 0) st = S_DONE;
@@ -445,15 +506,20 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     int cc = -1, kc = -1, kl = -1;
     double c_dens = 0.0, c_u1 = 0.0, c_u2 = 0.0;
     double c_Tl = 0.0; // BW T after this lane's last absorption in cell cc (0: none yet)
+    // interaction state of cell ic (loaded at the first interaction in a
+    // cell, alongside that interaction's own round trips): 2*class + C%thick,
+    // the cell's mass, the cell T the next random-walk check reads (known to
+    // the lane: its own update, or a load issued at the interaction) and
+    // {KDext, KDabs} at kd_iT
+    P_IC = -1.0;
+    P_TC = 0.0;
+    P_KDIT = -1.0;
+    P_TRW = 0.0;
+    P_MASS = 0.0;
     double2 k1 = make_double2(0.0, 0.0), k2 = make_double2(0.0, 0.0);
     TallyAcc acc;
     acc.c = -1;
     acc.e1 = acc.l0 = acc.l1 = acc.l2 = 0ull;
-    // a pending inverse-CDF lookup and its result
-    Lookup lk;
-    memset(&lk, 0, sizeof(lk));
-    int lk_b = 0;
-    double lk_fb = 0.0, lk_fbm1 = 0.0;
 
     for (;;) {
         // ---- wave-aggregated packet acquisition -------------------------
@@ -490,7 +556,8 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                         st = S_DONE;
                     } else {
                         pid = mine;
-                        g.init(A.seed, (uint64_t)pid);
+                        g.pid = (unsigned long long)pid;
+                        g.head = g.fill = 0;
                         p.lastc = -1;
                         p.nstep = 0;
                         st = S_EMIT;
@@ -499,6 +566,37 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
             }
         }
         if (__all(st == S_DONE)) break;
+
+        // ---- rng service: the wavefront computes, 64 blocks per pass, the
+        //      next RB/4 Philox blocks of every lane whose ring is at most
+        //      half full
+        {
+            const bool low = (st != S_DONE) && (g.fill - g.head <= (unsigned long long)(RB >> 1));
+            const unsigned long long m = __ballot(low);
+            if (m) {
+                const int wbase = threadIdx.x & ~63;
+                if (low) {
+                    const int rank = __popcll(m & ((1ull << lane) - 1ull));
+                    rq_pid[wbase + rank] = g.pid;
+                    rq_blk[wbase + rank] = g.fill >> 1;
+                    rq_own[wbase + rank] = threadIdx.x;
+                }
+                __builtin_amdgcn_wave_barrier();
+                const int per = (int)(RB >> 2); // blocks per request
+                const int items = __popcll(m) * per;
+                for (int w = lane; w < items; w += 64) {
+                    const int r = w / per, k = w - r * per;
+                    const unsigned long long b = rq_blk[wbase + r] + (unsigned long long)k;
+                    double d0, d1;
+                    philox_block(g.k0, g.k1, b, rq_pid[wbase + r], d0, d1);
+                    double *ring = sRing + (size_t)rq_own[wbase + r] * RB;
+                    ring[(unsigned)(2 * b) & (RB - 1u)] = d0;
+                    ring[(unsigned)(2 * b + 1) & (RB - 1u)] = d1;
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (low) g.fill += (unsigned long long)(RB >> 1);
+            }
+        }
 
         // ---- stellar emission (MCRadiation.f:347-383, 500-513, 554) ------
         if (st == S_EMIT) {
@@ -647,10 +745,26 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
             }
         }
 
+        // a pending inverse-CDF lookup and its result: set and consumed within
+        // one trip of the loop, so they are not live across the cell step
+        Lookup lk;
+        lk.R0 = lk.R1 = nullptr;
+        lk.e0 = lk.e1 = lk.ws = lk.t = 0.0;
+        lk.n = 0;
+        lk.kind = L_SAME;
+        int lk_b = 0;
+        double lk_fb = 0.0, lk_fbm1 = 0.0;
+
         // ---- interaction up to its CDF lookup (Radiation.f:41-355) ------
         if (st == S_INTERACT) {
             CNT(C_NINTERACT);
             const int c = p.i * nt + p.jj;
+            if ((double)c != P_IC) {
+                P_IC = (double)c;
+                P_MASS = md.mass[c];
+                P_TC = A.rw.on ? (double)(A.rw.thick[c] | (A.rw.cls[c] << 1)) : 0.0;
+                P_KDIT = -1.0;
+            }
             const int l1 = p.l1, l2 = p.l1 + 1;
             const double wl2 = 1.0 - p.wl1;
             double Ksca = k1.y * p.wl1 + k2.y * wl2;
@@ -686,7 +800,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                         q2 = Kp0[jw + 2];
                     }
                 }
-                const double mass = md.mass[c];
+                const double mass = P_MASS;
                 const double Thint = __longlong_as_double((long long)__hip_atomic_load(
                     &ty.Tcell[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
                 unsigned long long q;
@@ -709,6 +823,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 double T1 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)(old + q) * A.unitE) / mass, T0);
                 c_Tl = T1;
                 atomicMax(&ty.Tcell[c], (unsigned long long)__double_as_longlong(T1));
+                P_TRW = (T1 > Thint) ? T1 : Thint; // the cell's T after this update
                 int iT0 = (int)(T0 / 1.0), iT1 = (int)(T1 / 1.0);
                 double e0 = T0 - (double)(float)iT0;
                 double e1 = T1 - (double)(float)iT1;
@@ -786,6 +901,9 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
             } else {
                 CNT(C_NSCAT);
                 p.scatt = true;
+                if ((int)P_TC & 1)
+                    P_TRW = __longlong_as_double((long long)__hip_atomic_load(&ty.Tcell[c], __ATOMIC_RELAXED,
+                                                                               __HIP_MEMORY_SCOPE_AGENT));
                 if (tb.scat_how == 1) {
                     random_direction(g, p.vx, p.vy, p.vz);
                     st = S_AFTER;
@@ -808,22 +926,24 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     double IF11 = MIF11 * 180.0, IF12 = MIF12 * 180.0;
                     double phi;
                     if (p.pol) {
-                        double Itot = 0.0;
+                        // the phi bins' cumulative intensity, bins 1..k:
+                        // I(k) = k IF11 E + IF12 (Q sum cos 2phi + U sum sin 2phi);
+                        // the first bin whose I exceeds the draw (I is
+                        // non-decreasing: |IF12 (Q cos + U sin)| <= IF11 E)
                         const double aE = IF11 * p.E, Qp = P_Q, Up = P_U;
-#pragma unroll 6
-                        for (int k = 1; k <= 180; k++) Itot = Itot + aE + IF12 * (Qp * sC2[k] + Up * sS2[k]);
-                        Itot = g.next() * Itot;
-                        double I1 = 0.0, I0 = 0.0;
+                        auto Ik = [&](int k) { return (double)k * aE + IF12 * (Qp * sPC2[k] + Up * sPS2[k]); };
+                        const double Itot = g.next() * Ik(180);
+                        int lo = 1, hi = 181;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (Ik(mid) > Itot) hi = mid;
+                            else lo = mid + 1;
+                        }
                         phi = MCX_PI;
-#pragma unroll 4
-                        for (int k = 1; k <= 180; k++) {
-                            I0 = I0 + aE + IF12 * (Qp * sC2[k] + Up * sS2[k]);
-                            if (I0 > Itot) {
-                                double wp1 = (I0 - Itot) / (I0 - I1);
-                                phi = (MCX_PI * ((double)k - wp1)) / 180.0;
-                                break;
-                            }
-                            I1 = I0;
+                        if (lo <= 180) {
+                            const double I0 = Ik(lo), I1 = Ik(lo - 1);
+                            const double wp1 = (I0 - Itot) / (I0 - I1);
+                            phi = (MCX_PI * ((double)lo - wp1)) / 180.0;
                         }
                     } else {
                         phi = g.next() * MCX_PI;
@@ -967,7 +1087,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 p.tau = 0.0;
                 p.ntrace = 0;
                 // trace2d starts with the random walk in thick cells (MCRadiation.f:916)
-                if (A.rw.on && A.rw.thick[p.i * nt + p.jj]) st = S_RW;
+                if ((int)P_TC & 1) st = S_RW;
             }
         }
 
@@ -980,26 +1100,41 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
             const double r = sqrt(rho + p.z * p.z);
             rho = sqrt(rho);
             const double az = fabs(p.z);
-            const double vR1 = fabs(r - rw.R[p.i]), vR2 = fabs(rw.R[p.i + 1] - r);
-            const double vT1 = fabs(sTh[p.jj] * rho - rw.sinT[p.jj] * az);
-            const double vT2 = fabs(sTh[p.jj + 1] * rho - rw.sinT[p.jj + 1] * az);
+            const double vR1 = fabs(r - sRau[p.i]), vR2 = fabs(sRau[p.i + 1] - r);
+            const double vT1 = fabs(sTh[p.jj] * rho - sSinT[p.jj] * az);
+            const double vT2 = fabs(sTh[p.jj + 1] * rho - sSinT[p.jj + 1] * az);
             double dmin = vR1;
             if (vR2 < dmin) dmin = vR2;
             if (vT1 < dmin) dmin = vT1;
             if (vT2 < dmin) dmin = vT2;
-            const double Thint = __longlong_as_double(
-                (long long)__hip_atomic_load(&ty.Tcell[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            const double Thint = P_TRW;
             int iT = (int)(Thint / 1.0);
             if (iT == 0) iT = 1;
             if (iT > TMAXD - 1) iT = TMAXD - 1;
-            const double2 kd = rw.KD[(size_t)rw.cls[c] * (TMAXD + 1) + iT];
+            if ((double)iT != P_KDIT) {
+                const double2 kq = rw.KD[(size_t)((int)P_TC >> 1) * (TMAXD + 1) + iT];
+                P_KDX = kq.x;
+                P_KDY = kq.y;
+                P_KDIT = (double)iT;
+            }
+            const double2 kd = make_double2(P_KDX, P_KDY);
             const double lr = 1.0 / ((AU * kd.x) * c_dens);
             if (dmin <= rw.factRW * lr) {
                 st = S_STEP;
             } else {
                 CNT(C_NRW);
-                const int iy = hunt_desc(rw.phi, NY_RW, g.next());
-                const double v = -((3.0 * log(rw.y[iy - 1])) * (dmin * dmin)) / (lr * (MCX_PI * MCX_PI));
+                // the window of the cell's table at the walk's start T, fetched
+                // together with the Eabs update
+                const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
+                int jw = (int)(Thint / 1.0);
+                if (jw < 1) jw = 1;
+                if (jw > TMAXD - 2) jw = TMAXD - 2;
+                const double w0 = kpc[jw], w1 = kpc[jw + 1], w2 = kpc[jw + 2];
+                const int iy = hunt_desc(sPhi, NY_RW, g.next());
+                // y(iy) of InitRandomWalk (RandomWalk.f:160-164): 1, then a
+                // single-precision ratio times 0.75
+                const double yv = (iy == 1) ? 1.0 : (double)((float)(NY_RW - iy + 1) / (float)(NY_RW - 1)) * 0.75;
+                const double v = -((3.0 * log(yv)) * (dmin * dmin)) / (lr * (MCX_PI * MCX_PI));
                 random_direction(g, p.vx, p.vy, p.vz);
                 p.x = p.x + dmin * p.vx;
                 p.y = p.y + dmin * p.vy;
@@ -1009,9 +1144,9 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 const unsigned long long old = add_eabs(ty, c, ejv * c_dens, A.invE, q, scnt);
                 acc_ejv(acc, ejv, c_u1, scnt);
                 p.nEJv = p.nEJv + ejv; // the module-level nEJv of RandomWalk.f:115
-                const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
-                const double T1 = dev_increaseT(kpc, ((double)(old + q) * A.unitE) / md.mass[c], Thint);
+                const double T1 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)(old + q) * A.unitE) / P_MASS, Thint);
                 atomicMax(&ty.Tcell[c], (unsigned long long)__double_as_longlong(T1));
+                P_TRW = (T1 > Thint) ? T1 : Thint;
                 tautot += v / dmin;
                 // leave in a direction of the forward hemisphere (MCRadiation.f:919-925)
                 double vx, vy, vz;
@@ -1059,6 +1194,13 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
 #undef P_SX
 #undef P_SY
 #undef P_SZ
+#undef P_TRW
+#undef P_KDX
+#undef P_KDY
+#undef P_MASS
+#undef P_IC
+#undef P_TC
+#undef P_KDIT
 }
 
 // ---------------------------------------------------------------------------
@@ -1266,6 +1408,7 @@ extern "C" int mcx_create(mcx_ctx **out, int device) {
     hipDeviceProp_t prop;
     (void)hipGetDeviceProperties(&prop, device);
     ctx->sm_count = prop.multiProcessorCount;
+    ctx->lds_max = prop.sharedMemPerBlock;
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess) ctx->wclk_khz = khz;
     *out = ctx;
@@ -1378,6 +1521,16 @@ extern "C" int mcx_set_tables(mcx_ctx *ctx, const mcx_tables *t) {
     UP(tb.IF12, t->IF12, (size_t)ng * nlam);
     UP(tb.cos2phi, t->cos2phi, 361);
     UP(tb.sin2phi, t->sin2phi, 361);
+    {
+        // prefix sums of cos 2phi(i), sin 2phi(i) over the scattering bins i = 1..180
+        std::vector<double> pc(181, 0.0), ps(181, 0.0);
+        for (int i = 1; i <= 180; i++) {
+            pc[i] = pc[i - 1] + t->cos2phi[i];
+            ps[i] = ps[i - 1] + t->sin2phi[i];
+        }
+        UP(tb.pc2, pc.data(), 181);
+        UP(tb.ps2, ps.data(), 181);
+    }
     if (full) {
         UP(tb.F, t->F, (size_t)ng * nlam * 6 * 180);
         UP(tb.C11, C11.data(), C11.size());
@@ -1617,8 +1770,15 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     // launch shape: a full device of resident wavefronts, or one lane
     int threads = 256, blocks = 1;
     if (run->nlanes == 1) threads = 64; // one wavefront: lane 0 traces, the others serve its lookups
-    size_t lds = sizeof(double) *
-                 ((size_t)(md.nr + 1) + 2 * (size_t)(md.nt + 1) + 2 * 361 + (size_t)(nlam - 1) + 6 * (size_t)threads);
+    size_t lds = sizeof(double) * ((size_t)(md.nr + 1) + 2 * (size_t)(md.nt + 1) + 2 * 181 + (size_t)(nlam - 1) +
+                                   NPL * (size_t)threads);
+    if (rw.on) lds += sizeof(double) * ((size_t)(md.nr + 1) + (size_t)(md.nt + 1) + NY_RW);
+    // rings of Philox draws: 16 per thread when they fit beside the rest
+    int rb = 16;
+    while (rb > 4 && lds + sizeof(double) * (size_t)rb * threads > ctx->lds_max) rb >>= 1;
+    lds += sizeof(double) * (size_t)rb * threads;
+    if (lds > ctx->lds_max)
+        return fail(ctx, MCX_EINVAL, "grid too large for the kernel's LDS tables (" + std::to_string(lds) + " bytes)");
     if (run->nlanes == 1) {
         blocks = 1;
     } else {
@@ -1650,6 +1810,7 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     A.nwords = nwords;
     A.serial = (run->nlanes == 1);
     A.coop = (run->coop_lanes < 0) ? COOP_LANES : run->coop_lanes;
+    A.rb_mask = rb - 1;
     float kms_total = 0.f;
     for (long long base = 0; base < run->nphot; base += batch) {
         long long end = base + batch;

@@ -15,6 +15,30 @@
 // Philox4x32-10 (Salmon et al., SC'11).  Stream of packet p under key `seed`:
 // counter = (block, p); each block yields two doubles, taken in order
 // (low 64 bits first).  Each double keeps the top 53 bits, centred: (0, 1).
+__device__ __forceinline__ void philox_block(uint32_t k0, uint32_t k1, uint64_t blk, uint64_t pid, double &d0,
+                                             double &d1) {
+    uint32_t c0 = (uint32_t)blk, c1 = (uint32_t)(blk >> 32);
+    uint32_t c2 = (uint32_t)pid, c3 = (uint32_t)(pid >> 32);
+    uint32_t a = k0, b = k1;
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        uint32_t n0 = hi1 ^ c1 ^ a;
+        uint32_t n2 = hi0 ^ c3 ^ b;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        a += 0x9E3779B9u;
+        b += 0xBB67AE85u;
+    }
+    uint64_t v0 = ((uint64_t)c1 << 32) | c0;
+    uint64_t v1 = ((uint64_t)c3 << 32) | c2;
+    d0 = ((double)(v0 >> 11) + 0.5) * 0x1p-53;
+    d1 = ((double)(v1 >> 11) + 0.5) * 0x1p-53;
+}
+
 struct PhiloxStream {
     uint32_t k0, k1;
     uint64_t pid;
@@ -35,33 +59,17 @@ struct PhiloxStream {
             have = false;
             return cache;
         }
-        uint32_t c0 = (uint32_t)blk, c1 = (uint32_t)(blk >> 32);
-        uint32_t c2 = (uint32_t)pid, c3 = (uint32_t)(pid >> 32);
-        uint32_t a = k0, b = k1;
-#pragma unroll
-        for (int r = 0; r < 10; r++) {
-            uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-            uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
-            uint32_t n0 = hi1 ^ c1 ^ a;
-            uint32_t n2 = hi0 ^ c3 ^ b;
-            c0 = n0;
-            c1 = lo1;
-            c2 = n2;
-            c3 = lo0;
-            a += 0x9E3779B9u;
-            b += 0xBB67AE85u;
-        }
+        double d0;
+        philox_block(k0, k1, blk, pid, d0, cache);
         blk++;
-        uint64_t v0 = ((uint64_t)c1 << 32) | c0;
-        uint64_t v1 = ((uint64_t)c3 << 32) | c2;
-        cache = ((double)(v1 >> 11) + 0.5) * 0x1p-53;
         have = true;
-        return ((double)(v0 >> 11) + 0.5) * 0x1p-53;
+        return d0;
     }
 };
 
 // Radiation.f:1007-1024
-__device__ __forceinline__ void random_direction(PhiloxStream &g, double &x, double &y, double &z) {
+template <typename Gen>
+__device__ __forceinline__ void random_direction(Gen &g, double &x, double &y, double &z) {
     double r;
     for (;;) {
         x = 2.0 * g.next() - 1.0;

@@ -50,6 +50,7 @@ struct DevTables {
     const double *C11, *C12; // [ng][nlam][180]
     const double *IF11, *IF12;
     const double *cos2phi, *sin2phi; // [361]
+    const double *pc2, *ps2;         // [181] prefix sums of cos 2phi, sin 2phi (bins 1..k)
 };
 
 struct DevModel {
@@ -129,6 +130,7 @@ struct mcx_ctx {
     double kernel_ms = 0.0, pass_ms = 0.0;
     int sm_count = 0;
     int wclk_khz = 0; // rate of the device's constant wall clock (wall_clock64)
+    size_t lds_max = 65536; // LDS bytes one workgroup may allocate
 };
 
 static inline int fail(mcx_ctx *ctx, int code, const std::string &msg) {
