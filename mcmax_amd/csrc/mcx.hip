@@ -43,6 +43,7 @@ struct TransportArgs {
     int serial;    // only lane 0 of the grid takes packets (in packet order)
     int coop;      // wave-cooperative lookups when at most this many lanes need one
     int rb_mask;   // draws per lane ring minus one (a power of two minus one)
+    int spec_steps; // histories longer than this predict their cell's energy
     double invE;   // 2^EABS_BITS / E0
     double unitE;  // E0 / 2^EABS_BITS
 };
@@ -414,7 +415,8 @@ struct RingRng {
 };
 
 #define INFLIGHT_DIV 64 // default cap on packets in flight: nphot / 64
-#define NPL 13          // per-thread LDS doubles (odd stride spreads the banks)
+#define NPL 17          // per-thread LDS doubles (odd stride spreads the banks)
+#define SPEC_STEPS 8192 // default: a history this long predicts its cell's energy (see S_INTERACT)
 #define COOP_LANES 6 // default: a wave serves lookups together when at most this many lanes need one
 
 __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
@@ -438,10 +440,9 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     // rings of Philox draws, RB per thread
     const unsigned RB = (unsigned)A.rb_mask + 1u;
     double *sRing = sCS + nbins + (size_t)blockDim.x * NPL;
-    // random walk: radial edges (AU), sin of the cone edges, survival table
+    // random walk: radial edges (AU), sin of the cone edges
     double *sRau = sRing + (size_t)blockDim.x * RB;
     double *sSinT = sRau + (nr + 1);
-    double *sPhi = sSinT + (nt + 1);
 #define P_Q pl[0]
 #define P_U pl[1]
 #define P_V pl[2]
@@ -455,6 +456,9 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
 #define P_IC pl[10]  // ic: the cell these entries belong to (-1: none)
 #define P_TC pl[11]  // 2*class + C%thick of cell ic
 #define P_KDIT pl[12] // kd_iT
+#define P_EC pl[13]   // cell of the energy prediction (-1: none)
+#define P_EPRED pl[14] // bits of the cell's Eabs after this lane's last add there
+#define P_EQ pl[15]   // bits of that add
     for (int k = threadIdx.x; k <= nr; k += blockDim.x) sR2[k] = md.R2[k];
     for (int k = threadIdx.x; k <= nt; k += blockDim.x) {
         sT2[k] = md.T2[k];
@@ -468,7 +472,6 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     if (A.rw.on) {
         for (int k = threadIdx.x; k <= nr; k += blockDim.x) sRau[k] = A.rw.R[k];
         for (int k = threadIdx.x; k <= nt; k += blockDim.x) sSinT[k] = A.rw.sinT[k];
-        for (int k = threadIdx.x; k < NY_RW; k += blockDim.x) sPhi[k] = A.rw.phi[k];
     }
     __syncthreads();
 
@@ -512,6 +515,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     // the lane: its own update, or a load issued at the interaction) and
     // {KDext, KDabs} at kd_iT
     P_IC = -1.0;
+    P_EC = -1.0;
     P_TC = 0.0;
     P_KDIT = -1.0;
     P_TRW = 0.0;
@@ -530,7 +534,10 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
             // longest history
             const bool fin = need && pid >= 0;
             unsigned long long fmask = __ballot(fin);
-            if (fin) atomicMax(&ty.counters[G_MAXSTEPS], (unsigned long long)p.nstep);
+            if (fin) {
+                atomicMax(&ty.counters[G_MAXSTEPS], (unsigned long long)p.nstep);
+                atomicAdd(&scnt[C_NSTEPS], (unsigned long long)p.nstep);
+            }
             if (fmask && lane == __ffsll((long long)fmask) - 1) {
                 const unsigned long long nf = (unsigned long long)__popcll(fmask);
                 const unsigned long long d0 = atomicAdd(&ty.counters[G_DONE], nf);
@@ -679,6 +686,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 cc = c;
                 acc.c = c;
                 c_Tl = 0.0;
+                P_EC = -1.0;
                 c_dens = nd;
                 c_u1 = n1;
                 c_u2 = n2;
@@ -701,8 +709,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 CNT(C_NGEOM);
                 st = S_NEED;
             } else {
-                CNT(C_NSTEPS);
-                p.nstep++;
+                p.nstep++; // added to C_NSTEPS when the packet ends
                 double tau1 = ((Kext * v) * c_dens) * AU;
                 if ((p.tau + tau1) > p.tau0) {
                     v = (v * (p.tau0 - p.tau)) / tau1;
@@ -754,6 +761,9 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
         lk.kind = L_SAME;
         int lk_b = 0;
         double lk_fb = 0.0, lk_fbm1 = 0.0;
+        // a predicted absorption's atomic, checked after the lookup
+        bool spec_chk = false;
+        unsigned long long spec_ret = 0ull;
 
         // ---- interaction up to its CDF lookup (Radiation.f:41-355) ------
         if (st == S_INTERACT) {
@@ -801,10 +811,29 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     }
                 }
                 const double mass = P_MASS;
-                const double Thint = __longlong_as_double((long long)__hip_atomic_load(
-                    &ty.Tcell[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-                unsigned long long q;
-                unsigned long long old = add_eabs(ty, c, Ex, A.invE, q, scnt);
+                // A long history (the tail of a pass, typically alone in its
+                // cell) takes its Bjorkman-Wood slice from the cell energy it
+                // expects after its own previous absorption here, and checks
+                // the atomic's answer after the lookup instead of waiting for
+                // it: the same slice when no other packet absorbed in the
+                // cell meanwhile (always so with one lane), and otherwise the
+                // next absorption starts from the corrected energy.
+                const bool spec = (p.nstep >= A.spec_steps) && (P_EC == (double)c) && (jw >= 1);
+                double Thint;
+                unsigned long long q, old;
+                if (spec) {
+                    Thint = c_Tl;
+                    old = (unsigned long long)__double_as_longlong(P_EPRED);
+                    spec_ret = add_eabs(ty, c, Ex, A.invE, q, scnt);
+                    spec_chk = true;
+                } else {
+                    Thint = __longlong_as_double((long long)__hip_atomic_load(&ty.Tcell[c], __ATOMIC_RELAXED,
+                                                                              __HIP_MEMORY_SCOPE_AGENT));
+                    old = add_eabs(ty, c, Ex, A.invE, q, scnt);
+                }
+                P_EC = (double)c;
+                P_EPRED = __longlong_as_double((long long)(old + q));
+                P_EQ = __longlong_as_double((long long)q);
                 if (jw < 1) {
                     jw = (int)(Thint / 1.0);
                     if (jw < 1) jw = 1;
@@ -912,16 +941,36 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     double KscaR = Ksca * g.next();
                     const double *wc = md.w + (size_t)c * ng;
                     size_t moff = (size_t)(ng - 1) * nlam + l1;
-                    for (int s = 0; s < ng; s++) {
-                        double w1 = wc[s] * 1.0;
-                        double w2 = (w1 * wl2) * tb.Ksca[(size_t)s * nlam + l2];
+                    double MIF11, MIF12;
+                    if (ng == 1) {
+                        // every table value the choice can need, in one round trip
+                        const double wv = wc[0], ks2 = tb.Ksca[l2], ks1 = tb.Ksca[l1];
+                        const double a11 = tb.IF11[l1], b11 = tb.IF11[l2], a12 = tb.IF12[l1], b12 = tb.IF12[l2];
+                        double w1 = wv * 1.0;
+                        double w2 = (w1 * wl2) * ks2;
                         KscaR = KscaR - w2;
-                        if (KscaR < 0.0) { moff = (size_t)s * nlam + l2; break; }
-                        w1 = (w1 * p.wl1) * tb.Ksca[(size_t)s * nlam + l1];
-                        KscaR = KscaR - w1;
-                        if (KscaR < 0.0) { moff = (size_t)s * nlam + l1; break; }
+                        if (KscaR < 0.0) {
+                            moff = l2;
+                        } else {
+                            w1 = (w1 * p.wl1) * ks1;
+                            KscaR = KscaR - w1;
+                            moff = l1;
+                        }
+                        MIF11 = (moff == (size_t)l2) ? b11 : a11;
+                        MIF12 = (moff == (size_t)l2) ? b12 : a12;
+                    } else {
+                        for (int s = 0; s < ng; s++) {
+                            double w1 = wc[s] * 1.0;
+                            double w2 = (w1 * wl2) * tb.Ksca[(size_t)s * nlam + l2];
+                            KscaR = KscaR - w2;
+                            if (KscaR < 0.0) { moff = (size_t)s * nlam + l2; break; }
+                            w1 = (w1 * p.wl1) * tb.Ksca[(size_t)s * nlam + l1];
+                            KscaR = KscaR - w1;
+                            if (KscaR < 0.0) { moff = (size_t)s * nlam + l1; break; }
+                        }
+                        MIF11 = tb.IF11[moff];
+                        MIF12 = tb.IF12[moff];
                     }
-                    const double MIF11 = tb.IF11[moff], MIF12 = tb.IF12[moff];
                     // scatangle (Radiation.f:381-495)
                     double IF11 = MIF11 * 180.0, IF12 = MIF12 * 180.0;
                     double phi;
@@ -1015,6 +1064,13 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
         }
 
         // ---- the interaction after its lookup ----------------------------
+        if (spec_chk) {
+            const unsigned long long qq = (unsigned long long)__double_as_longlong(P_EQ);
+            if (spec_ret + qq != (unsigned long long)__double_as_longlong(P_EPRED)) {
+                CNT(C_NSPECMISS);
+                P_EPRED = __longlong_as_double((long long)(spec_ret + qq));
+            }
+        }
         if (st == S_LOOKUP) {
             if (lk.kind != L_SCAT) {
                 // emit()'s wavelength rule (Radiation.f:966-989)
@@ -1130,7 +1186,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 if (jw < 1) jw = 1;
                 if (jw > TMAXD - 2) jw = TMAXD - 2;
                 const double w0 = kpc[jw], w1 = kpc[jw + 1], w2 = kpc[jw + 2];
-                const int iy = hunt_desc(sPhi, NY_RW, g.next());
+                const int iy = hunt_desc(rw.phi, NY_RW, g.next());
                 // y(iy) of InitRandomWalk (RandomWalk.f:160-164): 1, then a
                 // single-precision ratio times 0.75
                 const double yv = (iy == 1) ? 1.0 : (double)((float)(NY_RW - iy + 1) / (float)(NY_RW - 1)) * 0.75;
@@ -1142,6 +1198,8 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 const double ejv = ((p.E * v) * AU) * kd.y;
                 unsigned long long q;
                 const unsigned long long old = add_eabs(ty, c, ejv * c_dens, A.invE, q, scnt);
+                P_EC = (double)c;
+                P_EPRED = __longlong_as_double((long long)(old + q));
                 acc_ejv(acc, ejv, c_u1, scnt);
                 p.nEJv = p.nEJv + ejv; // the module-level nEJv of RandomWalk.f:115
                 const double T1 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)(old + q) * A.unitE) / P_MASS, Thint);
@@ -1201,6 +1259,9 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
 #undef P_IC
 #undef P_TC
 #undef P_KDIT
+#undef P_EC
+#undef P_EPRED
+#undef P_EQ
 }
 
 // ---------------------------------------------------------------------------
@@ -1772,7 +1833,7 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     if (run->nlanes == 1) threads = 64; // one wavefront: lane 0 traces, the others serve its lookups
     size_t lds = sizeof(double) * ((size_t)(md.nr + 1) + 2 * (size_t)(md.nt + 1) + 2 * 181 + (size_t)(nlam - 1) +
                                    NPL * (size_t)threads);
-    if (rw.on) lds += sizeof(double) * ((size_t)(md.nr + 1) + (size_t)(md.nt + 1) + NY_RW);
+    if (rw.on) lds += sizeof(double) * ((size_t)(md.nr + 1) + (size_t)(md.nt + 1));
     // rings of Philox draws: 16 per thread when they fit beside the rest
     int rb = 16;
     while (rb > 4 && lds + sizeof(double) * (size_t)rb * threads > ctx->lds_max) rb >>= 1;
@@ -1811,6 +1872,7 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     A.serial = (run->nlanes == 1);
     A.coop = (run->coop_lanes < 0) ? COOP_LANES : run->coop_lanes;
     A.rb_mask = rb - 1;
+    A.spec_steps = (run->spec_steps < 0) ? SPEC_STEPS : run->spec_steps;
     float kms_total = 0.f;
     for (long long base = 0; base < run->nphot; base += batch) {
         long long end = base + batch;
@@ -1909,6 +1971,7 @@ extern "C" int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *o) {
     o->tautot = tt;
     o->Eescape = Eesc;
     o->maxsteps = (int64_t)cnt[G_MAXSTEPS];
+    o->nspecmiss = (int64_t)cnt[C_NSPECMISS];
     for (int k = 0; k < 5; k++) {
         const long long t0 = (long long)cnt[G_TLINE], t1 = (long long)cnt[G_TLINE + 1 + k];
         o->t_done[k] = (t1 != 0 && ctx->wclk_khz > 0) ? (double)(t1 - t0) / (double)ctx->wclk_khz : -1.0;

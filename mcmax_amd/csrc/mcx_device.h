@@ -92,7 +92,11 @@ __host__ __device__ __forceinline__ bool hit_sphere(double Rad2, double r, doubl
     v = 1e200;
     double cc = r - Rad2;
     double discr = b * b - 4.0 * cc;
-    if (discr >= 0.0) {
+    // q = -(b + sign(b) sqrt(discr))/2 has the sign bit of -b (b > 0) or of
+    // +0 (b <= 0); the roots are q and cc/q.  When neither can be positive
+    // the square root and the division are skipped: the result is the same.
+    const bool qneg = (b > 0.0);
+    if (discr >= 0.0 && (!qneg || (qneg == (bool)__builtin_signbit(cc)))) {
         discr = sqrt(discr);
         double q = (b > 0.0) ? -0.5 * (b + discr) : -0.5 * (b - discr);
         double vr1 = q, vr2 = cc / q;
@@ -109,7 +113,10 @@ __host__ __device__ __forceinline__ bool hit_cone(double Thet, double r, double 
     double bt = Thet * b - (2.0 * z) * vz;
     double ct = Thet * r - z * z;
     double discr = bt * bt - (4.0 * at) * ct;
-    if (discr >= 0.0) {
+    // the roots q/at and ct/q, q with the sign bit of -bt (bt > 0) or of +0
+    // (bt <= 0): a root can be positive only when the signs agree
+    const bool qneg = (bt > 0.0);
+    if (discr >= 0.0 && ((qneg == (bool)__builtin_signbit(at)) || (qneg == (bool)__builtin_signbit(ct)))) {
         discr = sqrt(discr);
         double q = (bt > 0.0) ? -0.5 * (bt + discr) : -0.5 * (bt - discr);
         double vt1 = q / at, vt2 = ct / q;

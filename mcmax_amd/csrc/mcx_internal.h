@@ -26,13 +26,13 @@
 #define EABS_BITS 24  // one nominal packet energy maps to 2^24
 
 enum { C_NEMIT = 0, C_NINTERACT, C_NABS, C_NSCAT, C_NESCAPE, C_NSTARHIT, C_NSTEPS, C_NRAAR, C_NMAXINT,
-       C_NGEOM, C_NOVERFLOW, C_NRW, C_NCOUNT };
+       C_NGEOM, C_NOVERFLOW, C_NRW, C_NSPECMISS, C_NCOUNT };
 
 // slots of DevTally::counters beyond the event counters: the longest packet
 // history (cell steps), packets finished in the current launch, and wall-clock
 // stamps of the launch's start and of 50/90/99/99.9/100% finished
-#define G_MAXSTEPS 12
-#define G_DONE 13
+#define G_MAXSTEPS 14
+#define G_DONE 15
 #define G_TLINE 16
 #define G_NSLOTS 24
 
