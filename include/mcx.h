@@ -106,13 +106,6 @@ typedef struct {
                                         this many lanes need one; -1 picks
                                         the default, 0 never (results do not
                                         depend on it)                       */
-    int32_t spec_steps;              /* a packet past this many cell steps
-                                        takes its Bjorkman-Wood slices from
-                                        its own prediction of the cell's
-                                        energy and checks it afterwards; -1
-                                        picks the default (8192), 0 always
-                                        (with one lane the results do not
-                                        depend on it)                       */
 } mcx_run;
 
 /* Pass tallies and statistics (MCRadiation.f:656-715). */
@@ -138,10 +131,6 @@ typedef struct {
                                         when 50, 90, 99, 99.9 and 100% of
                                         the packets had finished (last
                                         launch of the pass; -1: unknown)    */
-    int64_t nspecmiss;               /* absorptions of long histories whose
-                                        predicted cell energy another packet
-                                        had changed meanwhile (0 with one
-                                        lane)                               */
 } mcx_tallies;
 
 /* Temperature products of Main.f:498-549 and the void cell (Main.f:546). */

@@ -42,7 +42,7 @@ class Run(C.Structure):
     _fields_ = [
         ("nphot", C.c_int64), ("seed", C.c_uint64), ("maxinteract", C.c_int32),
         ("nlanes", C.c_int32), ("bitmap_bytes_max", C.c_int64), ("randomwalk", C.c_int32),
-        ("factRW", C.c_double), ("coop_lanes", C.c_int32), ("spec_steps", C.c_int32),
+        ("factRW", C.c_double), ("coop_lanes", C.c_int32),
     ]
 
 
@@ -55,7 +55,7 @@ class Tallies(C.Structure):
         ("nsteps", C.c_int64), ("nraar", C.c_int64), ("nmaxinteract", C.c_int64),
         ("ngeomfail", C.c_int64), ("tautot", C.c_double), ("Eescape", C.c_double),
         ("nrw", C.c_int64), ("noverflow", C.c_int64), ("thick", _ip),
-        ("maxsteps", C.c_int64), ("t_done", C.c_double * 5), ("nspecmiss", C.c_int64),
+        ("maxsteps", C.c_int64), ("t_done", C.c_double * 5),
     ]
 
 

@@ -43,7 +43,6 @@ struct TransportArgs {
     int serial;    // only lane 0 of the grid takes packets (in packet order)
     int coop;      // wave-cooperative lookups when at most this many lanes need one
     int rb_mask;   // draws per lane ring minus one (a power of two minus one)
-    int spec_steps; // histories longer than this predict their cell's energy
     double invE;   // 2^EABS_BITS / E0
     double unitE;  // E0 / 2^EABS_BITS
 };
@@ -415,8 +414,7 @@ struct RingRng {
 };
 
 #define INFLIGHT_DIV 64 // default cap on packets in flight: nphot / 64
-#define NPL 17          // per-thread LDS doubles (odd stride spreads the banks)
-#define SPEC_STEPS 8192 // default: a history this long predicts its cell's energy (see S_INTERACT)
+#define NPL 13          // per-thread LDS doubles (odd stride spreads the banks)
 #define COOP_LANES 6 // default: a wave serves lookups together when at most this many lanes need one
 
 __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
@@ -456,9 +454,6 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
 #define P_IC pl[10]  // ic: the cell these entries belong to (-1: none)
 #define P_TC pl[11]  // 2*class + C%thick of cell ic
 #define P_KDIT pl[12] // kd_iT
-#define P_EC pl[13]   // cell of the energy prediction (-1: none)
-#define P_EPRED pl[14] // bits of the cell's Eabs after this lane's last add there
-#define P_EQ pl[15]   // bits of that add
     for (int k = threadIdx.x; k <= nr; k += blockDim.x) sR2[k] = md.R2[k];
     for (int k = threadIdx.x; k <= nt; k += blockDim.x) {
         sT2[k] = md.T2[k];
@@ -515,7 +510,6 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     // the lane: its own update, or a load issued at the interaction) and
     // {KDext, KDabs} at kd_iT
     P_IC = -1.0;
-    P_EC = -1.0;
     P_TC = 0.0;
     P_KDIT = -1.0;
     P_TRW = 0.0;
@@ -686,7 +680,6 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 cc = c;
                 acc.c = c;
                 c_Tl = 0.0;
-                P_EC = -1.0;
                 c_dens = nd;
                 c_u1 = n1;
                 c_u2 = n2;
@@ -761,9 +754,6 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
         lk.kind = L_SAME;
         int lk_b = 0;
         double lk_fb = 0.0, lk_fbm1 = 0.0;
-        // a predicted absorption's atomic, checked after the lookup
-        bool spec_chk = false;
-        unsigned long long spec_ret = 0ull;
 
         // ---- interaction up to its CDF lookup (Radiation.f:41-355) ------
         if (st == S_INTERACT) {
@@ -811,29 +801,10 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     }
                 }
                 const double mass = P_MASS;
-                // A long history (the tail of a pass, typically alone in its
-                // cell) takes its Bjorkman-Wood slice from the cell energy it
-                // expects after its own previous absorption here, and checks
-                // the atomic's answer after the lookup instead of waiting for
-                // it: the same slice when no other packet absorbed in the
-                // cell meanwhile (always so with one lane), and otherwise the
-                // next absorption starts from the corrected energy.
-                const bool spec = (p.nstep >= A.spec_steps) && (P_EC == (double)c) && (jw >= 1);
-                double Thint;
-                unsigned long long q, old;
-                if (spec) {
-                    Thint = c_Tl;
-                    old = (unsigned long long)__double_as_longlong(P_EPRED);
-                    spec_ret = add_eabs(ty, c, Ex, A.invE, q, scnt);
-                    spec_chk = true;
-                } else {
-                    Thint = __longlong_as_double((long long)__hip_atomic_load(&ty.Tcell[c], __ATOMIC_RELAXED,
-                                                                              __HIP_MEMORY_SCOPE_AGENT));
-                    old = add_eabs(ty, c, Ex, A.invE, q, scnt);
-                }
-                P_EC = (double)c;
-                P_EPRED = __longlong_as_double((long long)(old + q));
-                P_EQ = __longlong_as_double((long long)q);
+                const double Thint = __longlong_as_double((long long)__hip_atomic_load(
+                    &ty.Tcell[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                unsigned long long q;
+                unsigned long long old = add_eabs(ty, c, Ex, A.invE, q, scnt);
                 if (jw < 1) {
                     jw = (int)(Thint / 1.0);
                     if (jw < 1) jw = 1;
@@ -1064,13 +1035,6 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
         }
 
         // ---- the interaction after its lookup ----------------------------
-        if (spec_chk) {
-            const unsigned long long qq = (unsigned long long)__double_as_longlong(P_EQ);
-            if (spec_ret + qq != (unsigned long long)__double_as_longlong(P_EPRED)) {
-                CNT(C_NSPECMISS);
-                P_EPRED = __longlong_as_double((long long)(spec_ret + qq));
-            }
-        }
         if (st == S_LOOKUP) {
             if (lk.kind != L_SCAT) {
                 // emit()'s wavelength rule (Radiation.f:966-989)
@@ -1198,8 +1162,6 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 const double ejv = ((p.E * v) * AU) * kd.y;
                 unsigned long long q;
                 const unsigned long long old = add_eabs(ty, c, ejv * c_dens, A.invE, q, scnt);
-                P_EC = (double)c;
-                P_EPRED = __longlong_as_double((long long)(old + q));
                 acc_ejv(acc, ejv, c_u1, scnt);
                 p.nEJv = p.nEJv + ejv; // the module-level nEJv of RandomWalk.f:115
                 const double T1 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)(old + q) * A.unitE) / P_MASS, Thint);
@@ -1259,9 +1221,6 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
 #undef P_IC
 #undef P_TC
 #undef P_KDIT
-#undef P_EC
-#undef P_EPRED
-#undef P_EQ
 }
 
 // ---------------------------------------------------------------------------
@@ -1872,7 +1831,6 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     A.serial = (run->nlanes == 1);
     A.coop = (run->coop_lanes < 0) ? COOP_LANES : run->coop_lanes;
     A.rb_mask = rb - 1;
-    A.spec_steps = (run->spec_steps < 0) ? SPEC_STEPS : run->spec_steps;
     float kms_total = 0.f;
     for (long long base = 0; base < run->nphot; base += batch) {
         long long end = base + batch;
@@ -1971,7 +1929,6 @@ extern "C" int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *o) {
     o->tautot = tt;
     o->Eescape = Eesc;
     o->maxsteps = (int64_t)cnt[G_MAXSTEPS];
-    o->nspecmiss = (int64_t)cnt[C_NSPECMISS];
     for (int k = 0; k < 5; k++) {
         const long long t0 = (long long)cnt[G_TLINE], t1 = (long long)cnt[G_TLINE + 1 + k];
         o->t_done[k] = (t1 != 0 && ctx->wclk_khz > 0) ? (double)(t1 - t0) / (double)ctx->wclk_khz : -1.0;

@@ -26,7 +26,7 @@
 #define EABS_BITS 24  // one nominal packet energy maps to 2^24
 
 enum { C_NEMIT = 0, C_NINTERACT, C_NABS, C_NSCAT, C_NESCAPE, C_NSTARHIT, C_NSTEPS, C_NRAAR, C_NMAXINT,
-       C_NGEOM, C_NOVERFLOW, C_NRW, C_NSPECMISS, C_NCOUNT };
+       C_NGEOM, C_NOVERFLOW, C_NRW, C_NCOUNT };
 
 // slots of DevTally::counters beyond the event counters: the longest packet
 // history (cell steps), packets finished in the current launch, and wall-clock

@@ -73,7 +73,7 @@ class Transport:
         self.model = m
 
     def transport(self, nphot, seed=1, nlanes=0, maxinteract=0, bitmap_bytes_max=0, randomwalk=None,
-                  factRW=None, coop_lanes=-1, spec_steps=-1):
+                  factRW=None, coop_lanes=-1):
         """One MC pass (mcx_transport); tallies stay on the device.  The
         random-walk switches default to the model's keywords."""
         m = self.model
@@ -81,7 +81,7 @@ class Transport:
         f = float(getattr(m, "factRW", 10.0)) if factRW is None else float(factRW)
         r = _lib.Run(nphot=int(nphot), seed=int(seed), maxinteract=int(maxinteract), nlanes=int(nlanes),
                      bitmap_bytes_max=int(bitmap_bytes_max), randomwalk=int(rw), factRW=f,
-                     coop_lanes=int(coop_lanes), spec_steps=int(spec_steps))
+                     coop_lanes=int(coop_lanes))
         rc = self.lib.mcx_transport(self.ctx, C.byref(r))
         if rc not in (0, _lib.MCX_EGEOM):
             self._check(rc)
@@ -106,7 +106,7 @@ class Transport:
         for k in ("spec", "scatspec", "ispec", "iscatspec"):
             out[k] = out[k].reshape(NANGLE + 1, m.nlam)
         for k in ("nemit", "ninteract", "nabs", "nscat", "nescape", "nstarhit", "nsteps", "nraar",
-                  "nmaxinteract", "ngeomfail", "tautot", "Eescape", "nrw", "noverflow", "maxsteps", "nspecmiss"):
+                  "nmaxinteract", "ngeomfail", "tautot", "Eescape", "nrw", "noverflow", "maxsteps"):
             out[k] = getattr(t, k)
         out["t_done_ms"] = list(t.t_done)
         return out
@@ -219,11 +219,10 @@ def make_obs(angle_deg=45.0, nphi=45, nj0=2, nphot=1000, nphotstar=1000, nexits=
                tracescat=int(tracescat), pfstop=pfstop, seed=seed)
 
 
-def mc_radiation(tr, nphot, seed=1, nlanes=0, maxinteract=0, coop_lanes=-1, spec_steps=-1):
+def mc_radiation(tr, nphot, seed=1, nlanes=0, maxinteract=0, coop_lanes=-1):
     """MCRadiation + temperature inversion; returns a dict with the tallies
     and T, dT, dEJv."""
-    rc = tr.transport(nphot, seed=seed, nlanes=nlanes, maxinteract=maxinteract, coop_lanes=coop_lanes,
-                      spec_steps=spec_steps)
+    rc = tr.transport(nphot, seed=seed, nlanes=nlanes, maxinteract=maxinteract, coop_lanes=coop_lanes)
     out = tr.tallies()
     out["T"], out["dT"], out["dEJv"] = tr.temperature()
     out["rc"] = rc
