@@ -51,10 +51,13 @@ def measured_traffic(config, nphot):
     return t.get("bytes_per_launch")
 
 
-def cpu_baseline(m, seconds=15.0):
+def cpu_baseline(m, pass_steps, nphot, seconds=15.0):
     """Oracle (FP64 C restatement of the reference loop) on one host core, on
     a bounded sample of the same workload: a pilot of 5000 packets sizes a
-    sample of about `seconds` of CPU time."""
+    sample of about `seconds` of CPU time.  Packet histories are
+    heavy-tailed, so a sample's packets/s overstates a full pass; the rate is
+    taken per cell step and applied to the cell steps of the measured pass
+    (pass_steps for nphot packets)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     o = pyoracle.Oracle(m)
@@ -64,10 +67,14 @@ def cpu_baseline(m, seconds=15.0):
     dt = time.perf_counter() - t0
     n2 = max(5000, int(n * seconds / max(dt, 1e-3)))
     t0 = time.perf_counter()
-    o.mcradiation(n2, rng="philox", seed=98)
+    r = o.mcradiation(n2, rng="philox", seed=98)
     dt = time.perf_counter() - t0
-    return {"value": n2 / dt, "unit": "packets/s", "cores": 1, "kind": "port",
-            "sample": f"{n2} packets of the same model on one host core ({dt:.1f} s)"}
+    steps_per_s = r["nsteps"] / dt
+    pass_s = pass_steps / steps_per_s
+    return {"value": nphot / pass_s, "unit": "packets/s", "cores": 1, "kind": "port",
+            "sample": f"{n2} packets of the same model on one host core: {r['nsteps']} cell steps in {dt:.1f} s "
+                      f"({steps_per_s:.3e} steps/s, {n2 / dt:.0f} packets/s in the sample); at that rate the "
+                      f"measured pass ({pass_steps} cell steps) takes {pass_s:.0f} s"}
 
 
 def main():
@@ -80,6 +87,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain", action="store_true", help="no modified random walk")
+    ap.add_argument("--lanes", type=int, default=0,
+                    help="packets in flight (0: the library's default; 1: one lane, packets in order)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many bytes (rocprof counter calibration)")
     a = ap.parse_args()
@@ -105,13 +114,13 @@ def main():
     tr = transport.Transport(0)
     tr.set_model(m)
     for w in range(a.warmup):
-        tr.transport(a.nphot, seed=a.seed + 1000 + w)
+        tr.transport(a.nphot, seed=a.seed + 1000 + w, nlanes=a.lanes)
         tr.temperature()
         print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
     kms = []
     t0 = time.perf_counter()
     for s in range(a.steps):
-        rc = tr.transport(a.nphot, seed=a.seed + s)
+        rc = tr.transport(a.nphot, seed=a.seed + s, nlanes=a.lanes)
         k, _ = tr.timing()
         kms.append(k)
         tr.temperature()
@@ -130,7 +139,7 @@ def main():
         "config": {"workload": f"{a.config} disk model, one MC pass + T inversion per step"
                                + ("" if a.plain else ", modified random walk (factRW=10)"),
                    "nphot": a.nphot, "nr": m.nr, "nt": m.nt, "nlam": m.nlam, "ng": m.ng,
-                   "randomwalk": not a.plain},
+                   "randomwalk": not a.plain, "lanes": a.lanes or "default"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": measured_traffic(a.config, a.nphot),
                      "kernel": "k_transport", "kernel_ms": kern_s * 1e3,
@@ -138,9 +147,11 @@ def main():
                                          "escape": 8},
                      "events": {k: int(tal[k]) for k in ("nsteps", "nabs", "nscat", "nrw", "nescape")}},
         "energy_error": abs(tal["Eescape"] / m.Lstar - 1.0),
+        "timeline": {"maxsteps": int(tal["maxsteps"]), "t_done_ms": tal["t_done_ms"],
+                     "done_fractions": [0.5, 0.9, 0.99, 0.999, 1.0]},
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(m)
+        out["cpu_baseline"] = cpu_baseline(m, int(tal["nsteps"]), a.nphot)
     tr.close()
     print(json.dumps(out))
 
