@@ -74,6 +74,7 @@ class Iteration:
             t0 = time.perf_counter()
             rc = tr.transport(self.nphot, seed=self.seed + len(self.history))
             tal = tr.tallies()
+            self.last_tallies = tal
             T, dT, dEJv = tr.temperature()
             EJv = tal["EJv"] / m.V  # C%EJv per volume (Main.f:505)
             ncells = 0

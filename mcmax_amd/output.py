@@ -1,0 +1,219 @@
+"""Output products of the stage, in the reference's text formats.
+
+- the structure file (outputstruct, InputOutput.f:6-227, the ASCII branch,
+  written as denstemp.dat with DENS, TEMP, COMP, GASDENS, DENS0:
+  Main.f:605), and its reader for a restart from temperatures (the Nphot = 0
+  path, Main.f:380-384);
+- the ray-traced spectrum of an observation of type SPECTRUM
+  (spectrum<inc>.dat, Telescope.f:140-164);
+- the Monte Carlo spectra per inclination bin and angle-integrated
+  (MCSpec<inc>.dat, MCSpec.dat, MCRadiation.f:733-762).
+
+Numbers are written with 17 significant digits in E format, one value per
+field as the reference's list-directed writes; any list-directed reader
+(including the reference's own) reads them back exactly.
+"""
+import math
+import os
+
+import numpy as np
+
+GAS2DUST = 100.0  # Init.f:276
+FORMAT_NUMBER = 5  # InputOutput.f:26
+
+
+class OutputError(ValueError):
+    pass
+
+
+def _num(x):
+    return f"{float(x): .16E}"
+
+
+def _line(*xs):
+    return "  ".join(_num(x) for x in xs) + "\n"
+
+
+def angle_tag(angle_deg):
+    """The inclination in a file name: i1 of angle/10 followed by f3.1 of
+    the remainder (Telescope.f:145-147, MCRadiation.f:740-741)."""
+    a = float(angle_deg)
+    tens = int(a / 10.0)
+    rest = a - 10.0 * tens
+    t, r = f"{tens:d}", f"{rest:.1f}"
+    # a value wider than its field prints as asterisks
+    return (t if len(t) <= 1 else "*") + (r.rjust(3) if len(r) <= 3 else "***")
+
+
+# ---------------------------------------------------------------------------
+def write_denstemp(path, m, T, state=None):
+    """The structure file of radii 1..nr-1 and all theta cells: DENS, TEMP,
+    COMP, GASDENS (times gas2dust), DENS0.  state holds dens, dens0, gasdens
+    [nr, nt] and w [nr, nt, ng] (default: the model's own, as set up)."""
+    nr, nt, ng = m.nr, m.nt, m.ng
+    if state is None:
+        state = dict(dens=m.dens, dens0=m.dens, gasdens=m.dens, w=m.w)
+    T = np.asarray(T, dtype=np.float64).reshape(nr, nt)
+    dens = np.asarray(state["dens"]).reshape(nr, nt)
+    dens0 = np.asarray(state["dens0"]).reshape(nr, nt)
+    gas = np.asarray(state["gasdens"]).reshape(nr, nt)
+    w = np.asarray(state["w"]).reshape(nr, nt, ng)
+    with open(path, "w") as f:
+        f.write("# Format number\n")
+        f.write(f"{FORMAT_NUMBER:6d}\n")
+        f.write("# NR, NT, NGRAINS, NGRAINS2\n")
+        f.write(f"{nr - 1:6d}{nt:6d}{ng:6d}{1:6d}\n")
+        f.write("# Spherical radius grid [cm] (middle of cell)\n")
+        for i in range(1, nr):
+            f.write(_line(m.R_av[i]))
+        f.write("# Theta grid [rad, from pole] (middle of cell)\n")
+        for j in range(nt):
+            f.write(_line(m.theta_av[j]))
+        blocks = (("Density", dens), ("Temperature", T))
+        for name, arr in blocks:
+            f.write(f"# {name} array (for ir=0,nr-1 do for it=0,nt-1 do ...)\n")
+            for i in range(1, nr):
+                for j in range(nt):
+                    f.write(_line(arr[i, j]))
+        f.write("# Composition array (for ir=0,nr-1 do for it=0,nt-1 do ...)\n")
+        for i in range(1, nr):
+            for j in range(nt):
+                f.write(_line(*w[i, j]))
+        f.write("# Gas density array (for ir=0,nr-1 do for it=0,nt-1 do ...)\n")
+        for i in range(1, nr):
+            for j in range(nt):
+                f.write(_line(gas[i, j] * GAS2DUST))
+        f.write("# Density0 array (for ir=0,nr-1 do for it=0,nt-1 do ...)\n")
+        for i in range(1, nr):
+            for j in range(nt):
+                f.write(_line(dens0[i, j]))
+
+
+def read_denstemp(path):
+    """Read a structure file written by write_denstemp (or by the reference's
+    ASCII outputstruct with the same variables).  Returns a dict with R_av
+    [nr-1], theta_av [nt], and dens, T, gasdens (divided by gas2dust), dens0
+    [nr-1, nt] and w [nr-1, nt, ng]."""
+    with open(path) as f:
+        lines = [l.strip() for l in f if l.strip()]
+    pos = 0
+
+    def header(expect):
+        nonlocal pos
+        if not lines[pos].startswith("#") or expect.lower() not in lines[pos].lower():
+            raise OutputError(f"{path}: expected '# {expect}', found {lines[pos]!r}")
+        pos += 1
+
+    def values(n, per=1):
+        nonlocal pos
+        out = []
+        while len(out) < n * per:
+            out.extend(float(x.replace("D", "E").replace("d", "e")) for x in lines[pos].split())
+            pos += 1
+        return np.array(out[: n * per])
+
+    header("Format number")
+    fmt = int(lines[pos]); pos += 1
+    if fmt != FORMAT_NUMBER:
+        raise OutputError(f"{path}: format number {fmt}, expected {FORMAT_NUMBER}")
+    header("NR, NT")
+    nr1, nt, ng, ng2 = (int(x) for x in lines[pos].split()[:4]); pos += 1
+    if ng2 != 1:
+        raise OutputError(f"{path}: temperature-dependent opacities (NGRAINS2={ng2}) are not supported")
+    out = {}
+    header("Spherical radius grid")
+    out["R_av"] = values(nr1)
+    header("Theta grid")
+    out["theta_av"] = values(nt)
+    header("Density array")
+    out["dens"] = values(nr1 * nt).reshape(nr1, nt)
+    header("Temperature array")
+    out["T"] = values(nr1 * nt).reshape(nr1, nt)
+    header("Composition array")
+    out["w"] = values(nr1 * nt, ng).reshape(nr1, nt, ng)
+    header("Gas density array")
+    out["gasdens"] = values(nr1 * nt).reshape(nr1, nt) / GAS2DUST
+    header("Density0 array")
+    out["dens0"] = values(nr1 * nt).reshape(nr1, nt)
+    return out
+
+
+# ---------------------------------------------------------------------------
+def arrinterpol_log(x, xarr, yarr):
+    """arrinterpol(x, xarr, yarr, n, 1) (Subroutines.f:2506-2532) as the
+    reference computes it: outside the table it returns the abscissa at the
+    end, and its scan keeps the last index i with x <= xarr(i), so inside the
+    table the power law through the last two points is evaluated."""
+    n = len(xarr)
+    if x <= xarr[0]:
+        return float(xarr[0])
+    if x >= xarr[n - 1]:
+        return float(xarr[n - 1])
+    ix = 1
+    for i in range(2, n + 1):
+        if x <= xarr[i - 1]:
+            ix = i
+    x1, x2, y1, y2 = xarr[ix - 2], xarr[ix - 1], yarr[ix - 2], yarr[ix - 1]
+    pw = math.log10(y2 / y1) / math.log10(x2 / x1)
+    return y1 * (x / x1) ** pw
+
+
+def ism_extinction(m):
+    """ExtISM of the spectra: 1 without foreground extinction (keyword av,
+    Reddening in Reddening.f, which this build does not restate)."""
+    av = float(getattr(m, "keywords", {}).get("av", 0.0))
+    if av != 0.0:
+        raise OutputError("interstellar extinction (av != 0) is not supported by this build")
+    return 1.0
+
+
+def write_spectrum(path, m, lam_obs, flux, scatflux, fluxQ=None):
+    """spectrum<inc>.dat: lambda, flux, scattered flux, stellar flux [Jy]
+    and, with full scattering, the Q flux (Telescope.f:150-163)."""
+    ext = ism_extinction(m)
+    d2 = m.distance_cm ** 2
+    with open(path, "w") as f:
+        for k, lam in enumerate(lam_obs):
+            fstar1 = arrinterpol_log(float(lam), m.lam, m.Fstar)
+            cols = [lam, 1e23 * flux[k] * ext / d2, 1e23 * scatflux[k] * ext / d2, 1e23 * fstar1 / d2]
+            if m.scat_how == 2:
+                cols.append(1e23 * fluxQ[k] * ext / d2)
+            f.write(_line(*cols))
+
+
+def spectrum_filename(angle_deg, flag="", noangle=False):
+    if noangle:
+        return f"spectrum{flag}.dat"
+    return f"spectrum{angle_tag(angle_deg)}{flag}.dat"
+
+
+def write_mcspec(outdir, m, tal):
+    """MCSpec<inc>.dat for the 30 inclination bins and MCSpec.dat (the
+    angle-integrated spectrum), from the pass's tallies (spec already holds
+    the reference's 1e23 wangle wlam weights).  The error column is
+    spec/sqrt(real(ispec)) (the multiwav=.false. branch); real() is single
+    precision."""
+    ext = ism_extinction(m)
+    d2 = m.distance_cm ** 2
+    spec = np.asarray(tal["spec"]).reshape(-1, m.nlam)
+    scat = np.asarray(tal["scatspec"]).reshape(-1, m.nlam)
+    iscat = np.asarray(tal["iscatspec"]).reshape(-1, m.nlam)
+    ispec = np.asarray(tal["ispec"]).reshape(-1, m.nlam)
+    nangle = spec.shape[0] - 1
+    paths = []
+    for j in range(1, nangle + 1):
+        ang = 180.0 * math.acos((float(j) - 0.5) / float(nangle)) / math.pi
+        p = os.path.join(outdir, f"MCSpec{angle_tag(ang)}.dat")
+        with np.errstate(divide="ignore", invalid="ignore"):
+            dspec = spec[j] / np.sqrt(ispec[j].astype(np.float32)).astype(np.float64)
+        with open(p, "w") as f:
+            for i in range(m.nlam):
+                f.write(_line(m.lam[i], spec[j, i] * ext / d2, dspec[i] * ext / d2, scat[j, i] * ext / d2)
+                        [:-1] + f"  {int(iscat[j, i]):12d}  " + _num(1e23 * m.Fstar[i] / d2) + "\n")
+        paths.append(p)
+    p = os.path.join(outdir, "MCSpec.dat")
+    with open(p, "w") as f:
+        for i in range(m.nlam):
+            f.write(_line(m.lam[i], spec[0, i] * ext / d2))
+    paths.append(p)
+    return paths
