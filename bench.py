@@ -2,8 +2,10 @@
 
 A step is one Monte Carlo pass (MCRadiation.f:6) of `--nphot` packets over the
 production disk model (200 r x 100 theta cells, 200 wavelengths, dust mass
-1e-3 Msun) followed by the Lucy temperature inversion (Main.f:498-549), with all
-tables resident in HBM before the timed region.  The pass uses the modified
+1e-3 Msun, ten DHS dust species of 0.05-1000 um grains from m = 1.7 + 0.03i,
+computed on the device by ComputePart's algorithm before the timed region)
+followed by the Lucy temperature inversion (Main.f:498-549), with all tables
+resident in HBM before the timed region.  The pass uses the modified
 random walk in optically thick cells (keyword randomwalk, factRW = 10), as the
 reference's example disks do; `--plain` switches it off.  Prints one JSON line.
 """
@@ -109,9 +111,11 @@ def main():
 
     from mcmax_amd import configs, transport
     work = tempfile.mkdtemp(prefix="mcx_bench_")
-    m = configs.load_config(a.config, work, randomwalk=not a.plain)
-    print(f"[bench] model {a.config}: {m.summary()}", file=sys.stderr, flush=True)
     tr = transport.Transport(0)
+    m = configs.load_config(a.config, work, tr=tr, randomwalk=not a.plain)
+    print(f"[bench] model {a.config}: {m.summary()}", file=sys.stderr, flush=True)
+    if m.dust_stats:
+        print(f"[bench] dust opacities on the device: {m.dust_stats['ms_total']:.1f} ms", file=sys.stderr, flush=True)
     tr.set_model(m)
     for w in range(a.warmup):
         tr.transport(a.nphot, seed=a.seed + 1000 + w, nlanes=a.lanes)
@@ -147,6 +151,7 @@ def main():
                                          "escape": 8},
                      "events": {k: int(tal[k]) for k in ("nsteps", "nabs", "nscat", "nrw", "nescape")}},
         "energy_error": abs(tal["Eescape"] / m.Lstar - 1.0),
+        "dust": m.dust_stats,
         "timeline": {"maxsteps": int(tal["maxsteps"]), "t_done_ms": tal["t_done_ms"],
                      "done_fractions": [0.5, 0.9, 0.99, 0.999, 1.0]},
     }

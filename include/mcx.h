@@ -5,8 +5,9 @@
  * together with the Lucy temperature inversion that follows it in the
  * driver (Main.f:498-558), and the steps either side of it: the diffusion
  * approximation (Diffuse), the vertical-structure iteration (TempAverage,
- * DiskStructure, compareCells) and the ray-traced SED (Observe SPECTRUM,
- * TraceMono).  The reference passes everything through module
+ * DiskStructure, compareCells), the ray-traced SED (Observe SPECTRUM,
+ * TraceMono) and the dust opacities the pass reads (ComputePart, the DHS
+ * particle from optical constants).  The reference passes everything through module
  * globals (Modules.f:141-150); here the caller hands the same quantities over
  * once as plain host arrays (mcx_set_tables, mcx_set_model) and the library
  * keeps them resident in HBM.
@@ -270,6 +271,58 @@ int mcx_compare_cells(mcx_ctx *ctx, const mcx_cellstate *prev, const mcx_cellsta
 /* Duration in milliseconds of the last mcx_transport's packet kernel and of
  * the whole pass, both measured with HIP events on the context's stream. */
 int mcx_last_timing(const mcx_ctx *ctx, double *kernel_ms, double *pass_ms);
+
+/* ---- dust opacities from optical constants ---------------------------- */
+
+/* One dust species for ComputePart (ComputePart.f:1-552, the particle that
+ * Init.f:3372 computes for a `computepartNN` keyword with standard = FILE):
+ * grains of nm materials mixed in volume, sizes amin..amax sampled at ns
+ * points with n(a) ~ a^-apow, each size a distribution of hollow spheres
+ * (DHS) with vacuum fractions 0.01..fmax at nf Gauss-Legendre points
+ * (fmax = 0: solid spheres).  Optical constants as the lnk files give them
+ * (n, k >= 0), already on the wavelength grid (readrefindCP,
+ * ComputePart.f:1102-1175) and blended by the caller where blend is set. */
+typedef struct {
+    int32_t nm, ns, nf, pad;
+    double amin, amax;          /* micron                                  */
+    double apow;                /* size-distribution power                 */
+    double fmax;                /* largest DHS vacuum fraction, < 1        */
+    double rscale, mscale;      /* p%rscale, p%mscale (ComputePart.f:144-155):
+                                   1, 1 unless normalize_abun is set       */
+    const float *e1, *e2;       /* [nm][nlam] n and k (REAL)               */
+    const float *frac;          /* [nm] volume fractions, summing to 1     */
+    const float *rho;           /* [nm] material densities, g cm^-3        */
+} mcx_part_spec;
+
+/* Results for one species: p%Kabs, p%Ksca, p%Kext [nlam] (cm^2 g^-1),
+ * p%F [nlam][6][180] (F11, F12, F22, F33, F34, F44 at 0.5..179.5 deg) and
+ * p%rho (the grains' mean density). */
+typedef struct {
+    double *Kabs, *Ksca, *Kext, *F, *rho;
+} mcx_part_out;
+
+/* Work of one mcx_compute_parts call. */
+typedef struct {
+    int64_t ndhs;               /* layered-sphere evaluations (DMiLay)     */
+    int64_t nhom;               /* homogeneous-sphere evaluations (MeerhoffMie) */
+    int64_t nerr;               /* terms whose layered sphere did not converge
+                                   and fell back to a homogeneous sphere  */
+    int64_t dhs_terms, hom_terms;  /* series terms summed                  */
+    int64_t scratch_bytes;      /* device scratch of the largest batch     */
+    double ms_total;            /* wall time of the call (HIP events)      */
+} mcx_part_stats;
+
+/* ComputePart for nspec species on one wavelength grid lam [nlam] (micron),
+ * all on the device: the layered-sphere series (Toon & Ackerman, DMiLay
+ * ComputePart.f:1256-1715) for every (species, wavelength, material, size,
+ * vacuum fraction), homogeneous spheres (MeerhoffMie, ComputePart.f:1883)
+ * where the reference uses them (x > 10^4, fmax = 0, or no convergence),
+ * and the size/shape averages with the reference's single-precision sums.
+ * stats may be NULL.  Inputs outside what the reference handles (nf > 1
+ * with 0 < fmax <= 0.01; |m| so close to 1 or below that the homogeneous
+ * series would start inside its own length) return MCX_EINVAL. */
+int mcx_compute_parts(mcx_ctx *ctx, int32_t nlam, const double *lam, int32_t nspec, const mcx_part_spec *specs,
+                      mcx_part_out *out, mcx_part_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -86,10 +86,30 @@ class CellState(C.Structure):
     _fields_ = [("T", _dp), ("dT", _dp), ("dens", _dp), ("ddens", _dp), ("EJv", _dp), ("dEJv", _dp)]
 
 
+_fp = C.POINTER(C.c_float)
+
+
+class PartSpec(C.Structure):
+    _fields_ = [("nm", C.c_int32), ("ns", C.c_int32), ("nf", C.c_int32), ("pad", C.c_int32),
+                ("amin", C.c_double), ("amax", C.c_double), ("apow", C.c_double), ("fmax", C.c_double),
+                ("rscale", C.c_double), ("mscale", C.c_double), ("e1", _fp), ("e2", _fp), ("frac", _fp),
+                ("rho", _fp)]
+
+
+class PartOut(C.Structure):
+    _fields_ = [("Kabs", _dp), ("Ksca", _dp), ("Kext", _dp), ("F", _dp), ("rho", _dp)]
+
+
+class PartStats(C.Structure):
+    _fields_ = [("ndhs", C.c_int64), ("nhom", C.c_int64), ("nerr", C.c_int64), ("dhs_terms", C.c_int64),
+                ("hom_terms", C.c_int64), ("scratch_bytes", C.c_int64), ("ms_total", C.c_double)]
+
+
 EXPORTS = (
     "mcx_create", "mcx_destroy", "mcx_last_error", "mcx_set_tables", "mcx_set_model",
     "mcx_transport", "mcx_get_tallies", "mcx_temperature", "mcx_last_timing", "mcx_sed",
     "mcx_set_temperature", "mcx_diffuse", "mcx_temp_average", "mcx_disk_structure", "mcx_compare_cells",
+    "mcx_compute_parts",
 )
 
 _lib = None
@@ -124,9 +144,11 @@ def load():
     lib.mcx_temp_average.argtypes = [vp, C.c_double, C.c_int32, _dp]
     lib.mcx_disk_structure.argtypes = [vp, C.POINTER(Struct), _dp, C.POINTER(Density), C.POINTER(C.c_int64)]
     lib.mcx_compare_cells.argtypes = [vp, C.POINTER(CellState), C.POINTER(CellState), _dp, _dp, _dp]
+    lib.mcx_compute_parts.argtypes = [vp, C.c_int32, _dp, C.c_int32, C.POINTER(PartSpec), C.POINTER(PartOut),
+                                      C.POINTER(PartStats)]
     for n in ("mcx_set_tables", "mcx_set_model", "mcx_transport", "mcx_get_tallies",
               "mcx_temperature", "mcx_last_timing", "mcx_sed", "mcx_set_temperature", "mcx_diffuse",
-              "mcx_temp_average", "mcx_disk_structure", "mcx_compare_cells"):
+              "mcx_temp_average", "mcx_disk_structure", "mcx_compare_cells", "mcx_compute_parts"):
         getattr(lib, n).restype = C.c_int
     _lib = lib
     return lib

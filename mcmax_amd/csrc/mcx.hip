@@ -300,7 +300,12 @@ __device__ __forceinline__ void acc_ejv2(TallyAcc &t, double x2, double invU2, u
 //  L_SCAT  f = e1 R1[b] + e0 R0[b] > t             scattering angle, E F11 + Q F12
 //                                                 (Radiation.f:436-452)
 // R0, R1 point at cumulative rows; for L_SCAT R1 = C11, R0 = C12 of the
-// chosen Mueller matrix and e1 = E, e0 = Q.
+// chosen Mueller matrix and e1 = E, e0 = Q.  With several species
+// (ng > 1) the reemission spectrum is the cell's mixture, sum_s w_s times
+// species s's difference spectrum (Radiation.f:199-214 with KabsTot): R0, R1
+// are species 0's rows, species s's lie (TMAX+1) n further per species, and
+// ws carries the bits of a pointer to the cell's abundances (no register of
+// its own); ng is the kernel's species count.
 enum { L_SAME = 0, L_DIFF = 1, L_SCAT = 2 };
 
 struct Lookup {
@@ -309,47 +314,69 @@ struct Lookup {
     int n, kind;
 };
 
-__device__ __forceinline__ double lk_eval(const Lookup &q, int b) {
+__device__ __forceinline__ double lk_eval(const Lookup &q, int b, int ng) {
     if (q.kind == L_SCAT) return q.e1 * q.R1[b] + q.e0 * q.R0[b];
-    if (q.kind == L_SAME) return ((q.R0[q.n + b] - q.R0[b]) * q.ws) * 1.0;
+    if (ng <= 1) {
+        if (q.kind == L_SAME) return ((q.R0[q.n + b] - q.R0[b]) * q.ws) * 1.0;
+        const double a2 = 1.0 - q.e1, a4 = 1.0 - q.e0;
+        return ((((q.e1 * q.R1[q.n + b] + a2 * q.R1[b]) - q.e0 * q.R0[q.n + b]) - a4 * q.R0[b]) * q.ws) * 1.0;
+    }
     const double a2 = 1.0 - q.e1, a4 = 1.0 - q.e0;
-    return ((((q.e1 * q.R1[q.n + b] + a2 * q.R1[b]) - q.e0 * q.R0[q.n + b]) - a4 * q.R0[b]) * q.ws) * 1.0;
+    const size_t stride = (size_t)(TMAXD + 1) * q.n;
+    const double *W = (const double *)__double_as_longlong(q.ws);
+    double acc = 0.0;
+#pragma unroll 1
+    for (int s = 0; s < ng; s++) {
+        const double *R0 = q.R0 + s * stride, *R1 = q.R1 + s * stride;
+        const double d = (q.kind == L_SAME) ? (R0[q.n + b] - R0[b])
+                                            : (((q.e1 * R1[q.n + b] + a2 * R1[b]) - q.e0 * R0[q.n + b]) - a4 * R0[b]);
+        acc = acc + (d * W[s]) * 1.0;
+    }
+    return acc;
 }
 
 __device__ __forceinline__ bool lk_hit(const Lookup &q, double v) { return (q.kind == L_SCAT) ? (v > q.t) : (v >= q.t); }
 
 // one lane alone: binary search (the CDFs are non-decreasing)
-__device__ __forceinline__ void lk_single(const Lookup &q, int &b, double &fb, double &fbm1) {
+__device__ __forceinline__ void lk_single(const Lookup &q, int ng, int &b, double &fb, double &fbm1) {
     int lo = 0, hi = q.n;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
-        if (lk_hit(q, lk_eval(q, mid))) hi = mid;
+        if (lk_hit(q, lk_eval(q, mid, ng))) hi = mid;
         else lo = mid + 1;
     }
     b = lo;
-    fb = (lo < q.n) ? lk_eval(q, lo) : 0.0;
-    fbm1 = (lo > 0 && lo < q.n) ? lk_eval(q, lo - 1) : 0.0;
+    fb = (lo < q.n) ? lk_eval(q, lo, ng) : 0.0;
+    fbm1 = (lo > 0 && lo < q.n) ? lk_eval(q, lo - 1, ng) : 0.0;
 }
 
+// lane src's value, made wave-uniform (scalar registers): every lane asks for
+// the same lane's value
+__device__ __forceinline__ int ushfl(int v, int src) { return __builtin_amdgcn_readfirstlane(__shfl(v, src)); }
+__device__ __forceinline__ double ushfl(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const int lo = ushfl((int)(unsigned)b, src), hi = ushfl((int)(unsigned)(b >> 32), src);
+    return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
 __device__ __forceinline__ const double *shfl_ptr(const double *p, int src) {
     unsigned long long v = (unsigned long long)p;
-    unsigned lo = __shfl((unsigned)v, src), hi = __shfl((unsigned)(v >> 32), src);
+    unsigned lo = (unsigned)ushfl((int)(unsigned)v, src), hi = (unsigned)ushfl((int)(unsigned)(v >> 32), src);
     return (const double *)(((unsigned long long)hi << 32) | lo);
 }
 
 // the whole wavefront serves lane src's lookup: every lane evaluates 4 bins
 // per round (all loads in flight together), a ballot finds the first hit.
 // Must be called by all 64 lanes.
-__device__ void lk_wave(const Lookup &mine, int src, int lane, int &b, double &fb, double &fbm1) {
+__device__ void lk_wave(const Lookup &mine, int src, int lane, int ng, int &b, double &fb, double &fbm1) {
     Lookup q;
     q.R0 = shfl_ptr(mine.R0, src);
     q.R1 = shfl_ptr(mine.R1, src);
-    q.e0 = __shfl(mine.e0, src);
-    q.e1 = __shfl(mine.e1, src);
-    q.ws = __shfl(mine.ws, src);
-    q.t = __shfl(mine.t, src);
-    q.n = __shfl(mine.n, src);
-    q.kind = __shfl(mine.kind, src);
+    q.e0 = ushfl(mine.e0, src);
+    q.e1 = ushfl(mine.e1, src);
+    q.ws = ushfl(mine.ws, src);
+    q.t = ushfl(mine.t, src);
+    q.n = ushfl(mine.n, src);
+    q.kind = ushfl(mine.kind, src);
     b = q.n;
     fb = 0.0;
     fbm1 = 0.0;
@@ -360,7 +387,7 @@ __device__ void lk_wave(const Lookup &mine, int src, int lane, int &b, double &f
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int bb = base + 64 * k + lane;
-            v[k] = (bb < q.n) ? lk_eval(q, bb) : 0.0;
+            v[k] = (bb < q.n) ? lk_eval(q, bb, ng) : 0.0;
             h[k] = (bb < q.n) && lk_hit(q, v[k]);
         }
         bool done = false;
@@ -866,35 +893,14 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     P_SZ = sz / r;
                 }
                 double Lr = g.next() * kp;
-                int s = 0;
-                double Lrs = Lr;
-                if (ng > 1) {
-                    double accs = 0.0;
-                    int last = -1;
-                    s = -1;
-                    for (int k = 0; k < ng; k++) {
-                        double kk = kps(k);
-                        if (kk > 0.0) last = k;
-                        double prev = accs;
-                        accs = accs + kk;
-                        if (kk > 0.0 && accs >= Lr) {
-                            s = k;
-                            Lrs = Lr - prev;
-                            break;
-                        }
-                    }
-                    if (s < 0) {
-                        s = (last >= 0) ? last : 0;
-                        Lrs = 2.0 * kps(s) + 1.0; // beyond the species total: top bin rule
-                    }
-                }
-                const double *Cs = tb.CUM + (size_t)s * (TMAXD + 1) * nbins;
-                lk.R0 = Cs + (size_t)iT0 * nbins;
-                lk.R1 = Cs + (size_t)iT1 * nbins;
+                // the wavelength from the cell's mixture spectrum (one
+                // species: its own cumulative rows)
+                lk.R0 = tb.CUM + (size_t)iT0 * nbins;
+                lk.R1 = tb.CUM + (size_t)iT1 * nbins;
                 lk.e0 = e0;
                 lk.e1 = e1;
-                lk.ws = wc[s];
-                lk.t = Lrs;
+                lk.ws = (ng == 1) ? wc[0] : __longlong_as_double((long long)wc);
+                lk.t = Lr;
                 lk.n = nbins;
                 lk.kind = same ? L_SAME : L_DIFF;
                 st = S_LOOKUP;
@@ -1021,7 +1027,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                         rem &= rem - 1;
                         int b;
                         double fb, fbm1;
-                        lk_wave(lk, src, lane, b, fb, fbm1);
+                        lk_wave(lk, src, lane, ng, b, fb, fbm1);
                         if (lane == src) {
                             lk_b = b;
                             lk_fb = fb;
@@ -1029,7 +1035,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                         }
                     }
                 } else if (st == S_LOOKUP) {
-                    lk_single(lk, lk_b, lk_fb, lk_fbm1);
+                    lk_single(lk, ng, lk_b, lk_fb, lk_fbm1);
                 }
             }
         }

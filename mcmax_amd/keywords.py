@@ -97,6 +97,7 @@ def write_file(path, kw):
         for k, v in kw.items():
             if isinstance(v, bool):
                 v = ".true." if v else ".false."
-            elif isinstance(v, str) and (k in ("startype", "denstype", "scattype") or k.startswith("part")):
+            elif isinstance(v, str) and (k in ("startype", "denstype", "scattype") or k.startswith("part") or
+                                         k.endswith(":file")):
                 v = f"'{v}'"
             f.write(f"{k}={v}\n")

@@ -19,9 +19,11 @@ import os
 
 import numpy as np
 
+from . import dust
 from . import keywords as kwmod
 
 TMAX = 10000
+MAXNG = 16  # dust species the device kernels hold
 PI = 3.14159265358979323846264338328
 AU = float(np.float32(1.49598e13))       # Modules.f:8 (default REAL literal)
 PARSEC = float(np.float32(3.08568025e18))
@@ -306,8 +308,10 @@ class Model:
                 f"Lstar={self.Lstar:.6e} Mdust={self.Mdust_g:.4e} g")
 
 
-def build_model(kw, basedir="."):
-    """Build a Model from a keyword dict (see keywords.parse_file)."""
+def build_model(kw, basedir=".", tr=None, device=0):
+    """Build a Model from a keyword dict (see keywords.parse_file).
+    computepart particles are computed on the device: through the Transport
+    tr when given, else through one opened on `device` for the call."""
     p = dict(DEFAULTS)
     for k, v in kw.items():
         p[k] = v
@@ -343,29 +347,61 @@ def build_model(kw, basedir="."):
     m.dens, m.V = min_density(m.R, m.Theta, m.R_av, m.theta_av, m.Mdust_g, m.Rin, m.Rout,
                               float(p["sh1au"]), float(p["shpow"]), float(p["denspow"]))
     m.scat_how = SCAT_HOW[str(p["scattype"]).upper()]
-    # particles
-    parts = sorted((int(k[4:]), v) for k, v in p.items() if k.startswith("part") and k[4:].isdigit())
-    if not parts:
-        raise SetupError("no part<NN> keyword")
-    ng = parts[-1][0]
-    if [k for k, _ in parts] != list(range(1, ng + 1)):
-        raise SetupError("part keywords must be numbered 1..ng")
+    # particles: part<NN> files and computepart<NN> particles (each split
+    # into its ngrains size bins, Init.f:1318-1487), in index order
+    files = {int(k[4:]): v for k, v in p.items() if k.startswith("part") and k[4:].isdigit()}
+    cps = dust.particle_keywords(p)
+    idx = sorted(set(files) | set(cps))
+    if not idx:
+        raise SetupError("no part<NN> or computepart<NN> keyword")
+    if idx != list(range(1, len(idx) + 1)) or set(files) & set(cps):
+        raise SetupError("particles must be numbered 1..n, one keyword kind per number")
+    arg_abun = any(k.startswith("abun") and k[4:].isdigit() for k in p)
+    species, warg = [], []
+    for k in idx:
+        wk = float(p.get(f"abun{k}", 1.0))
+        if k in files:
+            species.append(("file", files[k]))
+            warg.append(wk)
+        else:
+            specs, wj = dust.species_specs(cps[k], m.lam, basedir)
+            species += [("compute", s_) for s_ in specs]
+            warg += [wk * w_ for w_ in wj]
+    ng = len(species)
+    if ng > MAXNG:
+        raise SetupError(f"{ng} dust species; this build holds at most {MAXNG}")
     m.ng = ng
     m.Kabs = np.zeros((ng, m.nlam))
     m.Ksca = np.zeros((ng, m.nlam))
     m.Kext = np.zeros((ng, m.nlam))
     m.F = np.zeros((ng, m.nlam, 6, 180))
-    for k, fname in parts:
-        path = fname if os.path.isabs(fname) else os.path.join(basedir, fname)
-        Ka, Ks, Ke, F = read_particle(path, m.lam)
-        Ka = np.maximum(Ka, 0.0)
-        Ks = np.maximum(Ks, 0.0)
-        m.Kabs[k - 1], m.Ksca[k - 1], m.F[k - 1] = Ka, Ks, F
-        m.Kext[k - 1] = Ka + Ks  # Init.f:3428-3436
+    m.rho_grain = np.zeros(ng)
+    comp = [i for i, (kind, _) in enumerate(species) if kind == "compute"]
+    m.dust_stats = None
+    if comp:
+        own = tr is None
+        if own:
+            from .transport import Transport
+            tr = Transport(device)
+        try:
+            res, m.dust_stats = tr.compute_parts(m.lam, [species[i][1] for i in comp])
+        finally:
+            if own:
+                tr.close()
+        for i, r in zip(comp, res):
+            m.Kabs[i], m.Ksca[i], m.F[i], m.rho_grain[i] = r["Kabs"], r["Ksca"], r["F"], r["rho"]
+    for i, (kind, fname) in enumerate(species):
+        if kind == "file":
+            path = fname if os.path.isabs(fname) else os.path.join(basedir, fname)
+            m.Kabs[i], m.Ksca[i], _, m.F[i] = read_particle(path, m.lam)
+    m.Kabs = np.maximum(m.Kabs, 0.0)
+    m.Ksca = np.maximum(m.Ksca, 0.0)
+    m.Kext = m.Kabs + m.Ksca  # Init.f:3428-3436
     finish_opacities(m, p)
     m.randomwalk = bool(p.get("randomwalk", False))
     m.factRW = float(p.get("factrw", 10.0))
-    warg = np.array([float(p.get(f"abun{k}", 1.0)) for k in range(1, ng + 1)])
+    # Init.f:2266-2272: without any abundance keyword every species weighs the same
+    warg = np.array(warg) if arg_abun else np.ones(ng)
     m.w = np.broadcast_to(warg / warg.sum(), (m.nr, m.nt, ng)).copy()
     return m
 

@@ -203,6 +203,42 @@ class Transport:
         self._check(self.lib.mcx_compare_cells(self.ctx, C.byref(a), C.byref(b), *[C.byref(x) for x in e]))
         return tuple(x.value for x in e)
 
+    def compute_parts(self, lam, parts):
+        """ComputePart on the device (mcx_compute_parts) for a list of
+        species, each a dict with e1, e2 [nm, nlam] (n, k), frac, rho [nm],
+        amin, amax, apow, fmax, ns, nf and optionally rscale, mscale.  Returns
+        (list of dicts with Kabs, Ksca, Kext [nlam], F [nlam, 6, 180], rho;
+        stats dict)."""
+        lam = _c(lam)
+        nlam = len(lam)
+        keep, specs, outs, res = [], [], [], []
+        for p in parts:
+            e1 = np.ascontiguousarray(np.atleast_2d(p["e1"]), dtype=np.float32)
+            e2 = np.ascontiguousarray(np.atleast_2d(p["e2"]), dtype=np.float32)
+            if e1.shape != e2.shape or e1.shape[1] != nlam:
+                raise ValueError("e1, e2 must be [nm, nlam]")
+            frac = np.ascontiguousarray(p["frac"], dtype=np.float32)
+            rho = np.ascontiguousarray(p["rho"], dtype=np.float32)
+            keep += [e1, e2, frac, rho]
+            fp = C.POINTER(C.c_float)
+            specs.append(_lib.PartSpec(nm=e1.shape[0], ns=int(p["ns"]), nf=int(p["nf"]), amin=float(p["amin"]),
+                                       amax=float(p["amax"]), apow=float(p["apow"]), fmax=float(p["fmax"]),
+                                       rscale=float(p.get("rscale", 1.0)), mscale=float(p.get("mscale", 1.0)),
+                                       e1=e1.ctypes.data_as(fp), e2=e2.ctypes.data_as(fp),
+                                       frac=frac.ctypes.data_as(fp), rho=rho.ctypes.data_as(fp)))
+            r = dict(Kabs=np.zeros(nlam), Ksca=np.zeros(nlam), Kext=np.zeros(nlam), F=np.zeros((nlam, 6, 180)),
+                     rho=np.zeros(1))
+            res.append(r)
+            outs.append(_lib.PartOut(**{k: ptr(v) for k, v in r.items()}))
+        sa = (_lib.PartSpec * len(specs))(*specs)
+        oa = (_lib.PartOut * len(outs))(*outs)
+        st = _lib.PartStats()
+        self._check(self.lib.mcx_compute_parts(self.ctx, nlam, ptr(lam), len(specs), sa, oa, C.byref(st)))
+        for r in res:
+            r["rho"] = float(r["rho"][0])
+        stats = {k: getattr(st, k) for k, _ in _lib.PartStats._fields_}
+        return res, stats
+
     def timing(self):
         k, p = C.c_double(), C.c_double()
         self._check(self.lib.mcx_last_timing(self.ctx, C.byref(k), C.byref(p)))

@@ -225,6 +225,22 @@ int orc_trace_ray(const orc_model *m, double x, double y, double z,
                   double vx, double vy, double vz, int i, int jj, int onEdge, int edgeNr,
                   int maxseg, int32_t *cells, double *lens, int *exit_code);
 
+/* ---- dust opacities from optical constants (mcx_oracle_dust.c) ---- */
+/* gauleg2 (ComputePart.f:1218-1250) */
+void orc_gauleg2(float x1, float x2, float *x, float *w, int n);
+/* DMiLay, the layered sphere (ComputePart.f:1256-1715); returns Err */
+int orc_dmilay(float rcore, float rshell, float wvno, float sh_re, float sh_im, float co_re, float co_im,
+               const float *mu, int numang, float *qext, float *qsca, float *gqsc, float *m1, float *m2,
+               float *s21, float *d21);
+/* MeerhoffMie, the homogeneous sphere (ComputePart.f:1883-1930) */
+int orc_meerhoff_mie(double rad, double lam, double e1, double e2, double *Csca, double *Cext, double *F11,
+                     double *F12, double *F33, double *F34, int na);
+/* ComputePart for standard = FILE, DHS (ComputePart.f:1-552) */
+int orc_compute_part(int nlam, const double *lam, int nm, const float *e1, const float *e2, const float *frac,
+                     const float *rho, double amin, double amax, double apow, double fmax, int ns, int nf,
+                     double rscale, double mscale, double *Kabs, double *Ksca, double *Kext, double *F,
+                     double *rho_av, long *counts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -121,11 +121,82 @@ def make_obs(angle_deg=45.0, nphi=45, nj0=2, nphot=1000, nphotstar=1000, nexits=
                   tracescat=int(tracescat), pfstop=pfstop)
 
 
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def gauleg2(x1, x2, n):
+    """gauleg2 nodes and weights (float32)."""
+    lib = load()
+    x, w = np.zeros(n, np.float32), np.zeros(n, np.float32)
+    fp = C.POINTER(C.c_float)
+    lib.orc_gauleg2(x1, x2, x.ctypes.data_as(fp), w.ctypes.data_as(fp), n)
+    return x, w
+
+
+def dhs_mu(nh=90):
+    """The cosines ComputePart hands the layered sphere (ComputePart.f:311-314)."""
+    i = np.arange(1, nh + 1, dtype=np.float32)
+    th = ((i - np.float32(0.5)) / np.float32(nh) * np.float32(3.1415926536)).astype(np.float64) / 2.0
+    return np.cos(th).astype(np.float32)
+
+
+def dmilay(rcore, rshell, wvno, m_shell, m_core=1.0 + 0.0j, mu=None):
+    """DMiLay: returns dict(err, qext, qsca, gqsc, m1, m2, s21, d21 [numang, 2]);
+    refractive indices are given as n - i k (imaginary part <= 0)."""
+    lib = load()
+    mu = _f32(dhs_mu() if mu is None else mu)
+    na = len(mu)
+    fp = C.POINTER(C.c_float)
+    out = {k: np.zeros((na, 2), np.float32) for k in ("m1", "m2", "s21", "d21")}
+    q = [C.c_float() for _ in range(3)]
+    err = lib.orc_dmilay(rcore, rshell, wvno, m_shell.real, m_shell.imag, m_core.real, m_core.imag,
+                         mu.ctypes.data_as(fp), na, *[C.byref(x) for x in q],
+                         *[out[k].ctypes.data_as(fp) for k in ("m1", "m2", "s21", "d21")])
+    out.update(err=err, qext=q[0].value, qsca=q[1].value, gqsc=q[2].value)
+    return out
+
+
+def meerhoff_mie(rad, lam, e1, e2, na=180):
+    """MeerhoffMie: (Csca, Cext, F11, F12, F33, F34) for one homogeneous
+    sphere, m = e1 - i e2; F from the forward direction in 1-degree bins."""
+    lib = load()
+    F = [np.zeros(na) for _ in range(4)]
+    cs, ce = C.c_double(), C.c_double()
+    rc = lib.orc_meerhoff_mie(rad, lam, e1, e2, C.byref(cs), C.byref(ce), *[p(x) for x in F], na)
+    if rc:
+        raise ValueError(f"orc_meerhoff_mie returned {rc}")
+    return (cs.value, ce.value, *F)
+
+
+def compute_part(lam, e1, e2, frac, rho, amin, amax, apow, fmax, ns, nf, rscale=1.0, mscale=1.0):
+    """ComputePart (FILE standard, DHS): e1, e2 [nm, nlam]; returns dict with
+    Kabs, Ksca, Kext [nlam], F [nlam, 6, 180], rho and the evaluation counts."""
+    lib = load()
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    nlam = len(lam)
+    e1, e2 = _f32(np.atleast_2d(e1)), _f32(np.atleast_2d(e2))
+    nm = e1.shape[0]
+    frac, rho = _f32(frac), _f32(rho)
+    fp = C.POINTER(C.c_float)
+    K = [np.zeros(nlam) for _ in range(3)]
+    F = np.zeros((nlam, 6, 180))
+    ra = C.c_double()
+    counts = (C.c_long * 3)()
+    rc = lib.orc_compute_part(nlam, p(lam), nm, e1.ctypes.data_as(fp), e2.ctypes.data_as(fp),
+                              frac.ctypes.data_as(fp), rho.ctypes.data_as(fp), amin, amax, apow, fmax, ns, nf,
+                              rscale, mscale, p(K[0]), p(K[1]), p(K[2]), p(F), C.byref(ra), counts)
+    if rc:
+        raise ValueError(f"orc_compute_part returned {rc}")
+    return dict(Kabs=K[0], Ksca=K[1], Kext=K[2], F=F, rho=ra.value, ndhs=counts[0], nhom=counts[1],
+                nerr=counts[2])
+
+
 def build(force=False):
     """Compile liboracle.so with gcc (no FMA contraction: plain IEEE FP64)."""
-    src = os.path.join(HERE, "mcx_oracle.c")
-    if force or not os.path.exists(SO) or os.path.getmtime(SO) < os.path.getmtime(src):
-        subprocess.check_call(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-o", SO, src, "-lm"])
+    srcs = [os.path.join(HERE, f) for f in ("mcx_oracle.c", "mcx_oracle_dust.c", "mcx_oracle.h")]
+    if force or not os.path.exists(SO) or os.path.getmtime(SO) < max(os.path.getmtime(s) for s in srcs):
+        subprocess.check_call(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-o", SO] + srcs[:2] + ["-lm"])
     return SO
 
 
@@ -194,6 +265,16 @@ def load():
     lib.orc_disk_structure.argtypes = [C.POINTER(OrcStruct), _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     lib.orc_disk_structure.restype = C.c_long
     lib.orc_compare_cells.argtypes = [C.c_int, C.c_int] + [_dp] * 12 + [_dp] * 3
+    _fp = C.POINTER(C.c_float)
+    lib.orc_gauleg2.argtypes = [C.c_float, C.c_float, _fp, _fp, C.c_int]
+    lib.orc_dmilay.argtypes = [C.c_float] * 7 + [_fp, C.c_int] + [_fp] * 7
+    lib.orc_dmilay.restype = C.c_int
+    lib.orc_meerhoff_mie.argtypes = [C.c_double] * 4 + [_dp] * 6 + [C.c_int]
+    lib.orc_meerhoff_mie.restype = C.c_int
+    lib.orc_compute_part.argtypes = ([C.c_int, _dp, C.c_int, _fp, _fp, _fp, _fp] + [C.c_double] * 4 +
+                                     [C.c_int, C.c_int, C.c_double, C.c_double] + [_dp] * 5 +
+                                     [C.POINTER(C.c_long)])
+    lib.orc_compute_part.restype = C.c_int
     _lib = lib
     return lib
 

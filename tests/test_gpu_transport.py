@@ -184,3 +184,35 @@ def test_parallel_statistics_tiny(gpu, tiny_model, oracle_lib):
 def test_parallel_statistics_small_random_walk(gpu, small_rw_model, oracle_lib):
     g, o = _statistics(gpu, small_rw_model, oracle_lib, 100000)
     assert g["nrw"] > 0
+
+
+def _dhs_overrides(ngrains):
+    return {"abun01": 1.0, "computepart01:file": "dhs.dat", "computepart01:amin": 0.05,
+            "computepart01:amax": 1000.0, "computepart01:apow": 3.5, "computepart01:fmax": 0.8,
+            "computepart01:ngrains": ngrains, "computepart01:blend": False}
+
+
+@pytest.fixture(scope="module")
+def tiny_dhs_model(workdir, gpu):
+    from mcmax_amd import configs
+    return configs.load_config("tiny", workdir + "/gpu_tiny_dhs", tr=gpu, mdust=1e-3, **_dhs_overrides(3))
+
+
+@pytest.fixture(scope="module")
+def tiny_dhs_rw_model(workdir, gpu):
+    from mcmax_amd import configs
+    return configs.load_config("tiny", workdir + "/gpu_tiny_dhs_rw", tr=gpu, mdust=1e-2, randomwalk=True,
+                               factrw=2.0, **_dhs_overrides(3))
+
+
+def test_serial_parity_dhs_species(gpu, tiny_dhs_model, oracle_lib):
+    """Three DHS species with different opacities, albedos and Mueller
+    matrices: the per-species choices of reemission and scattering."""
+    m = tiny_dhs_model
+    assert m.ng == 3 and not np.allclose(m.Kabs[0], m.Kabs[2])
+    _serial_compare(gpu, m, oracle_lib, 300, seed=11)
+
+
+def test_serial_parity_dhs_species_random_walk(gpu, tiny_dhs_rw_model, oracle_lib):
+    g, o = _serial_compare(gpu, tiny_dhs_rw_model, oracle_lib, 200, seed=13)
+    assert o["nrw"] > 20
