@@ -39,16 +39,16 @@ def algorithmic_bytes(t, ng, nlam):
     return total, (Bx, Ba, Bs, Brw)
 
 
-def measured_traffic(config, nphot):
+def measured_traffic(config, nphot, ng):
     """HBM bytes per k_transport launch for this workload, from the rocprofv3
     FETCH_SIZE/WRITE_SIZE runs summarised in profiles/ (scaled by a plain copy
-    of known size); None when no such measurement exists."""
+    of known size); None when no measurement of this workload exists."""
     path = os.path.join(ROOT, "profiles", "k_transport_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    if t.get("config") != config or int(t.get("nphot", -1)) != int(nphot):
+    if t.get("config") != config or int(t.get("nphot", -1)) != int(nphot) or int(t.get("ng", -1)) != int(ng):
         return None
     return t.get("bytes_per_launch")
 
@@ -145,7 +145,7 @@ def main():
                    "nphot": a.nphot, "nr": m.nr, "nt": m.nt, "nlam": m.nlam, "ng": m.ng,
                    "randomwalk": not a.plain, "lanes": a.lanes or "default"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic(a.config, a.nphot),
+                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic(a.config, a.nphot, m.ng),
                      "kernel": "k_transport", "kernel_ms": kern_s * 1e3,
                      "bytes_per_event": {"step": Bx, "absorption": Ba, "scattering": Bs, "rw_step": Brw,
                                          "escape": 8},

@@ -444,12 +444,19 @@ struct RingRng {
 #define NPL 13          // per-thread LDS doubles (odd stride spreads the banks)
 #define COOP_LANES 6 // default: a wave serves lookups together when at most this many lanes need one
 
+// MODE 0: one dust species; 1: several, with per-class reemission tables;
+// 2: several, their tables summed at every reemission probe
+template <int MODE>
 __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     extern __shared__ double lds[];
     const DevTables &tb = A.tb;
     const DevModel &md = A.md;
     const DevTally &ty = A.ty;
     const int nr = md.nr, nt = md.nt, nlam = tb.nlam, ng = tb.ng;
+    constexpr bool ONE = MODE == 0, MIX = MODE == 1;
+    // species summed per reemission probe (class tables: one row, as for one
+    // species)
+    const int lng = (MODE == 2) ? ng : 1;
     const int nbins = nlam - 1;
     double *sR2 = lds;
     double *sT2 = sR2 + (nr + 1);
@@ -821,7 +828,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     w0 = kpc[jw];
                     w1 = kpc[jw + 1];
                     w2 = kpc[jw + 2];
-                    if (ng == 1) {
+                    if (ONE) {
                         q0 = Kp0[jw];
                         q1 = Kp0[jw + 1];
                         q2 = Kp0[jw + 2];
@@ -839,7 +846,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     w0 = kpc[jw];
                     w1 = kpc[jw + 1];
                     w2 = kpc[jw + 2];
-                    if (ng == 1) {
+                    if (ONE) {
                         q0 = Kp0[jw];
                         q1 = Kp0[jw + 1];
                         q2 = Kp0[jw + 2];
@@ -866,7 +873,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 auto kps = [&](int s) {
                     const double *Kp = tb.Kp + (size_t)s * (TMAXD + 1);
                     double k00, k01, k10, k11;
-                    if (ng == 1) {
+                    if (ONE) {
                         k00 = win_at(Kp, jw, q0, q1, q2, iT0);
                         k01 = win_at(Kp, jw, q0, q1, q2, iT0 + 1);
                         k10 = win_at(Kp, jw, q0, q1, q2, iT1);
@@ -881,7 +888,15 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     return (d * wc[s]) * 1.0;
                 };
                 double kp = 0.0;
-                for (int s = 0; s < ng; s++) kp = kp + kps(s);
+                if (MIX) {
+                    // the class tables: kp from the cell's mixture Kp
+                    const double k00 = win_at(kpc, jw, w0, w1, w2, iT0), k01 = win_at(kpc, jw, w0, w1, w2, iT0 + 1);
+                    const double k10 = win_at(kpc, jw, w0, w1, w2, iT1), k11 = win_at(kpc, jw, w0, w1, w2, iT1 + 1);
+                    kp = same ? (k01 - k00) : (((a1 * k11 + a2 * k10) - a3 * k01) - a4 * k00);
+                } else {
+#pragma unroll 4
+                    for (int s = 0; s < ng; s++) kp = kp + kps(s);
+                }
                 // emit(): new direction, then the wavelength
                 CNT(C_NEMIT);
                 random_direction(g, p.vx, p.vy, p.vz);
@@ -893,13 +908,15 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     P_SZ = sz / r;
                 }
                 double Lr = g.next() * kp;
-                // the wavelength from the cell's mixture spectrum (one
-                // species: its own cumulative rows)
-                lk.R0 = tb.CUM + (size_t)iT0 * nbins;
-                lk.R1 = tb.CUM + (size_t)iT1 * nbins;
+                // the wavelength from the cell's mixture spectrum: its class
+                // table, else the species' rows summed per probe (one
+                // species: its own rows)
+                const double *Cb = MIX ? md.CUMmix + (size_t)md.cls[c] * (TMAXD + 1) * nbins : tb.CUM;
+                lk.R0 = Cb + (size_t)iT0 * nbins;
+                lk.R1 = Cb + (size_t)iT1 * nbins;
                 lk.e0 = e0;
                 lk.e1 = e1;
-                lk.ws = (ng == 1) ? wc[0] : __longlong_as_double((long long)wc);
+                lk.ws = MIX ? 1.0 : ONE ? wc[0] : __longlong_as_double((long long)wc);
                 lk.t = Lr;
                 lk.n = nbins;
                 lk.kind = same ? L_SAME : L_DIFF;
@@ -919,7 +936,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     const double *wc = md.w + (size_t)c * ng;
                     size_t moff = (size_t)(ng - 1) * nlam + l1;
                     double MIF11, MIF12;
-                    if (ng == 1) {
+                    if (ONE) {
                         // every table value the choice can need, in one round trip
                         const double wv = wc[0], ks2 = tb.Ksca[l2], ks1 = tb.Ksca[l1];
                         const double a11 = tb.IF11[l1], b11 = tb.IF11[l2], a12 = tb.IF12[l1], b12 = tb.IF12[l2];
@@ -936,14 +953,31 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                         MIF11 = (moff == (size_t)l2) ? b11 : a11;
                         MIF12 = (moff == (size_t)l2) ? b12 : a12;
                     } else {
+                        // every species' values are loaded whatever the draw
+                        // (no early exit), so the loads are in flight together;
+                        // the first running sum below zero picks, as in the
+                        // reference's loop
+                        bool found = false;
+#pragma unroll 4
                         for (int s = 0; s < ng; s++) {
-                            double w1 = wc[s] * 1.0;
-                            double w2 = (w1 * wl2) * tb.Ksca[(size_t)s * nlam + l2];
-                            KscaR = KscaR - w2;
-                            if (KscaR < 0.0) { moff = (size_t)s * nlam + l2; break; }
-                            w1 = (w1 * p.wl1) * tb.Ksca[(size_t)s * nlam + l1];
-                            KscaR = KscaR - w1;
-                            if (KscaR < 0.0) { moff = (size_t)s * nlam + l1; break; }
+                            const double ws = wc[s], k2 = tb.Ksca[(size_t)s * nlam + l2],
+                                         k1 = tb.Ksca[(size_t)s * nlam + l1];
+                            double w1 = ws * 1.0;
+                            const double w2 = (w1 * wl2) * k2;
+                            w1 = (w1 * p.wl1) * k1;
+                            if (!found) {
+                                KscaR = KscaR - w2;
+                                if (KscaR < 0.0) {
+                                    moff = (size_t)s * nlam + l2;
+                                    found = true;
+                                } else {
+                                    KscaR = KscaR - w1;
+                                    if (KscaR < 0.0) {
+                                        moff = (size_t)s * nlam + l1;
+                                        found = true;
+                                    }
+                                }
+                            }
                         }
                         MIF11 = tb.IF11[moff];
                         MIF12 = tb.IF12[moff];
@@ -1027,7 +1061,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                         rem &= rem - 1;
                         int b;
                         double fb, fbm1;
-                        lk_wave(lk, src, lane, ng, b, fb, fbm1);
+                        lk_wave(lk, src, lane, lng, b, fb, fbm1);
                         if (lane == src) {
                             lk_b = b;
                             lk_fb = fb;
@@ -1035,7 +1069,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                         }
                     }
                 } else if (st == S_LOOKUP) {
-                    lk_single(lk, ng, lk_b, lk_fb, lk_fbm1);
+                    lk_single(lk, lng, lk_b, lk_fb, lk_fbm1);
                 }
             }
         }
@@ -1252,6 +1286,20 @@ __global__ void k_cell_kp(int ncell, int ng, const double *__restrict__ Kp, cons
     double e = 0.0;
     for (int s = 0; s < ng; s++) e = e + (Kp[(size_t)s * (TMAXD + 1) + T] * w[(size_t)c * ng + s]) * 1.0;
     KpCell[idx] = e;
+}
+
+// the reemission cumulatives of each species-mix class: sum_s w_s CUM_s
+// (the cell's KabsTot spectrum, Radiation.f:199-214, per integer T)
+__global__ void k_cum_mix(int nclass, int ng, int nbins, const double *__restrict__ CUM, const double *__restrict__ wcls,
+                          double *CUMmix) {
+    const size_t n1 = (size_t)(TMAXD + 1) * nbins;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)nclass * n1) return;
+    const int k = (int)(idx / n1);
+    const size_t tb = idx - (size_t)k * n1;
+    double e = 0.0;
+    for (int s = 0; s < ng; s++) e = e + (wcls[(size_t)k * ng + s] * CUM[(size_t)s * n1 + tb]) * 1.0;
+    CUMmix[idx] = e;
 }
 
 // fixed-point units: a path segment inside a spherical shell is at most the
@@ -1668,6 +1716,21 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
         if ((r = dupload(ctx, ctx->mod_alloc, &dcls, cls.data(), (size_t)ncell))) return r;
         ctx->nclass = nclass;
         ctx->rw.cls = dcls;
+        md.cls = dcls;
+        // several species: one reemission cumulative per species-mix class,
+        // so that an absorption searches one table instead of summing the
+        // species' tables at every probe
+        md.CUMmix = nullptr;
+        const int nb = nlam - 1;
+        const size_t nmix = (size_t)nclass * (TMAXD + 1) * nb;
+        if (ng > 1 && nmix * sizeof(double) <= ((size_t)4 << 30)) {
+            double *mix;
+            if ((r = dalloc(ctx, ctx->mod_alloc, &mix, nmix))) return r;
+            hipLaunchKernelGGL(k_cum_mix, dim3((unsigned)((nmix + 255) / 256)), dim3(256), 0, ctx->stream, nclass, ng,
+                               nb, ctx->tb.CUM, ctx->d_wcls, mix);
+            HIPCHK(ctx, hipGetLastError());
+            md.CUMmix = mix;
+        }
         ctx->rw.R = dR;
         if (ctx->d_Kext && ctx->d_gsym && gr->thet &&
             (size_t)nclass * (TMAXD + 1) * sizeof(double2) <= ((size_t)4 << 30)) {
@@ -1793,6 +1856,8 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     hipLaunchKernelGGL(k_cell_units, dim3((ncell + 255) / 256), dim3(256), 0, s, md.nr, md.nt, nlam, md.R2, md.KT, E0,
                        (double *)ty.invU1, (double *)ty.invU2);
     HIPCHK(ctx, hipGetLastError());
+    // the kernel for the model's species (one; several with class tables; several)
+    void (*kern)(TransportArgs) = (ctx->ng == 1) ? k_transport<0> : md.CUMmix ? k_transport<1> : k_transport<2>;
     // launch shape: a full device of resident wavefronts, or one lane
     int threads = 256, blocks = 1;
     if (run->nlanes == 1) threads = 64; // one wavefront: lane 0 traces, the others serve its lookups
@@ -1809,7 +1874,7 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
         blocks = 1;
     } else {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport, threads, lds) != hipSuccess || per_cu < 1)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) != hipSuccess || per_cu < 1)
             per_cu = 1;
         blocks = per_cu * ctx->sm_count;
         // Packets in flight at once take their Bjorkman-Wood slices in
@@ -1849,7 +1914,7 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
         HIPCHK(ctx, hipMemsetAsync(ty.counters + G_DONE, 0, sizeof(unsigned long long), s));
         HIPCHK(ctx, hipMemsetAsync(ty.counters + G_TLINE, 0, sizeof(unsigned long long) * 6, s));
         HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
-        hipLaunchKernelGGL(k_transport, dim3(blocks), dim3(threads), lds, s, A);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, s, A);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(ctx->ev[2], s));
         hipLaunchKernelGGL(k_popcount, dim3(ncell), dim3(256), 0, s, ncell, nwords, ty.bitmap, ctx->d_Ni);

@@ -61,6 +61,9 @@ struct DevModel {
     const double *w;               // [ncell*ng]
     const double2 *KT;             // [ncell*nlam] {KabsTot, KscaTot}
     const double *KpCell;          // [ncell*(TMAX+1)]
+    const int *cls;                // [ncell] species-mix class of the cell
+    const double *CUMmix;          // [nclass][TMAX+1][nlam-1] sum_s w_s CUM_s of each
+                                   // class (several species; null: sum per event)
 };
 
 struct DevTally {
