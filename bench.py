@@ -79,6 +79,52 @@ def cpu_baseline(m, pass_steps, nphot, seconds=15.0):
                       f"measured pass ({pass_steps} cell steps) takes {pass_s:.0f} s"}
 
 
+def dust_bench(a, work, lam_stride=10):
+    """The dust-opacity generator (ComputePart on the device) for the config's
+    computepart particle: a warmup call, then `--steps` timed calls; the CPU
+    oracle on one core for every species at every lam_stride-th wavelength,
+    scaled to the full grid."""
+    from mcmax_amd import configs, dust, keywords, model, transport
+    path = configs.write_config(a.config, work)
+    kw = keywords.parse_file(path)
+    lam, _ = model.lam_grid(int(kw["nlam"]), float(kw["lam1"]), float(kw["lam2"]))
+    specs = []
+    for d in dust.particle_keywords(kw).values():
+        specs += dust.species_specs(d, lam, work)[0]
+    if not specs:
+        raise SystemExit(f"config {a.config} has no computepart particle")
+    tr = transport.Transport(0)
+    for _ in range(a.warmup):
+        tr.compute_parts(lam, specs)
+    ms = []
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        _, st = tr.compute_parts(lam, specs)
+        ms.append(1e3 * (time.perf_counter() - t0))
+    tr.close()
+    rows = len(specs) * len(lam)
+    out = {"metric": "dust opacities: (species, wavelength) rows per second", "value": rows / (min(ms) / 1e3),
+           "unit": "rows/s", "steps": a.steps, "warmup": a.warmup, "ms_per_call": min(ms),
+           "device_ms": st["ms_total"], "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{a.config} computepart particle", "species": len(specs), "nlam": len(lam),
+                      "ns": specs[0]["ns"], "nf": specs[0]["nf"]},
+           "work": st}
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        sel = list(range(0, len(lam), lam_stride))
+        t0 = time.perf_counter()
+        for p in specs:
+            pyoracle.compute_part(lam[sel], p["e1"][:, sel], p["e2"][:, sel], p["frac"], p["rho"], p["amin"],
+                                  p["amax"], p["apow"], p["fmax"], p["ns"], p["nf"])
+        dt = time.perf_counter() - t0
+        n = len(specs) * len(sel)
+        out["cpu_baseline"] = {"value": n / dt, "unit": "rows/s", "cores": 1, "kind": "port",
+                               "sample": f"every species at {len(sel)} of {len(lam)} wavelengths ({n} rows) "
+                                         f"in {dt:.1f} s on one host core"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +137,8 @@ def main():
     ap.add_argument("--plain", action="store_true", help="no modified random walk")
     ap.add_argument("--lanes", type=int, default=0,
                     help="packets in flight (0: the library's default; 1: one lane, packets in order)")
+    ap.add_argument("--dust", action="store_true",
+                    help="measure the dust-opacity generator on the model's computepart particle instead")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many bytes (rocprof counter calibration)")
     a = ap.parse_args()
@@ -111,6 +159,9 @@ def main():
 
     from mcmax_amd import configs, transport
     work = tempfile.mkdtemp(prefix="mcx_bench_")
+    if a.dust:
+        print(json.dumps(dust_bench(a, work)))
+        return
     tr = transport.Transport(0)
     m = configs.load_config(a.config, work, tr=tr, randomwalk=not a.plain)
     print(f"[bench] model {a.config}: {m.summary()}", file=sys.stderr, flush=True)
