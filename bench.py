@@ -125,6 +125,52 @@ def dust_bench(a, work, lam_stride=10):
     return out
 
 
+def _median_rel(Ta, Tb):
+    """median |Ta/Tb - 1| over the disk cells (radial index >= 1)."""
+    import numpy as np
+    a, b = np.asarray(Ta)[1:].ravel(), np.asarray(Tb)[1:].ravel()
+    ok = b > 0
+    return float(np.median(np.abs(a[ok] / b[ok] - 1.0)))
+
+
+def converge_bench(a, work):
+    """Wall time of the structure iteration (Main.f:202-791: pass, inversion,
+    Diffuse, TempAverage, DiskStructure, compareCells) on the config, with
+    --nphot packets per pass, until the reference's rule ends it (errT <
+    epsiter on two iterations in a row, or maxiter, then one last pass).
+    Reports per pass the wall time, errT and the median |dT|/T against the
+    previous pass, and the time until that median first drops to 1%."""
+    from mcmax_amd import configs, iterate, transport
+    tr = transport.Transport(0)
+    m = configs.load_config(a.config, work, tr=tr, randomwalk=not a.plain)
+    log = lambda s_: print(f"[converge] {s_}", file=sys.stderr, flush=True)
+    it = iterate.Iteration(tr, m, a.nphot, seed=a.seed, struct_iter=True, log=log)
+    t0 = time.perf_counter()
+    it.run()
+    wall = time.perf_counter() - t0
+    tr.close()
+    passes, cum, t_conv, prev = [], 0.0, None, None
+    for k, r in enumerate(it.history):
+        cum += r["wall_s"]
+        rel = _median_rel(r["T"], prev) if prev is not None else None
+        passes.append({"pass": k, "wall_s": r["wall_s"], "cumulative_s": cum, "kernel_ms": r["kernel_ms"],
+                       "errT": r.get("errT"), "median_dT_over_T_vs_previous": rel, "ndiffuse": r["ndiffuse"]})
+        if t_conv is None and rel is not None and rel <= 0.01:
+            t_conv = cum
+        prev = r["T"]
+    final = it.history[-1]["T"]
+    for q, r in zip(passes, it.history):
+        q["median_dT_over_T_vs_final"] = _median_rel(r["T"], final)
+    return {"metric": "time to converged T-field at <=1% median |dT|/T (structure iteration)",
+            "value": t_conv, "unit": "s", "higher_is_better": False, "n_gpus": 1, "wall_s": wall,
+            "passes": passes, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"{a.config} disk model, structure iteration", "nphot": a.nphot,
+                       "maxiter": it.maxiter, "epsiter": it.epsiter, "ng": m.ng,
+                       "randomwalk": not a.plain},
+            "converged_by": "median |dT|/T between consecutive passes <= 1% (the reference's own rule, "
+                            "errT < epsiter twice, decides when the loop stops)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +183,8 @@ def main():
     ap.add_argument("--plain", action="store_true", help="no modified random walk")
     ap.add_argument("--lanes", type=int, default=0,
                     help="packets in flight (0: the library's default; 1: one lane, packets in order)")
+    ap.add_argument("--converge", action="store_true",
+                    help="time the structure iteration (passes until the T-field is converged) instead")
     ap.add_argument("--dust", action="store_true",
                     help="measure the dust-opacity generator on the model's computepart particle instead")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -161,6 +209,9 @@ def main():
     work = tempfile.mkdtemp(prefix="mcx_bench_")
     if a.dust:
         print(json.dumps(dust_bench(a, work)))
+        return
+    if a.converge:
+        print(json.dumps(converge_bench(a, work)))
         return
     tr = transport.Transport(0)
     m = configs.load_config(a.config, work, tr=tr, randomwalk=not a.plain)
