@@ -4,9 +4,10 @@
 
 Reads the keyword file (with -s overrides), builds the model, runs the
 Monte Carlo passes with the structure iteration when `iter=.true.`
-(Main.f:202-791), writes denstemp.dat and the MCSpec spectra, then the
-ray-traced spectrum of every SPECTRUM observation.  Nphot = 0 reads the
-temperatures of <outdir>/denstemp.dat instead of running passes (Main.f:380).
+(Main.f:202-791), writes denstemp.dat (denstemp.fits.gz with the keyword
+outputfits) and the MCSpec spectra, then the ray-traced spectrum of every
+SPECTRUM observation.  Nphot = 0 reads the temperatures of that structure
+file in outdir instead of running passes (Main.f:380).
 Everything numeric runs on the GPU through libmcx.so.
 """
 import argparse
@@ -33,6 +34,9 @@ def main(argv=None):
     kw = keywords.parse_file(a.input, overrides=a.overrides)
     m = model.build_model(kw, basedir=os.path.dirname(os.path.abspath(a.input)))
     print(f"model: {m.summary()}", flush=True)
+    # Init.f:1556-1560: the keyword outputfits picks the FITS structure file
+    fits_out = bool(kw.get("outputfits", False))
+    structure_name = "denstemp.fits.gz" if fits_out else "denstemp.dat"
     tr = Transport(a.device)
     try:
         if a.nphot > 0:
@@ -42,16 +46,19 @@ def main(argv=None):
             T, state = last["T"], it.state
             output.write_mcspec(a.outdir, m, it.last_tallies)
         else:
-            d = output.read_denstemp(os.path.join(a.outdir, "denstemp.dat"))
+            d = output.read_denstemp(os.path.join(a.outdir, structure_name))
             if d["T"].shape != (m.nr - 1, m.nt):
-                raise SystemExit("denstemp.dat does not match the model's grid")
+                raise SystemExit(f"{structure_name} does not match the model's grid")
             T = np.zeros((m.nr, m.nt))
             T[1:] = d["T"]
             T[0] = 1.0
             state = iterate.density_state(m)
             tr.set_model(m)
             tr.set_temperature(T)
-        output.write_denstemp(os.path.join(a.outdir, "denstemp.dat"), m, T, state)
+        if fits_out:
+            output.write_denstemp_fits(os.path.join(a.outdir, structure_name), m, T, state)
+        else:
+            output.write_denstemp(os.path.join(a.outdir, structure_name), m, T, state)
         for path in a.obs:
             for tel in observe.parse_observation_file(path, m):
                 p, _, _ = observe.observe(tr, m, tel, a.outdir, T=T, seed=a.seed)

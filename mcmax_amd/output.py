@@ -1,9 +1,10 @@
 """Output products of the stage, in the reference's text formats.
 
-- the structure file (outputstruct, InputOutput.f:6-227, the ASCII branch,
-  written as denstemp.dat with DENS, TEMP, COMP, GASDENS, DENS0:
-  Main.f:605), and its reader for a restart from temperatures (the Nphot = 0
-  path, Main.f:380-384);
+- the structure file (outputstruct, InputOutput.f:6-227, written as
+  denstemp.dat with DENS, TEMP, COMP, GASDENS, DENS0: Main.f:605; with the
+  keyword outputfits its FITS form denstemp.fits.gz, InputOutput.f:230-572),
+  and its reader for a restart from temperatures (the Nphot = 0 path,
+  Main.f:380-384);
 - the ray-traced spectrum of an observation of type SPECTRUM
   (spectrum<inc>.dat, Telescope.f:140-164);
 - the Monte Carlo spectra per inclination bin and angle-integrated
@@ -91,9 +92,12 @@ def write_denstemp(path, m, T, state=None):
 
 def read_denstemp(path):
     """Read a structure file written by write_denstemp (or by the reference's
-    ASCII outputstruct with the same variables).  Returns a dict with R_av
-    [nr-1], theta_av [nt], and dens, T, gasdens (divided by gas2dust), dens0
-    [nr-1, nt] and w [nr-1, nt, ng]."""
+    ASCII outputstruct with the same variables), or its FITS form (names
+    ending in .fits or .fits.gz).  Returns a dict with R_av [nr-1], theta_av
+    [nt], and dens, T, gasdens (divided by gas2dust), dens0 [nr-1, nt] and w
+    [nr-1, nt, ng]."""
+    if path.endswith(".fits") or path.endswith(".fits.gz"):
+        return read_denstemp_fits(path)
     with open(path) as f:
         lines = [l.strip() for l in f if l.strip()]
     pos = 0
@@ -135,6 +139,54 @@ def read_denstemp(path):
     out["gasdens"] = values(nr1 * nt).reshape(nr1, nt) / GAS2DUST
     header("Density0 array")
     out["dens0"] = values(nr1 * nt).reshape(nr1, nt)
+    return out
+
+
+def write_denstemp_fits(path, m, T, state=None):
+    """The structure file as FITS (outputstruct_fits, InputOutput.f:230-572,
+    written as denstemp.fits.gz when the keyword outputfits is set): the
+    primary HDU holds the grid (R_av/AU and theta_av for every cell, shape
+    [2, nt, nr-1]) with the keys Rin, Rout [AU], nR, nTheta, ngrains,
+    ngrains2, nlam, nHDU and the names of the extensions, which hold DENS,
+    TEMP, COMP ([ngrains2+1, ngrains, nt, nr-1]: w, then wopac = 1),
+    GASDENS (times gas2dust) and DENS0 ([nt, nr-1] each).  Keyword names
+    are upper case and HDU<k> has no blank (valid FITS keywords)."""
+    from . import fits
+    from .model import AU
+    nr, nt, ng = m.nr, m.nt, m.ng
+    if state is None:
+        state = dict(dens=m.dens, dens0=m.dens, gasdens=m.dens, w=m.w)
+    T = np.asarray(T, dtype=np.float64).reshape(nr, nt)
+    cells = lambda a: np.asarray(a, dtype=np.float64).reshape(nr, nt)[1:].T.copy()  # [nt, nr-1]
+    grid = np.zeros((2, nt, nr - 1))
+    grid[0] = (np.asarray(m.R_av)[1:] / AU)[None, :]
+    grid[1] = np.asarray(m.theta_av)[:, None]
+    w = np.asarray(state["w"], dtype=np.float64).reshape(nr, nt, ng)
+    comp = np.ones((2, ng, nt, nr - 1))
+    comp[0] = np.transpose(w[1:], (2, 1, 0))
+    names = ("DENS", "TEMP", "COMP", "GASDENS", "DENS0")
+    keys = [("RIN", float(m.Rin), "[AU]"), ("ROUT", float(m.Rout), "[AU]"), ("NR", nr - 1), ("NTHETA", nt),
+            ("NGRAINS", ng), ("NGRAINS2", 1), ("NLAM", int(m.nlam)), ("NHDU", len(names))]
+    keys += [(f"HDU{k}", n) for k, n in enumerate(names, 1)]
+    arrays = [grid, cells(state["dens"]), cells(T), comp, cells(state["gasdens"]) * GAS2DUST, cells(state["dens0"])]
+    fits.write(path, arrays, keys)
+
+
+def read_denstemp_fits(path):
+    """Read a structure file written by write_denstemp_fits: the same dict
+    as read_denstemp (arrays [nr-1, nt])."""
+    from . import fits
+    from .model import AU
+    hdus = fits.read(path)
+    hdr, grid = hdus[0]
+    names = [hdr.get(f"HDU{k}") for k in range(1, int(hdr.get("NHDU", 0)) + 1)]
+    if names[:5] != ["DENS", "TEMP", "COMP", "GASDENS", "DENS0"] or len(hdus) < 6:
+        raise OutputError(f"{path}: expected DENS, TEMP, COMP, GASDENS, DENS0 extensions, found {names}")
+    out = {"R_av": grid[0, 0, :] * AU, "theta_av": grid[1, :, 0].copy()}
+    for k, key in ((1, "dens"), (2, "T"), (4, "gasdens"), (5, "dens0")):
+        out[key] = hdus[k][1].T.copy()
+    out["gasdens"] = out["gasdens"] / GAS2DUST
+    out["w"] = np.transpose(hdus[3][1][0], (2, 1, 0)).copy()
     return out
 
 

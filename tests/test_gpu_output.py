@@ -37,3 +37,18 @@ def test_cli_pass_outputs_and_restart(tmp_path):
     assert cli.main([inp, "0", str(obs), "-o", str(out2)]) == 0
     s2 = np.loadtxt(out2 / "spectrum60.0.dat")
     assert np.allclose(s1, s2, rtol=1e-12, atol=0)
+
+
+def test_cli_fits_structure_and_restart(tmp_path):
+    """outputfits: the structure file is denstemp.fits.gz, and Nphot = 0
+    restarts from it."""
+    inp = configs.write_config("tiny", str(tmp_path / "in"))
+    out = tmp_path / "run"
+    print("cli: tiny, outputfits", flush=True)
+    assert cli.main([inp, "20000", "-o", str(out), "-s", "outputfits=.true."]) == 0
+    assert os.path.exists(out / "denstemp.fits.gz") and not os.path.exists(out / "denstemp.dat")
+    d = output.read_denstemp(str(out / "denstemp.fits.gz"))
+    assert np.all(d["T"] > 0)
+    assert cli.main([inp, "0", "-o", str(out), "-s", "outputfits=.true."]) == 0
+    e = output.read_denstemp(str(out / "denstemp.fits.gz"))
+    assert np.array_equal(d["T"], e["T"])

@@ -216,3 +216,11 @@ def test_serial_parity_dhs_species(gpu, tiny_dhs_model, oracle_lib):
 def test_serial_parity_dhs_species_random_walk(gpu, tiny_dhs_rw_model, oracle_lib):
     g, o = _serial_compare(gpu, tiny_dhs_rw_model, oracle_lib, 200, seed=13)
     assert o["nrw"] > 20
+
+
+def test_parallel_statistics_dhs_species(gpu, tiny_dhs_model, oracle_lib):
+    """Three DHS species (class reemission tables, species pick) at full
+    device concurrency against the oracle's serial loop: the temperature
+    criteria and energy conservation."""
+    g, o = _statistics(gpu, tiny_dhs_model, oracle_lib, 100000)
+    assert tiny_dhs_model.ng == 3

@@ -135,3 +135,65 @@ def test_observation_file(tiny_model, tmp_path):
     p.write_text("type=IMAGE\n")
     with pytest.raises(observe.ObservationError):
         observe.parse_observation_file(str(p), m)
+
+
+def _parse_fits_independently(raw):
+    """A separate reading of the FITS rules: 80-column cards, '= ' in columns
+    9-10, END, 2880-byte blocks, big-endian doubles."""
+    import re
+    assert len(raw) % 2880 == 0
+    pos, out = 0, []
+    while pos < len(raw):
+        cards = {}
+        while True:
+            blk = raw[pos:pos + 2880]
+            pos += 2880
+            lines = [blk[i:i + 80].decode("ascii") for i in range(0, 2880, 80)]
+            end = False
+            for ln in lines:
+                assert len(ln) == 80
+                if ln.startswith("END     "):
+                    end = True
+                    break
+                mo = re.match(r"^([A-Z0-9_-]{1,8})\s*= +(.*)$", ln)
+                if mo:
+                    cards[mo.group(1)] = mo.group(2).split(" / ")[0].strip()
+            if end:
+                break
+        dims = [int(cards[f"NAXIS{k}"]) for k in range(1, int(cards["NAXIS"]) + 1)]
+        n = int(np.prod(dims))
+        data = np.frombuffer(raw[pos:pos + 8 * n], dtype=">f8").reshape(dims[::-1])
+        pos += 8 * n
+        pos += (-pos) % 2880
+        out.append((cards, data))
+    return out
+
+
+def test_fits_structure_file(tiny_model, tmp_path):
+    import gzip
+    m = tiny_model
+    st = iterate.density_state(m)
+    rng = np.random.default_rng(5)
+    T = rng.uniform(10.0, 900.0, (m.nr, m.nt))
+    st["gasdens"] = st["dens"] * 0.5
+    p = tmp_path / "denstemp.fits.gz"
+    output.write_denstemp_fits(str(p), m, T, st)
+    raw = gzip.open(p).read()
+    hdus = _parse_fits_independently(raw)
+    assert len(hdus) == 6
+    h0, grid = hdus[0]
+    assert h0["SIMPLE"] == "T" and h0["BITPIX"] == "-64" and h0["EXTEND"] == "T"
+    assert int(h0["NR"]) == m.nr - 1 and int(h0["NTHETA"]) == m.nt and int(h0["NGRAINS"]) == m.ng
+    assert [hdus[k][0]["XTENSION"].strip("' ") for k in range(1, 6)] == ["IMAGE"] * 5
+    # Fortran a(i, j) with i (radius) fastest: numpy [j, i]
+    from mcmax_amd.model import AU
+    assert np.array_equal(grid[0, 3, :], m.R_av[1:] / AU)
+    assert np.array_equal(grid[1, :, 2], m.theta_av)
+    assert np.array_equal(hdus[2][1], T[1:].T)
+    assert hdus[3][1].shape == (2, m.ng, m.nt, m.nr - 1)
+    assert np.array_equal(hdus[4][1], (st["gasdens"] * 100.0)[1:].T)
+    # and the library's own reader gives back the ASCII reader's dict
+    d = output.read_denstemp(str(p))
+    assert np.array_equal(d["T"], T[1:]) and np.array_equal(d["dens"], st["dens"][1:])
+    assert np.allclose(d["gasdens"], st["gasdens"][1:], rtol=1e-15, atol=0)
+    assert np.array_equal(d["w"], m.w[1:])
