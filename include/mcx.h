@@ -272,6 +272,35 @@ int mcx_compare_cells(mcx_ctx *ctx, const mcx_cellstate *prev, const mcx_cellsta
  * the whole pass, both measured with HIP events on the context's stream. */
 int mcx_last_timing(const mcx_ctx *ctx, double *kernel_ms, double *pass_ms);
 
+/* ---- images ------------------------------------------------------------ */
+
+/* The pixel image of an IMAGE observation (Telescope.f:166-178 with
+ * MakeImage, TraceMono.f:2759-3020, no PSF, noise or planets). */
+typedef struct {
+    int32_t npix;                    /* tel%npixel [500]                    */
+    int32_t nint;                    /* tel%nint: samples per polar cell
+                                        [1000]                              */
+    double Rmax;                     /* half width of the field in AU
+                                        (fov/2)                             */
+    double pa_deg;                   /* D%PA, keyword posangle [0]          */
+    uint64_t seed;                   /* Philox key of the samples (the
+                                        reference draws from ran2)          */
+} mcx_image_opts;
+
+/* An IMAGE observation at wavelength lam (micron): the image rays of
+ * TracePath laid out at lam (TraceMono.f:2098-2229), their I, Q, U, V
+ * (TraceFlux, TraceMono.f:325-480, as for mcx_sed), then MakeImage's
+ * spread onto npix x npix pixels: im, imQ, imU, imV [npix*npix], pixel
+ * (ix, iy) at (ix-1)*npix + iy-1, before MakeImage's final scaling by
+ * 1e23 zscale / distance^2.  imQ, imU, imV are needed with full scattering
+ * (scat_how = 2) and may be NULL otherwise.  *nimg receives the number of
+ * rings; when Rimg [ray_cap] or rays [4][ray_cap*nphi] are given, the
+ * rings' radii (AU) and the rays' I, Q, U, V (ray = ring*nphi + azimuth)
+ * are copied there (MCX_EINVAL if ray_cap < *nimg). */
+int mcx_image(mcx_ctx *ctx, const mcx_obs *obs, const double *T, const double *theta_av, double lam,
+              const mcx_image_opts *opts, double *im, double *imQ, double *imU, double *imV, int32_t *nimg,
+              int32_t ray_cap, double *Rimg, double *rays);
+
 /* ---- dust opacities from optical constants ---------------------------- */
 
 /* One dust species for ComputePart (ComputePart.f:1-552, the particle that

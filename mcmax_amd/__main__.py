@@ -61,8 +61,8 @@ def main(argv=None):
             output.write_denstemp(os.path.join(a.outdir, structure_name), m, T, state)
         for path in a.obs:
             for tel in observe.parse_observation_file(path, m):
-                p, _, _ = observe.observe(tr, m, tel, a.outdir, T=T, seed=a.seed)
-                print(f"wrote {p}", flush=True)
+                for p in observe.observe(tr, m, tel, a.outdir, T=T, seed=a.seed):
+                    print(f"wrote {p}", flush=True)
     finally:
         tr.close()
     return 0

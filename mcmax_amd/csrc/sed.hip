@@ -870,9 +870,9 @@ struct RayArgs {
     const long long *off;
     const int *cnt, *seg_c, *hitstar;
     const unsigned char *seg_irg, *seg_k, *seg_j1, *seg_j2;
-    const double *seg_v, *fracirg, *dens, *V, *Kext, *Alb, *emis, *sf, *sQ;
+    const double *seg_v, *fracirg, *dens, *V, *Kext, *Alb, *emis, *sf, *sQ, *sU, *sV;
     double Istar;
-    double *image, *scatim, *imageQ;
+    double *image, *scatim, *imageQ, *imageU, *imageV; // U, V may be null (SED)
     int nray;
 };
 
@@ -881,13 +881,14 @@ __global__ void k_rays(RayArgs A) {
     if (ray >= A.nray) return;
     const long long s0 = A.off[ray];
     const int n = A.cnt[ray];
-    double img = 0.0, sim = 0.0, imQ = 0.0, tau = 0.0, fact = 1.0;
+    double img = 0.0, sim = 0.0, imQ = 0.0, imU = 0.0, imV = 0.0, tau = 0.0, fact = 1.0;
+    const bool uv = A.imageU != nullptr && A.scat_how == 2;
     bool stopped = false;
     const double tau_max = 1e8;
     for (int s = 0; s < n; s++) {
         const long long q = s0 + s;
         const int c = A.seg_c[q], ip = c / A.nt, jp = c % A.nt, irg = A.seg_irg[q], kp = A.seg_k[q];
-        double x_scat = 0.0, x_scatQ = 0.0;
+        double x_scat = 0.0, x_scatQ = 0.0, x_scatU = 0.0, x_scatV = 0.0;
         if (A.mc) {
             const int jj1 = A.seg_j1[q], jj2 = A.seg_j2[q];
             const int djj = (jj1 <= jj2) ? 1 : -1;
@@ -897,18 +898,26 @@ __global__ void k_rays(RayArgs A) {
                 const size_t k = sf_idx(A.NRG, A.rgoff, ip, jp, irg, jj, kp);
                 x_scat = x_scat + ww * A.sf[k];
                 if (A.scat_how == 2) x_scatQ = x_scatQ + ww * A.sQ[k];
+                if (uv) {
+                    x_scatU = x_scatU + ww * A.sU[k];
+                    x_scatV = x_scatV + ww * A.sV[k];
+                }
                 njj++;
                 if (jj == jj2) break;
                 jj = jj + djj;
             }
             x_scat = x_scat / (double)(float)njj;
             x_scatQ = x_scatQ / (double)(float)njj;
+            x_scatU = x_scatU / (double)(float)njj;
+            x_scatV = x_scatV / (double)(float)njj;
         }
         const double tau_e = ((A.seg_v[q] * A.dens[c]) * A.Kext[c]) * MCX_AU;
         if (ip != 0) {
             if (tau_e < 1e-6) {
                 img = img + ((2.0 * x_scat + A.emis[c]) * tau_e) * fact;
                 imQ = imQ + ((2.0 * x_scatQ) * tau_e) * fact;
+                imU = imU + ((2.0 * x_scatU) * tau_e) * fact;
+                imV = imV + ((2.0 * x_scatV) * tau_e) * fact;
                 sim = sim + ((2.0 * x_scat) * tau_e) * fact;
                 fact = fact * (1.0 - tau_e);
             } else {
@@ -916,6 +925,8 @@ __global__ void k_rays(RayArgs A) {
                 const double frac = 1.0 - ex;
                 img = img + ((2.0 * x_scat + A.emis[c]) * frac) * fact;
                 imQ = imQ + ((2.0 * x_scatQ) * frac) * fact;
+                imU = imU + ((2.0 * x_scatU) * frac) * fact;
+                imV = imV + ((2.0 * x_scatV) * frac) * fact;
                 sim = sim + ((2.0 * x_scat) * frac) * fact;
                 fact = fact * ex;
             }
@@ -930,6 +941,149 @@ __global__ void k_rays(RayArgs A) {
     A.image[ray] = img;
     A.scatim[ray] = sim;
     A.imageQ[ray] = imQ;
+    if (A.imageU) {
+        A.imageU[ray] = imU;
+        A.imageV[ray] = imV;
+    }
+}
+
+// MakeImage (TraceMono.f:2759-3020): the ray-traced image on its rings
+// (nimg radii, nphi azimuths over half a turn) spread over a square of
+// npix^2 pixels of half-width Rmax (AU).  One thread per polar cell (ring
+// i, azimuth j, 1-based as the reference); nint samples at random radius
+// and azimuth, each deposited at phi and at -phi (U, V change sign there).
+// Full scattering interpolates I-|P|, and the two linear-polarisation
+// amplitudes, in log between the four neighbouring rays where the ring
+// spacing exceeds a pixel, falling back to linear interpolation where that
+// yields a non-finite value; otherwise log-linear I.  Draws: Philox stream
+// (seed, (i-1)*nphi + j-1), two per sample.
+struct ImgArgs {
+    int nimg, nphi, npix, nint, scat_how;
+    const double *R;                // [nimg] ring radii (AU)
+    const double *I, *Q, *U, *V;    // [nimg*nphi] ray images, ray = ring*nphi + azimuth
+    double Rmax, pa, au;            // half field (AU), position angle (deg), AU in cm
+    unsigned long long seed;
+    double *im, *imQ, *imU, *imV;   // [npix*npix], pixel (ix, iy) at (ix-1)*npix + iy-1
+};
+
+// number_invalid (Settling.f:652-674): not a finite value below DBL_MAX/2
+__device__ __forceinline__ bool img_invalid(double a) {
+    const double b = (a * 2.0) / 2.0, c = a - 1e100;
+    return !(b == a && c < a);
+}
+
+__global__ __launch_bounds__(64) void k_image(ImgArgs A) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (A.nimg - 1) * A.nphi) return;
+    const int i = t / A.nphi + 1, j = t % A.nphi + 1;
+    const int np = A.nphi;
+    auto at = [&](const double *X, int ir, int jp) { return X[(size_t)(ir - 1) * np + (jp - 1)]; };
+    const double Ri = A.R[i - 1], Ri1 = A.R[i];
+    const double w2 = ((((2.0 * MCX_PI) * (A.au * A.au)) * (Ri1 - Ri)) / (double)(float)np);
+    const double den = (double)(float)(2 * A.nint);
+    const double fpix = (double)(float)A.npix;
+    const double par = (A.pa * MCX_PI) / 180.0;
+    PhiloxStream g;
+    g.init(A.seed, (unsigned long long)t);
+    for (int k = 0; k < A.nint; k++) {
+        const double ranR = g.next(), ranPhi = g.next();
+        const double R = Ri + (Ri1 - Ri) * ranR;
+        const double phi = (MCX_PI * ((double)(float)(j - 1) + ranPhi)) / (double)(float)np;
+        double wp1, wp2;
+        int jp1, jp2;
+        if (ranPhi < 0.5) {
+            wp1 = 0.5 - ranPhi;
+            jp1 = j - 1;
+            if (jp1 == 0) jp1 = 1;
+            wp2 = ranPhi + 0.5;
+            jp2 = j;
+        } else {
+            wp1 = ranPhi - 0.5;
+            jp1 = j + 1;
+            if (jp1 == np + 1) jp1 = np;
+            wp2 = 1.5 - ranPhi;
+            jp2 = j;
+        }
+        const double wr1 = 1.0 - ranR, wr2 = ranR;
+        const int ir1 = i, ir2 = i + 1;
+        auto lin = [&](const double *X) {
+            return (wr1 * (wp1 * at(X, ir1, jp1) + wp2 * at(X, ir1, jp2)) +
+                    wr2 * (wp1 * at(X, ir2, jp1) + wp2 * at(X, ir2, jp2))) * w2 * R;
+        };
+        double flux, fQ = 0.0, fU = 0.0, fV = 0.0;
+        if (A.scat_how == 2) {
+            if (fabs(A.R[ir2 - 1] - A.R[ir1 - 1]) > (A.Rmax / fpix)) {
+                double iv[4], al[4], ar[4], cv[4], sv[4];
+                const int irs[4] = {ir1, ir1, ir2, ir2}, jps[4] = {jp1, jp2, jp1, jp2};
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const double Iq = at(A.I, irs[q], jps[q]), Qq = at(A.Q, irs[q], jps[q]);
+                    const double Uq = at(A.U, irs[q], jps[q]), Vq = at(A.V, irs[q], jps[q]);
+                    const double pq = sqrt(Qq * Qq + Uq * Uq + Vq * Vq);
+                    iv[q] = Iq - pq;
+                    al[q] = sqrt((pq + Qq) / 2.0);
+                    ar[q] = sqrt((pq - Qq) / 2.0);
+                    cv[q] = Uq / ((2.0 * al[q]) * ar[q]);
+                    sv[q] = Vq / ((2.0 * al[q]) * ar[q]);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    iv[q] = log10(iv[q]);
+                    al[q] = log10(al[q]);
+                    ar[q] = log10(ar[q]);
+                }
+                auto mix = [&](const double *v) { return wr1 * (wp1 * v[0] + wp2 * v[1]) + wr2 * (wp1 * v[2] + wp2 * v[3]); };
+                const double ea = pow(10.0, mix(al)), eb = pow(10.0, mix(ar));
+                flux = ea * ea;
+                flux = flux + eb * eb;
+                flux = flux + pow(10.0, mix(iv));
+                flux = (flux * w2) * R;
+                fQ = ea * ea;
+                fQ = fQ - eb * eb;
+                fQ = (fQ * w2) * R;
+                fU = ea;
+                fU = fU * eb;
+                fU = (fU * 2.0) * mix(cv);
+                fU = (fU * w2) * R;
+                fV = ea;
+                fV = fV * eb;
+                fV = (fV * 2.0) * mix(sv);
+                fV = (fV * w2) * R;
+                if (img_invalid(flux) || img_invalid(fQ) || img_invalid(fU) || img_invalid(fV)) {
+                    flux = lin(A.I);
+                    fQ = lin(A.Q);
+                    fU = lin(A.U);
+                    fV = lin(A.V);
+                }
+            } else {
+                flux = lin(A.I);
+                fQ = lin(A.Q);
+                fU = lin(A.U);
+                fV = lin(A.V);
+            }
+        } else {
+            flux = wr1 * (wp1 * log10(at(A.I, ir1, jp1)) + wp2 * log10(at(A.I, ir1, jp2))) +
+                   wr2 * (wp1 * log10(at(A.I, ir2, jp1)) + wp2 * log10(at(A.I, ir2, jp2)));
+            flux = (pow(10.0, flux) * w2) * R;
+        }
+        for (int side = 0; side < 2; side++) {
+            const double ph = side ? -phi : phi;
+            double x = -R * cos(ph + par);
+            x = (fpix * (x + A.Rmax)) / (2.0 * A.Rmax) + 1.0;
+            double y = R * sin(ph + par);
+            y = (fpix * (y + A.Rmax)) / (2.0 * A.Rmax) + 1.0;
+            const int ix = (int)x, iy = (int)y;
+            if (ix <= A.npix && iy <= A.npix && ix > 0 && iy > 0) {
+                const size_t px = (size_t)(ix - 1) * A.npix + (iy - 1);
+                atomicAdd(&A.im[px], flux / den);
+                if (A.scat_how == 2) {
+                    atomicAdd(&A.imQ[px], fQ / den);
+                    atomicAdd(&A.imU[px], side ? -(fU / den) : fU / den);
+                    atomicAdd(&A.imV[px], side ? -(fV / den) : fV / den);
+                }
+            }
+        }
+    }
 }
 
 // ===========================================================================
@@ -1074,8 +1228,26 @@ restart:
     return nimg;
 }
 
+// the rays of the last wavelength, kept for an image
+struct RayImages {
+    std::vector<double> R;          // ring radii (AU)
+    int nimg = 0, nphi = 0;
+    std::vector<double> I, Q, U, V; // [nimg*nphi]
+};
+
+static int sed_core(mcx_ctx *ctx, const mcx_obs *o, const double *T, const double *theta_av, int32_t nlam_obs,
+                    const double *lam_obs, double *flux, double *scatflux, double *fluxQ, double path_lam,
+                    RayImages *ro);
+
 extern "C" int mcx_sed(mcx_ctx *ctx, const mcx_obs *o, const double *T, const double *theta_av, int32_t nlam_obs,
                        const double *lam_obs, double *flux, double *scatflux, double *fluxQ) {
+    // Telescope.f:140: the SED's rays are laid out at 0.55 micron
+    return sed_core(ctx, o, T, theta_av, nlam_obs, lam_obs, flux, scatflux, fluxQ, 0.55, nullptr);
+}
+
+static int sed_core(mcx_ctx *ctx, const mcx_obs *o, const double *T, const double *theta_av, int32_t nlam_obs,
+                    const double *lam_obs, double *flux, double *scatflux, double *fluxQ, double path_lam,
+                    RayImages *ro) {
     if (!ctx || !o || !theta_av || nlam_obs < 0 || (nlam_obs > 0 && (!lam_obs || !flux))) return MCX_EINVAL;
     if (!ctx->have_model) return fail(ctx, MCX_ESTATE, "mcx_set_model first");
     if (!ctx->d_Kext || ctx->h_thet.empty())
@@ -1145,7 +1317,7 @@ extern "C" int mcx_sed(mcx_ctx *ctx, const mcx_obs *o, const double *T, const do
         dirs[5 * NPHIS + i - 1] = z / rr;
     }
     for (int a = 1; a <= 180; a++) pisin[a - 1] = MCX_PI * sin((MCX_PI * ((double)a - 0.5)) / 180.0);
-    const double R01 = tau01_midplane(ctx, 0.55);
+    const double R01 = tau01_midplane(ctx, path_lam);
     std::vector<double> Rimg;
     const int nimg = image_radii(ctx, o, theta_av, R01, Rimg);
     const int nphi = o->nphi, nray = nimg * nphi;
@@ -1241,10 +1413,14 @@ extern "C" int mcx_sed(mcx_ctx *ctx, const mcx_obs *o, const double *T, const do
     if ((r = dalloc(ctx, tmp, &sQ, nsf))) return r;
     if ((r = dalloc(ctx, tmp, &sU, nsf))) return r;
     if ((r = dalloc(ctx, tmp, &sV, nsf))) return r;
-    double *image, *scatim, *imageQ;
+    double *image, *scatim, *imageQ, *imageU = nullptr, *imageV = nullptr;
     if ((r = dalloc(ctx, tmp, &image, (size_t)nray))) return r;
     if ((r = dalloc(ctx, tmp, &scatim, (size_t)nray))) return r;
     if ((r = dalloc(ctx, tmp, &imageQ, (size_t)nray))) return r;
+    if (ro) {
+        if ((r = dalloc(ctx, tmp, &imageU, (size_t)nray))) return r;
+        if ((r = dalloc(ctx, tmp, &imageV, (size_t)nray))) return r;
+    }
     std::vector<double> h_img(nray), h_sim(nray), h_imQ(nray);
     const unsigned cb = (unsigned)((ncell + 255) / 256);
     for (int il = 0; il < nlam_obs; il++) {
@@ -1333,6 +1509,10 @@ extern "C" int mcx_sed(mcx_ctx *ctx, const mcx_obs *o, const double *T, const do
         B.emis = emis;
         B.sf = sf;
         B.sQ = sQ;
+        B.sU = sU;
+        B.sV = sV;
+        B.imageU = imageU;
+        B.imageV = imageV;
         B.Istar = (((ctx->h_Fstar[M.l1] * M.wl1 + ctx->h_Fstar[M.l2] * M.wl2) * 1.0) * 1.0) /
                   (MCX_PI * ((ctx->h_R[0] * MCX_AU) * (ctx->h_R[0] * MCX_AU)));
         B.image = image;
@@ -1359,6 +1539,89 @@ extern "C" int mcx_sed(mcx_ctx *ctx, const mcx_obs *o, const double *T, const do
         flux[il] = fl;
         if (scatflux) scatflux[il] = sfl;
         if (fluxQ) fluxQ[il] = fq;
+        if (ro && il == nlam_obs - 1) {
+            ro->R = Rimg;
+            ro->nimg = nimg;
+            ro->nphi = nphi;
+            ro->I = h_img;
+            ro->Q = h_imQ;
+            ro->U.assign(nray, 0.0);
+            ro->V.assign(nray, 0.0);
+            HIPCHK(ctx, hipMemcpyAsync(ro->U.data(), imageU, sizeof(double) * nray, hipMemcpyDeviceToHost, st));
+            HIPCHK(ctx, hipMemcpyAsync(ro->V.data(), imageV, sizeof(double) * nray, hipMemcpyDeviceToHost, st));
+            HIPCHK(ctx, hipStreamSynchronize(st));
+        }
     }
+    return MCX_OK;
+}
+
+extern "C" int mcx_image(mcx_ctx *ctx, const mcx_obs *o, const double *T, const double *theta_av, double lam,
+                         const mcx_image_opts *io, double *im, double *imQ, double *imU, double *imV,
+                         int32_t *nimg, int32_t ray_cap, double *Rimg, double *rays) {
+    if (!ctx || !o || !io || !im || !nimg) return MCX_EINVAL;
+    if (io->npix < 1 || io->nint < 1 || !(io->Rmax > 0.0) || !(lam > 0.0))
+        return fail(ctx, MCX_EINVAL, "image: need npix, nint >= 1, Rmax > 0 and lam > 0");
+    if (ctx->tb.scat_how == 2 && (!imQ || !imU || !imV))
+        return fail(ctx, MCX_EINVAL, "image: full scattering needs the Q, U and V images");
+    RayImages ro;
+    double fl = 0.0, sfl = 0.0, fq = 0.0;
+    int r = sed_core(ctx, o, T, theta_av, 1, &lam, &fl, &sfl, &fq, lam, &ro);
+    if (r) return r;
+    *nimg = ro.nimg;
+    const int nray = ro.nimg * ro.nphi;
+    if (Rimg || rays) {
+        if (ray_cap < ro.nimg) return fail(ctx, MCX_EINVAL, "image: ray_cap below the number of rings (see *nimg)");
+        if (Rimg) memcpy(Rimg, ro.R.data(), sizeof(double) * ro.nimg);
+        if (rays) {
+            const std::vector<double> *src[4] = {&ro.I, &ro.Q, &ro.U, &ro.V};
+            for (int q = 0; q < 4; q++) memcpy(rays + (size_t)q * nray, src[q]->data(), sizeof(double) * nray);
+        }
+    }
+    hipStream_t st = ctx->stream;
+    std::vector<void *> tmp;
+    struct Guard {
+        std::vector<void *> &l;
+        ~Guard() { free_list(l); }
+    } guard{tmp};
+    const size_t npx = (size_t)io->npix * io->npix;
+    double *dR, *dI, *dQ, *dU, *dV, *dim;
+    if ((r = dupload(ctx, tmp, &dR, ro.R.data(), ro.R.size()))) return r;
+    if ((r = dupload(ctx, tmp, &dI, ro.I.data(), (size_t)nray))) return r;
+    if ((r = dupload(ctx, tmp, &dQ, ro.Q.data(), (size_t)nray))) return r;
+    if ((r = dupload(ctx, tmp, &dU, ro.U.data(), (size_t)nray))) return r;
+    if ((r = dupload(ctx, tmp, &dV, ro.V.data(), (size_t)nray))) return r;
+    if ((r = dalloc(ctx, tmp, &dim, 4 * npx))) return r;
+    // zeroed by copying a host array of zeros
+    std::vector<double> zero(4 * npx, 0.0);
+    HIPCHK(ctx, hipMemcpyAsync(dim, zero.data(), sizeof(double) * 4 * npx, hipMemcpyHostToDevice, st));
+    ImgArgs A;
+    A.nimg = ro.nimg;
+    A.nphi = ro.nphi;
+    A.npix = io->npix;
+    A.nint = io->nint;
+    A.scat_how = ctx->tb.scat_how;
+    A.R = dR;
+    A.I = dI;
+    A.Q = dQ;
+    A.U = dU;
+    A.V = dV;
+    A.Rmax = io->Rmax;
+    A.pa = io->pa_deg;
+    A.au = MCX_AU;
+    A.seed = io->seed;
+    A.im = dim;
+    A.imQ = dim + npx;
+    A.imU = dim + 2 * npx;
+    A.imV = dim + 3 * npx;
+    const int ncellp = (ro.nimg - 1) * ro.nphi;
+    if (ncellp > 0) {
+        hipLaunchKernelGGL(k_image, dim3((unsigned)((ncellp + 63) / 64)), dim3(64), 0, st, A);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    double *outs[4] = {im, imQ, imU, imV};
+    for (int q = 0; q < 4; q++)
+        if (outs[q])
+            HIPCHK(ctx, hipMemcpyAsync(outs[q], dim + (size_t)q * npx, sizeof(double) * npx, hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
     return MCX_OK;
 }

@@ -135,6 +135,32 @@ class Transport:
                                      ptr(out[1]), ptr(out[2])))
         return out.T.copy()
 
+    def image(self, obs, lam, npix, nint, Rmax, pa_deg=0.0, seed=1, T=None, rays=False):
+        """Raw pixel image of an IMAGE observation at wavelength lam
+        (mcx_image): array [4, npix, npix] of I, Q, U, V with pixel (ix, iy)
+        at [:, ix-1, iy-1], before the 1e23 zscale / distance^2 scaling.
+        With rays=True also returns (Rimg, ray images [4, nimg, nphi])."""
+        m = self.model
+        tav = _c(m.theta_av)
+        Tp = None
+        if T is not None:
+            Tc = _c(np.asarray(T).ravel())
+            Tp = ptr(Tc)
+        out = np.zeros((4, npix, npix))
+        opts = _lib.ImageOpts(int(npix), int(nint), float(Rmax), float(pa_deg), int(seed))
+        nimg = C.c_int32()
+        cap = 4 * (m.nr + 4 * m.nt + 400) + 10 * max(int(obs.nj0), 1) * m.nr
+        Rimg = np.zeros(cap) if rays else None
+        rbuf = np.zeros(4 * cap * int(obs.nphi)) if rays else None
+        self._check(self.lib.mcx_image(self.ctx, C.byref(obs), Tp, ptr(tav), float(lam), C.byref(opts),
+                                       ptr(out[0]), ptr(out[1]), ptr(out[2]), ptr(out[3]), C.byref(nimg),
+                                       cap if rays else 0, ptr(Rimg) if rays else None,
+                                       ptr(rbuf) if rays else None))
+        if not rays:
+            return out
+        n = nimg.value
+        return out, Rimg[:n].copy(), rbuf[:4 * n * int(obs.nphi)].reshape(4, n, int(obs.nphi)).copy()
+
     def set_temperature(self, T, dT=None, Ni=None):
         """Load a temperature state (mcx_set_temperature)."""
         Tc = _c(np.asarray(T).ravel())

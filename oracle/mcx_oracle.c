@@ -2088,9 +2088,30 @@ static int image_path(const sed_t *S, double Rad, double phi, path_t *P)
     return -1;
 }
 
+static int trace_flux_impl(const orc_model *m, const double *thet, const double *Tcell, const orc_obs *o,
+                           const double *Rimg, int nimg, double lam0, int rng, uint64_t seed, orc_ran1_state *rs,
+                           double *flux_out, double *scatflux_out, double *fluxQ_out, double *rays);
+
 int orc_trace_flux(const orc_model *m, const double *thet, const double *Tcell, const orc_obs *o,
                    const double *Rimg, int nimg, double lam0, int rng, uint64_t seed, orc_ran1_state *rs,
                    double *flux_out, double *scatflux_out, double *fluxQ_out)
+{
+    return trace_flux_impl(m, thet, Tcell, o, Rimg, nimg, lam0, rng, seed, rs, flux_out, scatflux_out, fluxQ_out,
+                           NULL);
+}
+
+/* TraceFlux's ray images I, Q, U, V (TraceMono.f:325-480) into rays
+ * [4][nimg*nphi] (ray = ring*nphi + azimuth), besides the flux sums. */
+int orc_trace_image(const orc_model *m, const double *thet, const double *Tcell, const orc_obs *o,
+                    const double *Rimg, int nimg, double lam0, uint64_t seed, double *rays)
+{
+    double f, sf, fq;
+    return trace_flux_impl(m, thet, Tcell, o, Rimg, nimg, lam0, 1, seed, NULL, &f, &sf, &fq, rays);
+}
+
+static int trace_flux_impl(const orc_model *m, const double *thet, const double *Tcell, const orc_obs *o,
+                           const double *Rimg, int nimg, double lam0, int rng, uint64_t seed, orc_ran1_state *rs,
+                           double *flux_out, double *scatflux_out, double *fluxQ_out, double *rays)
 {
     sed_t S;
     memset(&S, 0, sizeof(S));
@@ -2281,31 +2302,39 @@ int orc_trace_flux(const orc_model *m, const double *thet, const double *Tcell, 
         for (int k = 1; k <= nphi; k++) {
             double phik = 1e-5 + PI * ((double)((float)k - 0.5f)) / (double)nphi;
             if (image_path(&S, Rimg[ir], phik, &P) != 0) { err = -1; break; }
-            double img = 0.0, sim = 0.0, imQ = 0.0, tau = 0.0, fact = 1.0;
+            double img = 0.0, sim = 0.0, imQ = 0.0, imU = 0.0, imV = 0.0, tau = 0.0, fact = 1.0;
             int stopped = 0;
             for (int s = 0; s < P.n; s++) {
                 int ip = P.i[s], jp = P.jj[s], kp = P.k[s], irg = P.irg[s];
                 int c = ip * nt + jp;
-                double x_scat = 0.0, x_scatQ = 0.0;
+                double x_scat = 0.0, x_scatQ = 0.0, x_scatU = 0.0, x_scatV = 0.0;
                 if (scattering && (nmc + nmcs) != 0) {
                     int jj1 = P.jphi1[s], jj2 = P.jphi2[s];
                     int djj = (jj1 <= jj2) ? 1 : -1, njj = 0, jj = jj1;
                     for (;;) {
                         double ww = S.Alb[c] / (m->V[c] * fracirg[(size_t)jp * (S.MAXRG + 1) + irg - 1]);
                         x_scat = x_scat + ww * S.sf[sf_idx(&S, ip, jp, irg, jj, kp)];
-                        if (m->scat_how == 2) x_scatQ = x_scatQ + ww * S.sQ[sf_idx(&S, ip, jp, irg, jj, kp)];
+                        if (m->scat_how == 2) {
+                            x_scatQ = x_scatQ + ww * S.sQ[sf_idx(&S, ip, jp, irg, jj, kp)];
+                            x_scatU = x_scatU + ww * S.sU[sf_idx(&S, ip, jp, irg, jj, kp)];
+                            x_scatV = x_scatV + ww * S.sV[sf_idx(&S, ip, jp, irg, jj, kp)];
+                        }
                         njj++;
                         if (jj == jj2) break;
                         jj = jj + djj;
                     }
                     x_scat = x_scat / (double)(float)njj;
                     x_scatQ = x_scatQ / (double)(float)njj;
+                    x_scatU = x_scatU / (double)(float)njj;
+                    x_scatV = x_scatV / (double)(float)njj;
                 }
                 double tau_e = ((P.v[s] * m->dens[c]) * S.Kext[c]) * AU_C;
                 if (ip != 0) {
                     if (tau_e < 1e-6) {
                         img = img + ((2.0 * x_scat + emis[c]) * tau_e) * fact;
                         imQ = imQ + ((2.0 * x_scatQ) * tau_e) * fact;
+                        imU = imU + ((2.0 * x_scatU) * tau_e) * fact;
+                        imV = imV + ((2.0 * x_scatV) * tau_e) * fact;
                         sim = sim + ((2.0 * x_scat) * tau_e) * fact;
                         fact = fact * (1.0 - tau_e);
                     } else {
@@ -2313,6 +2342,8 @@ int orc_trace_flux(const orc_model *m, const double *thet, const double *Tcell, 
                         double frac = 1.0 - ex;
                         img = img + ((2.0 * x_scat + emis[c]) * frac) * fact;
                         imQ = imQ + ((2.0 * x_scatQ) * frac) * fact;
+                        imU = imU + ((2.0 * x_scatU) * frac) * fact;
+                        imV = imV + ((2.0 * x_scatV) * frac) * fact;
                         sim = sim + ((2.0 * x_scat) * frac) * fact;
                         fact = fact * ex;
                     }
@@ -2326,6 +2357,13 @@ int orc_trace_flux(const orc_model *m, const double *thet, const double *Tcell, 
             image[(size_t)ir * nphi + k - 1] = img;
             scatim[(size_t)ir * nphi + k - 1] = sim;
             imageQ[(size_t)ir * nphi + k - 1] = imQ;
+            if (rays) {
+                const size_t nray = (size_t)nimg * nphi, q = (size_t)ir * nphi + k - 1;
+                rays[q] = img;
+                rays[nray + q] = imQ;
+                rays[2 * nray + q] = imU;
+                rays[3 * nray + q] = imV;
+            }
         }
     /* flux quadrature (TraceMono.f:499-520), mask = 1 */
     double flux = 0.0, scatflux = 0.0, fluxQ = 0.0;
@@ -2796,4 +2834,128 @@ void orc_compare_cells(int nr, int nt, const double *T1, const double *dT1, cons
     *errT = eT / N;
     *errRho = sqrt(eR / N);
     *errE = eE / N;
+}
+
+/* number_invalid (Settling.f:652-674) */
+static int img_invalid(double a)
+{
+    double b = a * 2.0;
+    b = b / 2.0;
+    double c = a - 1e100;
+    return !(b == a && c < a);
+}
+
+/* MakeImage's spread of the ray image onto pixels (TraceMono.f:2816-3020),
+ * sequentially: rings i = 1..nimg-1, azimuths j = 1..nphi, nint samples
+ * each with the two draws 2k, 2k+1 of Philox stream (seed, (i-1)*nphi +
+ * j-1).  rays [4][nimg*nphi] = I, Q, U, V; outputs [npix*npix] at
+ * (ix-1)*npix + iy-1, accumulated (not zeroed here). */
+void orc_make_image(int nimg, int nphi, const double *Rimg, const double *rays, int scat_how, int npix, int nint,
+                    double Rmax, double pa_deg, uint64_t seed, double *im, double *imQ, double *imU, double *imV)
+{
+    const size_t nray = (size_t)nimg * nphi;
+    const double *I = rays, *Q = rays + nray, *U = rays + 2 * nray, *V = rays + 3 * nray;
+#define AT(X, ir, jp) (X)[(size_t)((ir) - 1) * nphi + ((jp) - 1)]
+    for (int i = 1; i <= nimg - 1; i++)
+        for (int j = 1; j <= nphi; j++) {
+            const uint64_t pid = (uint64_t)(i - 1) * (uint64_t)nphi + (uint64_t)(j - 1);
+            double w2 = 2.0 * PI;
+            w2 = w2 * (AU_C * AU_C);
+            w2 = w2 * (Rimg[i] - Rimg[i - 1]);
+            w2 = w2 / (double)(float)nphi;
+            for (int k = 0; k < nint; k++) {
+                double ranR = orc_philox_uniform(seed, pid, 2 * (uint64_t)k);
+                double ranPhi = orc_philox_uniform(seed, pid, 2 * (uint64_t)k + 1);
+                double R = Rimg[i - 1] + (Rimg[i] - Rimg[i - 1]) * ranR;
+                double phi = (PI * ((double)(float)(j - 1) + ranPhi)) / (double)(float)nphi;
+                double wp1, wp2;
+                int jp1, jp2;
+                if (ranPhi < 0.5) {
+                    wp1 = 0.5 - ranPhi; jp1 = j - 1; if (jp1 == 0) jp1 = 1;
+                    wp2 = ranPhi + 0.5; jp2 = j;
+                } else {
+                    wp1 = ranPhi - 0.5; jp1 = j + 1; if (jp1 == nphi + 1) jp1 = nphi;
+                    wp2 = 1.5 - ranPhi; jp2 = j;
+                }
+                double wr1 = 1.0 - ranR, wr2 = ranR;
+                int ir1 = i, ir2 = i + 1;
+                double flux, fQ = 0.0, fU = 0.0, fV = 0.0;
+#define LIN(X) ((wr1 * (wp1 * AT(X, ir1, jp1) + wp2 * AT(X, ir1, jp2)) + \
+                 wr2 * (wp1 * AT(X, ir2, jp1) + wp2 * AT(X, ir2, jp2))) * w2 * R)
+                if (scat_how == 2) {
+                    if (fabs(Rimg[ir2 - 1] - Rimg[ir1 - 1]) > (Rmax / (double)(float)npix)) {
+                        int irs[4] = {ir1, ir1, ir2, ir2}, jps[4] = {jp1, jp2, jp1, jp2};
+                        double iv[4], al[4], ar[4], cv[4], sv[4];
+                        for (int q = 0; q < 4; q++) {
+                            double Iq = AT(I, irs[q], jps[q]), Qq = AT(Q, irs[q], jps[q]);
+                            double Uq = AT(U, irs[q], jps[q]), Vq = AT(V, irs[q], jps[q]);
+                            double pq = sqrt(Qq * Qq + Uq * Uq + Vq * Vq);
+                            iv[q] = Iq - pq;
+                            al[q] = sqrt((pq + Qq) / 2.0);
+                            ar[q] = sqrt((pq - Qq) / 2.0);
+                            cv[q] = Uq / ((2.0 * al[q]) * ar[q]);
+                            sv[q] = Vq / ((2.0 * al[q]) * ar[q]);
+                        }
+                        for (int q = 0; q < 4; q++) {
+                            iv[q] = log10(iv[q]);
+                            al[q] = log10(al[q]);
+                            ar[q] = log10(ar[q]);
+                        }
+#define MIX(v) (wr1 * (wp1 * (v)[0] + wp2 * (v)[1]) + wr2 * (wp1 * (v)[2] + wp2 * (v)[3]))
+                        double ea = pow(10.0, MIX(al)), eb = pow(10.0, MIX(ar));
+                        flux = ea * ea;
+                        flux = flux + eb * eb;
+                        flux = flux + pow(10.0, MIX(iv));
+                        flux = (flux * w2) * R;
+                        fQ = ea * ea;
+                        fQ = fQ - eb * eb;
+                        fQ = (fQ * w2) * R;
+                        fU = ea;
+                        fU = fU * eb;
+                        fU = (fU * 2.0) * MIX(cv);
+                        fU = (fU * w2) * R;
+                        fV = ea;
+                        fV = fV * eb;
+                        fV = (fV * 2.0) * MIX(sv);
+                        fV = (fV * w2) * R;
+#undef MIX
+                        if (img_invalid(flux) || img_invalid(fQ) || img_invalid(fU) || img_invalid(fV)) {
+                            flux = LIN(I); fQ = LIN(Q); fU = LIN(U); fV = LIN(V);
+                        }
+                    } else {
+                        flux = LIN(I); fQ = LIN(Q); fU = LIN(U); fV = LIN(V);
+                    }
+                } else {
+                    flux = wr1 * (wp1 * log10(AT(I, ir1, jp1)) + wp2 * log10(AT(I, ir1, jp2))) +
+                           wr2 * (wp1 * log10(AT(I, ir2, jp1)) + wp2 * log10(AT(I, ir2, jp2)));
+                    flux = (pow(10.0, flux) * w2) * R;
+                }
+#undef LIN
+                double den = (double)(float)(2 * nint), fpix = (double)(float)npix;
+                double par = (pa_deg * PI) / 180.0;
+                for (int side = 0; side < 2; side++) {
+                    double ph = side ? -phi : phi;
+                    double x = -R * cos(ph + par);
+                    x = (fpix * (x + Rmax)) / (2.0 * Rmax) + 1.0;
+                    double y = R * sin(ph + par);
+                    y = (fpix * (y + Rmax)) / (2.0 * Rmax) + 1.0;
+                    int ix = (int)x, iy = (int)y;
+                    if (ix <= npix && iy <= npix && ix > 0 && iy > 0) {
+                        size_t px = (size_t)(ix - 1) * npix + (iy - 1);
+                        im[px] = im[px] + flux / den;
+                        if (scat_how == 2) {
+                            imQ[px] = imQ[px] + fQ / den;
+                            if (side) {
+                                imU[px] = imU[px] - fU / den;
+                                imV[px] = imV[px] - fV / den;
+                            } else {
+                                imU[px] = imU[px] + fU / den;
+                                imV[px] = imV[px] + fV / den;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+#undef AT
 }

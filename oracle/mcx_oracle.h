@@ -225,6 +225,13 @@ int orc_trace_ray(const orc_model *m, double x, double y, double z,
                   double vx, double vy, double vz, int i, int jj, int onEdge, int edgeNr,
                   int maxseg, int32_t *cells, double *lens, int *exit_code);
 
+/* TraceFlux's ray images I, Q, U, V (rays [4][nimg*nphi]), Philox mode */
+int orc_trace_image(const orc_model *m, const double *thet, const double *Tcell, const orc_obs *o,
+                    const double *Rimg, int nimg, double lam0, uint64_t seed, double *rays);
+/* MakeImage's spread of a ray image onto npix^2 pixels (TraceMono.f:2816-3020) */
+void orc_make_image(int nimg, int nphi, const double *Rimg, const double *rays, int scat_how, int npix, int nint,
+                    double Rmax, double pa_deg, uint64_t seed, double *im, double *imQ, double *imU, double *imV);
+
 /* ---- dust opacities from optical constants (mcx_oracle_dust.c) ---- */
 /* gauleg2 (ComputePart.f:1218-1250) */
 void orc_gauleg2(float x1, float x2, float *x, float *w, int n);

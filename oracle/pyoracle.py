@@ -249,6 +249,12 @@ def load():
     lib.orc_trace_flux.argtypes = [C.POINTER(OrcModel), _dp, _dp, C.POINTER(OrcObs), _dp, C.c_int, C.c_double,
                                    C.c_int, C.c_uint64, C.POINTER(Ran1State), _dp, _dp, _dp]
     lib.orc_trace_flux.restype = C.c_int
+    lib.orc_trace_image.argtypes = [C.POINTER(OrcModel), _dp, _dp, C.POINTER(OrcObs), _dp, C.c_int, C.c_double,
+                                    C.c_uint64, _dp]
+    lib.orc_trace_image.restype = C.c_int
+    lib.orc_make_image.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                   C.c_uint64, _dp, _dp, _dp, _dp]
+    lib.orc_make_image.restype = None
     lib.orc_arrinterpol_log.argtypes = [C.c_double, _dp, _dp, C.c_int]
     lib.orc_arrinterpol_log.restype = C.c_double
     lib.orc_diffcoeff.argtypes = [C.POINTER(OrcModel), C.c_int, C.c_int, C.c_int, _dp, _dp]
@@ -414,6 +420,28 @@ class Oracle:
             out[k] = (f.value, s.value, q.value)
         return out
 
+    def ray_image(self, Tcell, obs, lam0, seed=1):
+        """TraceFlux's ray images at one wavelength (Telescope.f:165-169:
+        paths traced at the image wavelength): returns (Rimg, rays) with
+        rays [4, nimg, nphi] = I, Q, U, V."""
+        m = self.m
+        Rimg, _ = self.image_radii(obs, lam_paths=float(lam0))
+        thet = np.ascontiguousarray(m.thet, dtype=np.float64)
+        T = np.ascontiguousarray(np.asarray(Tcell).ravel(), dtype=np.float64)
+        rays = np.zeros(4 * len(Rimg) * obs.nphi)
+        rc = self.lib.orc_trace_image(C.byref(self.om), p(thet), p(T), C.byref(obs), p(Rimg), len(Rimg),
+                                      float(lam0), int(seed), p(rays))
+        if rc != 0:
+            raise RuntimeError(f"oracle ray image failed at {lam0}: {rc}")
+        return Rimg, rays.reshape(4, len(Rimg), obs.nphi)
+
+    def make_image(self, Rimg, rays, scat_how, npix, nint, Rmax, pa_deg=0.0, seed=1):
+        """MakeImage's spread of ray images onto npix x npix pixels; returns
+        the raw images [4, npix, npix] (I, Q, U, V), pixel (ix, iy) at
+        [:, ix-1, iy-1]."""
+        lib = self.lib
+        return make_image(lib, Rimg, rays, scat_how, npix, nint, Rmax, pa_deg, seed)
+
     def trace_ray(self, x, y, z, vx, vy, vz, i, jj, onEdge=0, edgeNr=0, maxseg=100000):
         cells = np.zeros(maxseg, dtype=np.int32)
         lens = np.zeros(maxseg)
@@ -421,3 +449,15 @@ class Oracle:
         n = self.lib.orc_trace_ray(C.byref(self.om), x, y, z, vx, vy, vz, i, jj, onEdge, edgeNr, maxseg,
                                    p(cells), p(lens), C.byref(code))
         return cells[:max(n, 0)], lens[:max(n, 0)], code.value
+
+
+def make_image(lib, Rimg, rays, scat_how, npix, nint, Rmax, pa_deg=0.0, seed=1):
+    """orc_make_image on given ray images (Rimg [nimg], rays [4, nimg, nphi])."""
+    Rimg = np.ascontiguousarray(Rimg, dtype=np.float64)
+    rays = np.ascontiguousarray(rays, dtype=np.float64)
+    nimg, nphi = rays.shape[1], rays.shape[2]
+    assert rays.shape[0] == 4 and len(Rimg) == nimg
+    out = np.zeros((4, npix, npix))
+    lib.orc_make_image(nimg, nphi, p(Rimg), p(rays), int(scat_how), int(npix), int(nint), float(Rmax), float(pa_deg),
+                       int(seed), p(out[0]), p(out[1]), p(out[2]), p(out[3]))
+    return out

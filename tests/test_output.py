@@ -132,7 +132,7 @@ def test_observation_file(tiny_model, tmp_path):
     # the model grid with both ends (Telescope.f:40-71)
     assert lb[0] == m.lam[0] and lb[-1] == m.lam[-1]
     assert np.array_equal(lb[1:-1], m.lam[1:-1])
-    p.write_text("type=IMAGE\n")
+    p.write_text("type=VISIBILITY\n")
     with pytest.raises(observe.ObservationError):
         observe.parse_observation_file(str(p), m)
 
