@@ -171,6 +171,75 @@ def converge_bench(a, work):
                             "errT < epsiter twice, decides when the loop stops)"}
 
 
+def image_bench(a, work, lam0=1.6):
+    """An IMAGE observation with the reference's defaults (determineoutput,
+    Telescope.f:590-657: 500 x 500 pixels, 1000 samples per polar cell,
+    nphi 45, nrad 2, 1000 + 1000 scattering packets, the field the disk's
+    diameter) at lam0 micron, inclination the model's, on the config's
+    disk with power-law temperatures: a warmup call, then `--steps` timed
+    calls of mcx_image (rays and spread).  The CPU leg runs the oracle's
+    MakeImage on the rings of a bounded sample and scales to all rings."""
+    import numpy as np
+    from mcmax_amd import configs, transport
+    tr = transport.Transport(0)
+    m = configs.load_config(a.config, work, tr=tr)
+    tr.set_model(m)
+    T = np.empty((m.nr, m.nt))
+    T[:] = np.minimum(1500.0, 400.0 * (m.R_av / 1.496e13) ** -0.5)[:, None]
+    tr.set_temperature(T)
+    npix, nint = 500, 1000
+    Rmax = float(m.R[-1])
+    ob = transport.make_obs(angle_deg=float(m.incangle), nphi=45, nj0=2, nphot=1000, nphotstar=1000, seed=a.seed)
+    for _ in range(a.warmup):
+        tr.image(ob, lam0, npix, nint, Rmax, seed=a.seed)
+    walls, rays_ms, spread_ms = [], [], []
+    for s in range(a.steps):
+        t0 = time.perf_counter()
+        raw, Rimg, rays = tr.image(ob, lam0, npix, nint, Rmax, seed=a.seed + s, rays=True)
+        walls.append(time.perf_counter() - t0)
+        r_ms, s_ms = tr.last_image_timing()
+        rays_ms.append(r_ms)
+        spread_ms.append(s_ms)
+        print(f"[bench] image {s}: {1e3 * walls[-1]:.1f} ms (rays {r_ms:.1f} ms, spread {s_ms:.2f} ms)",
+              file=sys.stderr, flush=True)
+    tr.close()
+    nimg, nphi = rays.shape[1], rays.shape[2]
+    samples = (nimg - 1) * nphi * nint
+    # per sample: the four neighbouring rays' Stokes entries read, two
+    # deposits of (1 or 4) atomic adds; all of it L2-resident
+    nst = 4 if m.scat_how == 2 else 1
+    b_sample = 4 * nst * 8 + 2 * nst * 8
+    spread_s = sum(spread_ms) / len(spread_ms) * 1e-3
+    wall = sum(walls) / len(walls)
+    out = {"metric": "IMAGE observations per second (rays + MakeImage spread)", "value": 1.0 / wall,
+           "unit": "images/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * wall,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": f"{a.config} disk model, IMAGE at {lam0} um, 500x500 px, nint 1000, nphi 45",
+                      "nimg": nimg, "nphi": nphi, "samples": samples, "scat_how": m.scat_how, "ng": m.ng},
+           "rays_ms": sum(rays_ms) / len(rays_ms), "spread_ms": 1e3 * spread_s,
+           "spread": {"kernel": "k_image", "samples_per_s": samples / spread_s,
+                      "bytes_per_sample": b_sample, "l2_bytes_per_s": samples * b_sample / spread_s},
+           "flux_check": float(raw[0].sum())}
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        lib = pyoracle.load()
+        # a contiguous block of rings in the middle of the image
+        n = max(2, int((nimg - 1) * 0.02))
+        i0 = (nimg - 1) // 2
+        t0 = time.perf_counter()
+        pyoracle.make_image(lib, Rimg[i0:i0 + n + 1], rays[:, i0:i0 + n + 1], m.scat_how, npix, nint, Rmax, 0.0,
+                            a.seed)
+        dt = time.perf_counter() - t0
+        smp = n * nphi * nint
+        out["cpu_baseline"] = {"value": smp / dt, "unit": "samples/s (spread)", "cores": 1, "kind": "port",
+                               "sample": f"MakeImage restated in C on {n} of {nimg - 1} rings ({smp} samples) in "
+                                         f"{dt:.1f} s on one host core; all rings would take "
+                                         f"{dt * (nimg - 1) / n:.0f} s against the device's {1e3 * spread_s:.1f} ms"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +256,8 @@ def main():
                     help="time the structure iteration (passes until the T-field is converged) instead")
     ap.add_argument("--dust", action="store_true",
                     help="measure the dust-opacity generator on the model's computepart particle instead")
+    ap.add_argument("--image", action="store_true",
+                    help="measure an IMAGE observation (ray images and MakeImage's spread) instead")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many bytes (rocprof counter calibration)")
     a = ap.parse_args()
@@ -212,6 +283,9 @@ def main():
         return
     if a.converge:
         print(json.dumps(converge_bench(a, work)))
+        return
+    if a.image:
+        print(json.dumps(image_bench(a, work)))
         return
     tr = transport.Transport(0)
     m = configs.load_config(a.config, work, tr=tr, randomwalk=not a.plain)

@@ -301,6 +301,12 @@ int mcx_image(mcx_ctx *ctx, const mcx_obs *obs, const double *T, const double *t
               const mcx_image_opts *opts, double *im, double *imQ, double *imU, double *imV, int32_t *nimg,
               int32_t ray_cap, double *Rimg, double *rays);
 
+/* Durations of the last mcx_image: the ray images (rays_ms, host clock
+ * around the rays' part, uploads and kernels included) and the spread onto
+ * pixels (spread_ms, HIP events around that kernel on the context's
+ * stream).  No counterpart in the reference (it prints no timings). */
+int mcx_last_image_timing(const mcx_ctx *ctx, double *rays_ms, double *spread_ms);
+
 /* ---- dust opacities from optical constants ---------------------------- */
 
 /* One dust species for ComputePart (ComputePart.f:1-552, the particle that

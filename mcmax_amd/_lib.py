@@ -114,7 +114,7 @@ EXPORTS = (
     "mcx_create", "mcx_destroy", "mcx_last_error", "mcx_set_tables", "mcx_set_model",
     "mcx_transport", "mcx_get_tallies", "mcx_temperature", "mcx_last_timing", "mcx_sed",
     "mcx_set_temperature", "mcx_diffuse", "mcx_temp_average", "mcx_disk_structure", "mcx_compare_cells",
-    "mcx_compute_parts", "mcx_image",
+    "mcx_compute_parts", "mcx_image", "mcx_last_image_timing",
 )
 
 _lib = None
@@ -151,11 +151,13 @@ def load():
     lib.mcx_compare_cells.argtypes = [vp, C.POINTER(CellState), C.POINTER(CellState), _dp, _dp, _dp]
     lib.mcx_compute_parts.argtypes = [vp, C.c_int32, _dp, C.c_int32, C.POINTER(PartSpec), C.POINTER(PartOut),
                                       C.POINTER(PartStats)]
+    lib.mcx_last_image_timing.argtypes = [vp, _dp, _dp]
     lib.mcx_image.argtypes = [vp, C.POINTER(Obs), _dp, _dp, C.c_double, C.POINTER(ImageOpts), _dp, _dp, _dp, _dp,
                               C.POINTER(C.c_int32), C.c_int32, _dp, _dp]
     for n in ("mcx_set_tables", "mcx_set_model", "mcx_transport", "mcx_get_tallies",
               "mcx_temperature", "mcx_last_timing", "mcx_sed", "mcx_set_temperature", "mcx_diffuse",
-              "mcx_temp_average", "mcx_disk_structure", "mcx_compare_cells", "mcx_compute_parts", "mcx_image"):
+              "mcx_temp_average", "mcx_disk_structure", "mcx_compare_cells", "mcx_compute_parts", "mcx_image",
+              "mcx_last_image_timing"):
         getattr(lib, n).restype = C.c_int
     _lib = lib
     return lib

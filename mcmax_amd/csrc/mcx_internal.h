@@ -131,6 +131,7 @@ struct mcx_ctx {
     unsigned long long *d_next = nullptr;
     double unitE = 0.0, E0 = 0.0;
     double kernel_ms = 0.0, pass_ms = 0.0;
+    double image_rays_ms = 0.0, image_spread_ms = 0.0;  // last mcx_image: ray part (host clock), k_image (events)
     int sm_count = 0;
     int wclk_khz = 0; // rate of the device's constant wall clock (wall_clock64)
     size_t lds_max = 65536; // LDS bytes one workgroup may allocate

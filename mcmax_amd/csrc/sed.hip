@@ -15,6 +15,8 @@
 //    180 observer directions of every scattered-field deposit between them;
 //  * k_rays: the formal solution along every image ray (one thread per ray).
 // The flux quadrature over the image rings runs on the host.
+#include <chrono>
+
 #include "mcx_internal.h"
 
 #define NPHIS 180 // observer directions of the scattered field (TraceMono.f:1309)
@@ -949,18 +951,30 @@ __global__ void k_rays(RayArgs A) {
 
 // MakeImage (TraceMono.f:2759-3020): the ray-traced image on its rings
 // (nimg radii, nphi azimuths over half a turn) spread over a square of
-// npix^2 pixels of half-width Rmax (AU).  One thread per polar cell (ring
-// i, azimuth j, 1-based as the reference); nint samples at random radius
-// and azimuth, each deposited at phi and at -phi (U, V change sign there).
+// npix^2 pixels of half-width Rmax (AU).  Polar cell (ring i, azimuth j,
+// 1-based as the reference) has nint samples at random radius and azimuth,
+// each deposited at phi and at -phi (U, V change sign there).
 // Full scattering interpolates I-|P|, and the two linear-polarisation
 // amplitudes, in log between the four neighbouring rays where the ring
 // spacing exceeds a pixel, falling back to linear interpolation where that
-// yields a non-finite value; otherwise log-linear I.  Draws: Philox stream
-// (seed, (i-1)*nphi + j-1), two per sample.
+// yields a non-finite value; otherwise log-linear I.  Draws: sample k of
+// cell t = (i-1)*nphi + j-1 takes Philox block k of stream (seed, t), its
+// two doubles being the radius and azimuth draws.
+//
+// One thread per (cell, chunk of clen samples); lanes of a wave are
+// consecutive cells of one chunk.  Most rings lie inside a few central
+// pixels, so a thread sums its consecutive deposits that land on the same
+// pixel (one accumulator per side) and adds them to the image when the
+// pixel changes; the last sums of a wave are added once per wave when all
+// its lanes hold the same pixel.
 struct ImgArgs {
     int nimg, nphi, npix, nint, scat_how;
+    int nchunk, clen;               // sample chunks per cell, samples per chunk
     const double *R;                // [nimg] ring radii (AU)
     const double *I, *Q, *U, *V;    // [nimg*nphi] ray images, ray = ring*nphi + azimuth
+    // per ray, from k_image_rays: log10(I - |P|) (log10 I without full
+    // scattering), log10 of the amplitudes al, ar, and U, V / (2 al ar)
+    const double *liv, *lal, *lar, *cv, *sv;
     double Rmax, pa, au;            // half field (AU), position angle (deg), AU in cm
     unsigned long long seed;
     double *im, *imQ, *imU, *imV;   // [npix*npix], pixel (ix, iy) at (ix-1)*npix + iy-1
@@ -972,21 +986,103 @@ __device__ __forceinline__ bool img_invalid(double a) {
     return !(b == a && c < a);
 }
 
+struct ImgAcc {
+    int px;           // pixel of the sums, -1 for none
+    double f, q, u, v;
+};
+
+__device__ __forceinline__ void img_add(const ImgArgs &A, const ImgAcc &c) {
+    atomicAdd(&A.im[c.px], c.f);
+    if (A.scat_how == 2) {
+        atomicAdd(&A.imQ[c.px], c.q);
+        atomicAdd(&A.imU[c.px], c.u);
+        atomicAdd(&A.imV[c.px], c.v);
+    }
+}
+
+// deposit (f, q, u, v) at pixel px (-1: outside the field) into c
+__device__ __forceinline__ void img_deposit(const ImgArgs &A, ImgAcc &c, int px, double f, double q, double u,
+                                            double v) {
+    if (px != c.px) {
+        if (c.px >= 0) img_add(A, c);
+        c.px = px;
+        c.f = c.q = c.u = c.v = 0.0;
+    }
+    if (px >= 0) {
+        c.f += f;
+        c.q += q;
+        c.u += u;
+        c.v += v;
+    }
+}
+
+__device__ __forceinline__ double img_wave_sum(double x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+// the last sums of every lane; called by the whole (converged) wave
+__device__ __forceinline__ void img_flush_wave(const ImgArgs &A, ImgAcc c) {
+    const int lead = __builtin_amdgcn_readfirstlane(c.px);
+    if (__all(c.px == lead)) {
+        if (lead < 0) return;
+        c.f = img_wave_sum(c.f);
+        if (A.scat_how == 2) {
+            c.q = img_wave_sum(c.q);
+            c.u = img_wave_sum(c.u);
+            c.v = img_wave_sum(c.v);
+        }
+        if (threadIdx.x == 0) img_add(A, c);
+    } else if (c.px >= 0) {
+        img_add(A, c);
+    }
+}
+
+// The per-ray factors of MakeImage's interpolation (TraceMono.f:2884-2927),
+// once per ray instead of once per sample: with full scattering
+// p = sqrt(Q^2 + U^2 + V^2), log10(I - p), log10 sqrt((p + Q)/2),
+// log10 sqrt((p - Q)/2) and U, V over 2 al ar; otherwise log10 I.
+__global__ __launch_bounds__(256) void k_image_rays(int nray, int scat_how, const double *I, const double *Q,
+                                                    const double *U, const double *V, double *liv, double *lal,
+                                                    double *lar, double *cv, double *sv) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nray) return;
+    if (scat_how != 2) {
+        liv[q] = log10(I[q]);
+        return;
+    }
+    const double Iq = I[q], Qq = Q[q], Uq = U[q], Vq = V[q];
+    const double pq = sqrt(Qq * Qq + Uq * Uq + Vq * Vq);
+    const double al = sqrt((pq + Qq) / 2.0), ar = sqrt((pq - Qq) / 2.0);
+    liv[q] = log10(Iq - pq);
+    lal[q] = log10(al);
+    lar[q] = log10(ar);
+    cv[q] = Uq / ((2.0 * al) * ar);
+    sv[q] = Vq / ((2.0 * al) * ar);
+}
+
 __global__ __launch_bounds__(64) void k_image(ImgArgs A) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (A.nimg - 1) * A.nphi) return;
-    const int i = t / A.nphi + 1, j = t % A.nphi + 1;
     const int np = A.nphi;
-    auto at = [&](const double *X, int ir, int jp) { return X[(size_t)(ir - 1) * np + (jp - 1)]; };
+    const long long ncell = (long long)(A.nimg - 1) * np;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long cell = t % ncell, chunk = t / ncell;
+    int kbeg = 0, kend = 0;   // no samples for threads past the last chunk
+    if (chunk < A.nchunk) {
+        kbeg = (int)chunk * A.clen;
+        kend = min(A.nint, kbeg + A.clen);
+    }
+    const int i = (int)(cell / np) + 1, j = (int)(cell % np) + 1;
     const double Ri = A.R[i - 1], Ri1 = A.R[i];
     const double w2 = ((((2.0 * MCX_PI) * (A.au * A.au)) * (Ri1 - Ri)) / (double)(float)np);
     const double den = (double)(float)(2 * A.nint);
     const double fpix = (double)(float)A.npix;
     const double par = (A.pa * MCX_PI) / 180.0;
-    PhiloxStream g;
-    g.init(A.seed, (unsigned long long)t);
-    for (int k = 0; k < A.nint; k++) {
-        const double ranR = g.next(), ranPhi = g.next();
+    const uint32_t key0 = (uint32_t)A.seed, key1 = (uint32_t)(A.seed >> 32);
+    ImgAcc acc[2] = {{-1, 0.0, 0.0, 0.0, 0.0}, {-1, 0.0, 0.0, 0.0, 0.0}};
+    for (int k = kbeg; k < kend; k++) {
+        double ranR, ranPhi;
+        philox_block(key0, key1, (uint64_t)k, (uint64_t)cell, ranR, ranPhi);
         const double R = Ri + (Ri1 - Ri) * ranR;
         const double phi = (MCX_PI * ((double)(float)(j - 1) + ranPhi)) / (double)(float)np;
         double wp1, wp2;
@@ -1006,84 +1102,64 @@ __global__ __launch_bounds__(64) void k_image(ImgArgs A) {
         }
         const double wr1 = 1.0 - ranR, wr2 = ranR;
         const int ir1 = i, ir2 = i + 1;
-        auto lin = [&](const double *X) {
-            return (wr1 * (wp1 * at(X, ir1, jp1) + wp2 * at(X, ir1, jp2)) +
-                    wr2 * (wp1 * at(X, ir2, jp1) + wp2 * at(X, ir2, jp2))) * w2 * R;
-        };
         double flux, fQ = 0.0, fU = 0.0, fV = 0.0;
+        const size_t r11 = (size_t)(ir1 - 1) * np + (jp1 - 1), r12 = (size_t)(ir1 - 1) * np + (jp2 - 1);
+        const size_t r21 = (size_t)(ir2 - 1) * np + (jp1 - 1), r22 = (size_t)(ir2 - 1) * np + (jp2 - 1);
+        auto mix = [&](const double *X) {
+            return wr1 * (wp1 * X[r11] + wp2 * X[r12]) + wr2 * (wp1 * X[r21] + wp2 * X[r22]);
+        };
         if (A.scat_how == 2) {
             if (fabs(A.R[ir2 - 1] - A.R[ir1 - 1]) > (A.Rmax / fpix)) {
-                double iv[4], al[4], ar[4], cv[4], sv[4];
-                const int irs[4] = {ir1, ir1, ir2, ir2}, jps[4] = {jp1, jp2, jp1, jp2};
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const double Iq = at(A.I, irs[q], jps[q]), Qq = at(A.Q, irs[q], jps[q]);
-                    const double Uq = at(A.U, irs[q], jps[q]), Vq = at(A.V, irs[q], jps[q]);
-                    const double pq = sqrt(Qq * Qq + Uq * Uq + Vq * Vq);
-                    iv[q] = Iq - pq;
-                    al[q] = sqrt((pq + Qq) / 2.0);
-                    ar[q] = sqrt((pq - Qq) / 2.0);
-                    cv[q] = Uq / ((2.0 * al[q]) * ar[q]);
-                    sv[q] = Vq / ((2.0 * al[q]) * ar[q]);
-                }
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    iv[q] = log10(iv[q]);
-                    al[q] = log10(al[q]);
-                    ar[q] = log10(ar[q]);
-                }
-                auto mix = [&](const double *v) { return wr1 * (wp1 * v[0] + wp2 * v[1]) + wr2 * (wp1 * v[2] + wp2 * v[3]); };
-                const double ea = pow(10.0, mix(al)), eb = pow(10.0, mix(ar));
+                const double ea = pow(10.0, mix(A.lal)), eb = pow(10.0, mix(A.lar));
                 flux = ea * ea;
                 flux = flux + eb * eb;
-                flux = flux + pow(10.0, mix(iv));
+                flux = flux + pow(10.0, mix(A.liv));
                 flux = (flux * w2) * R;
                 fQ = ea * ea;
                 fQ = fQ - eb * eb;
                 fQ = (fQ * w2) * R;
                 fU = ea;
                 fU = fU * eb;
-                fU = (fU * 2.0) * mix(cv);
+                fU = (fU * 2.0) * mix(A.cv);
                 fU = (fU * w2) * R;
                 fV = ea;
                 fV = fV * eb;
-                fV = (fV * 2.0) * mix(sv);
+                fV = (fV * 2.0) * mix(A.sv);
                 fV = (fV * w2) * R;
                 if (img_invalid(flux) || img_invalid(fQ) || img_invalid(fU) || img_invalid(fV)) {
-                    flux = lin(A.I);
-                    fQ = lin(A.Q);
-                    fU = lin(A.U);
-                    fV = lin(A.V);
+                    flux = mix(A.I) * w2 * R;
+                    fQ = mix(A.Q) * w2 * R;
+                    fU = mix(A.U) * w2 * R;
+                    fV = mix(A.V) * w2 * R;
                 }
             } else {
-                flux = lin(A.I);
-                fQ = lin(A.Q);
-                fU = lin(A.U);
-                fV = lin(A.V);
+                flux = mix(A.I) * w2 * R;
+                fQ = mix(A.Q) * w2 * R;
+                fU = mix(A.U) * w2 * R;
+                fV = mix(A.V) * w2 * R;
             }
         } else {
-            flux = wr1 * (wp1 * log10(at(A.I, ir1, jp1)) + wp2 * log10(at(A.I, ir1, jp2))) +
-                   wr2 * (wp1 * log10(at(A.I, ir2, jp1)) + wp2 * log10(at(A.I, ir2, jp2)));
+            flux = mix(A.liv);
             flux = (pow(10.0, flux) * w2) * R;
         }
         for (int side = 0; side < 2; side++) {
             const double ph = side ? -phi : phi;
-            double x = -R * cos(ph + par);
+            double sn, cs;
+            sincos(ph + par, &sn, &cs);
+            double x = -R * cs;
             x = (fpix * (x + A.Rmax)) / (2.0 * A.Rmax) + 1.0;
-            double y = R * sin(ph + par);
+            double y = R * sn;
             y = (fpix * (y + A.Rmax)) / (2.0 * A.Rmax) + 1.0;
             const int ix = (int)x, iy = (int)y;
-            if (ix <= A.npix && iy <= A.npix && ix > 0 && iy > 0) {
-                const size_t px = (size_t)(ix - 1) * A.npix + (iy - 1);
-                atomicAdd(&A.im[px], flux / den);
-                if (A.scat_how == 2) {
-                    atomicAdd(&A.imQ[px], fQ / den);
-                    atomicAdd(&A.imU[px], side ? -(fU / den) : fU / den);
-                    atomicAdd(&A.imV[px], side ? -(fV / den) : fV / den);
-                }
-            }
+            const int px = (ix <= A.npix && iy <= A.npix && ix > 0 && iy > 0) ? (ix - 1) * A.npix + (iy - 1) : -1;
+            if (side)
+                img_deposit(A, acc[1], px, flux / den, fQ / den, -(fU / den), -(fV / den));
+            else
+                img_deposit(A, acc[0], px, flux / den, fQ / den, fU / den, fV / den);
         }
     }
+    img_flush_wave(A, acc[0]);
+    img_flush_wave(A, acc[1]);
 }
 
 // ===========================================================================
@@ -1565,8 +1641,11 @@ extern "C" int mcx_image(mcx_ctx *ctx, const mcx_obs *o, const double *T, const 
         return fail(ctx, MCX_EINVAL, "image: full scattering needs the Q, U and V images");
     RayImages ro;
     double fl = 0.0, sfl = 0.0, fq = 0.0;
+    const auto t0 = std::chrono::steady_clock::now();
     int r = sed_core(ctx, o, T, theta_av, 1, &lam, &fl, &sfl, &fq, lam, &ro);
     if (r) return r;
+    ctx->image_rays_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ctx->image_spread_ms = 0.0;
     *nimg = ro.nimg;
     const int nray = ro.nimg * ro.nphi;
     if (Rimg || rays) {
@@ -1591,6 +1670,14 @@ extern "C" int mcx_image(mcx_ctx *ctx, const mcx_obs *o, const double *T, const 
     if ((r = dupload(ctx, tmp, &dU, ro.U.data(), (size_t)nray))) return r;
     if ((r = dupload(ctx, tmp, &dV, ro.V.data(), (size_t)nray))) return r;
     if ((r = dalloc(ctx, tmp, &dim, 4 * npx))) return r;
+    double *dfac;
+    if ((r = dalloc(ctx, tmp, &dfac, 5 * (size_t)std::max(nray, 1)))) return r;
+    if (nray > 0) {
+        hipLaunchKernelGGL(k_image_rays, dim3((unsigned)((nray + 255) / 256)), dim3(256), 0, st, nray,
+                           ctx->tb.scat_how, dI, dQ, dU, dV, dfac, dfac + nray, dfac + 2 * (size_t)nray,
+                           dfac + 3 * (size_t)nray, dfac + 4 * (size_t)nray);
+        HIPCHK(ctx, hipGetLastError());
+    }
     // zeroed by copying a host array of zeros
     std::vector<double> zero(4 * npx, 0.0);
     HIPCHK(ctx, hipMemcpyAsync(dim, zero.data(), sizeof(double) * 4 * npx, hipMemcpyHostToDevice, st));
@@ -1605,6 +1692,11 @@ extern "C" int mcx_image(mcx_ctx *ctx, const mcx_obs *o, const double *T, const 
     A.Q = dQ;
     A.U = dU;
     A.V = dV;
+    A.liv = dfac;
+    A.lal = dfac + nray;
+    A.lar = dfac + 2 * (size_t)nray;
+    A.cv = dfac + 3 * (size_t)nray;
+    A.sv = dfac + 4 * (size_t)nray;
     A.Rmax = io->Rmax;
     A.pa = io->pa_deg;
     A.au = MCX_AU;
@@ -1613,15 +1705,39 @@ extern "C" int mcx_image(mcx_ctx *ctx, const mcx_obs *o, const double *T, const 
     A.imQ = dim + npx;
     A.imU = dim + 2 * npx;
     A.imV = dim + 3 * npx;
-    const int ncellp = (ro.nimg - 1) * ro.nphi;
+    const long long ncellp = (long long)(ro.nimg - 1) * ro.nphi;
+    // chunks of samples per cell: about 8192 waves over the whole image
+    const long long want = 8192LL * 64;
+    long long nchunk = ncellp > 0 ? std::min<long long>(io->nint, std::max<long long>(1, (want + ncellp - 1) / ncellp))
+                                  : 1;
+    const int clen = (int)((io->nint + nchunk - 1) / nchunk);
+    nchunk = (io->nint + clen - 1) / clen;
+    A.nchunk = (int)nchunk;
+    A.clen = clen;
+    const long long nthreads = ncellp * nchunk;
+    if (ncellp > 0 && (nthreads + 63) / 64 > 0x7fffffffLL)
+        return fail(ctx, MCX_EINVAL, "image: too many ring cells and samples for one launch");
     if (ncellp > 0) {
-        hipLaunchKernelGGL(k_image, dim3((unsigned)((ncellp + 63) / 64)), dim3(64), 0, st, A);
+        HIPCHK(ctx, hipEventRecord(ctx->ev[0], st));
+        hipLaunchKernelGGL(k_image, dim3((unsigned)((nthreads + 63) / 64)), dim3(64), 0, st, A);
         HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventRecord(ctx->ev[1], st));
+        HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        ctx->image_spread_ms = ms;
     }
     double *outs[4] = {im, imQ, imU, imV};
     for (int q = 0; q < 4; q++)
         if (outs[q])
             HIPCHK(ctx, hipMemcpyAsync(outs[q], dim + (size_t)q * npx, sizeof(double) * npx, hipMemcpyDeviceToHost, st));
     HIPCHK(ctx, hipStreamSynchronize(st));
+    return MCX_OK;
+}
+
+extern "C" int mcx_last_image_timing(const mcx_ctx *ctx, double *rays_ms, double *spread_ms) {
+    if (!ctx) return MCX_EINVAL;
+    if (rays_ms) *rays_ms = ctx->image_rays_ms;
+    if (spread_ms) *spread_ms = ctx->image_spread_ms;
     return MCX_OK;
 }

@@ -161,6 +161,12 @@ class Transport:
         n = nimg.value
         return out, Rimg[:n].copy(), rbuf[:4 * n * int(obs.nphi)].reshape(4, n, int(obs.nphi)).copy()
 
+    def last_image_timing(self):
+        """(rays_ms, spread_ms) of the last image() (mcx_last_image_timing)."""
+        a, b = C.c_double(), C.c_double()
+        self._check(self.lib.mcx_last_image_timing(self.ctx, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def set_temperature(self, T, dT=None, Ni=None):
         """Load a temperature state (mcx_set_temperature)."""
         Tc = _c(np.asarray(T).ravel())
