@@ -400,18 +400,39 @@ __global__ void k_mix(int nclass, int nlam, int ng, SedMono M, const double *__r
 }
 
 // the emission cumulative over cells in the reference's order (i outer, j
-// inner, TraceMono.f:1206-1212), and the totals Etot, Etot2 (one thread)
-__global__ void k_scan(int ncell, int nt, const double *__restrict__ Eem, const double *__restrict__ EmisDis,
-                       double *cum, double *tot) {
+// inner, TraceMono.f:1206-1212), and the totals Etot, Etot2.  The sums stay
+// sequential (one thread adds, in cell order, so the cumulative is the
+// reference's to the last bit); the block stages tiles of the inputs and
+// the results through LDS so the adding thread never waits on HBM.
+#define SCAN_TILE 1024
+__global__ __launch_bounds__(256) void k_scan(int ncell, int nt, const double *__restrict__ Eem,
+                                              const double *__restrict__ EmisDis, double *__restrict__ cum,
+                                              double *__restrict__ tot) {
+    __shared__ double se[SCAN_TILE], sd[SCAN_TILE], sc[SCAN_TILE];
     double Et = 0.0, E1 = 0.0, E2 = 0.0;
-    for (int c = nt; c < ncell; c++) {
-        E1 = E1 + Eem[c];
-        E2 = E2 + EmisDis[c];
-        Et = Et + EmisDis[c];
-        cum[c] = Et;
+    for (int base = nt; base < ncell; base += SCAN_TILE) {
+        const int n = min(SCAN_TILE, ncell - base);
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            se[k] = Eem[base + k];
+            sd[k] = EmisDis[base + k];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < n; k++) {
+                E1 = E1 + se[k];
+                E2 = E2 + sd[k];
+                Et = Et + sd[k];
+                sc[k] = Et;
+            }
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < n; k += blockDim.x) cum[base + k] = sc[k];
+        __syncthreads();
     }
-    tot[0] = E1;
-    tot[1] = E2;
+    if (threadIdx.x == 0) {
+        tot[0] = E1;
+        tot[1] = E2;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1525,7 +1546,7 @@ static int sed_core(mcx_ctx *ctx, const mcx_obs *o, const double *T, const doubl
             const unsigned mb = (unsigned)((nclass * 6 * 180 + 255) / 256);
             hipLaunchKernelGGL(k_mix, dim3(mb), dim3(256), 0, st, nclass, nlam, ng, M, ctx->d_wcls, ctx->tb.F,
                                ctx->d_Ksca, ctx->tb.IF11, ctx->tb.IF12, nel, Fmix, MIF11, MIF12);
-            hipLaunchKernelGGL(k_scan, dim3(1), dim3(1), 0, st, ncell, nt, Eem, EmisDis, cum, tot);
+            hipLaunchKernelGGL(k_scan, dim3(1), dim3(256), 0, st, ncell, nt, Eem, EmisDis, cum, tot);
             HIPCHK(ctx, hipGetLastError());
             McArgs A;
             A.G = G;
