@@ -982,15 +982,14 @@ __global__ void k_rays(RayArgs A) {
 // cell t = (i-1)*nphi + j-1 takes Philox block k of stream (seed, t), its
 // two doubles being the radius and azimuth draws.
 //
-// One thread per (cell, chunk of clen samples); lanes of a wave are
-// consecutive cells of one chunk.  Most rings lie inside a few central
-// pixels, so a thread sums its consecutive deposits that land on the same
-// pixel (one accumulator per side) and adds them to the image when the
-// pixel changes; the last sums of a wave are added once per wave when all
-// its lanes hold the same pixel.
+// One block (one wave) per polar cell, lane l taking samples l, l+64, ...
+// Most rings lie inside a few central pixels, and an outer cell's samples
+// spread over tens: a lane sums its consecutive deposits on one pixel, and
+// adds each run to an LDS tile over the pixels the cell can reach on that
+// side (the +phi and the -phi half), which goes to the image once at the
+// end; cells whose box exceeds the tile add their runs to the image.
 struct ImgArgs {
     int nimg, nphi, npix, nint, scat_how;
-    int nchunk, clen;               // sample chunks per cell, samples per chunk
     const double *R;                // [nimg] ring radii (AU)
     const double *I, *Q, *U, *V;    // [nimg*nphi] ray images, ray = ring*nphi + azimuth
     // per ray, from k_image_rays: log10(I - |P|) (log10 I without full
@@ -1021,45 +1020,6 @@ __device__ __forceinline__ void img_add(const ImgArgs &A, const ImgAcc &c) {
     }
 }
 
-// deposit (f, q, u, v) at pixel px (-1: outside the field) into c
-__device__ __forceinline__ void img_deposit(const ImgArgs &A, ImgAcc &c, int px, double f, double q, double u,
-                                            double v) {
-    if (px != c.px) {
-        if (c.px >= 0) img_add(A, c);
-        c.px = px;
-        c.f = c.q = c.u = c.v = 0.0;
-    }
-    if (px >= 0) {
-        c.f += f;
-        c.q += q;
-        c.u += u;
-        c.v += v;
-    }
-}
-
-__device__ __forceinline__ double img_wave_sum(double x) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
-    return x;
-}
-
-// the last sums of every lane; called by the whole (converged) wave
-__device__ __forceinline__ void img_flush_wave(const ImgArgs &A, ImgAcc c) {
-    const int lead = __builtin_amdgcn_readfirstlane(c.px);
-    if (__all(c.px == lead)) {
-        if (lead < 0) return;
-        c.f = img_wave_sum(c.f);
-        if (A.scat_how == 2) {
-            c.q = img_wave_sum(c.q);
-            c.u = img_wave_sum(c.u);
-            c.v = img_wave_sum(c.v);
-        }
-        if (threadIdx.x == 0) img_add(A, c);
-    } else if (c.px >= 0) {
-        img_add(A, c);
-    }
-}
-
 // The per-ray factors of MakeImage's interpolation (TraceMono.f:2884-2927),
 // once per ray instead of once per sample: with full scattering
 // p = sqrt(Q^2 + U^2 + V^2), log10(I - p), log10 sqrt((p + Q)/2),
@@ -1083,16 +1043,76 @@ __global__ __launch_bounds__(256) void k_image_rays(int nray, int scat_how, cons
     sv[q] = Vq / ((2.0 * al) * ar);
 }
 
-__global__ __launch_bounds__(64) void k_image(ImgArgs A) {
-    const int np = A.nphi;
-    const long long ncell = (long long)(A.nimg - 1) * np;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long cell = t % ncell, chunk = t / ncell;
-    int kbeg = 0, kend = 0;   // no samples for threads past the last chunk
-    if (chunk < A.nchunk) {
-        kbeg = (int)chunk * A.clen;
-        kend = min(A.nint, kbeg + A.clen);
+#define IMG_TILE 256   // pixels of a side's LDS tile
+
+// a side's pixel box: the pixels that a cell's samples at angles a0..a1
+// (radians, after the position angle) and radii Ri..Ri1 can reach, one
+// pixel of margin each way, clipped to the image
+struct ImgBox {
+    int x0, y0, nx, ny;
+    bool ok;   // fits the tile
+};
+
+__device__ __forceinline__ ImgBox img_box(const ImgArgs &A, double Ri, double Ri1, double a0, double a1,
+                                          double fpix) {
+    double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+    auto put = [&](double R, double a) {
+        const double x = -R * cos(a), y = R * sin(a);
+        xmin = fmin(xmin, x);
+        xmax = fmax(xmax, x);
+        ymin = fmin(ymin, y);
+        ymax = fmax(ymax, y);
+    };
+    put(Ri, a0);
+    put(Ri, a1);
+    put(Ri1, a0);
+    put(Ri1, a1);
+    const double q = 0.5 * MCX_PI;
+    for (double m = floor(a0 / q) + 1.0; m * q < a1; m += 1.0) {   // the axes the sector crosses
+        put(Ri, m * q);
+        put(Ri1, m * q);
     }
+    auto pix = [&](double v) {   // pixel index of a coordinate, kept within [-1, npix + 2]
+        const double t = (fpix * (v + A.Rmax)) / (2.0 * A.Rmax) + 1.0;
+        return (int)floor(fmin(fmax(t, -1.0), fpix + 2.0));
+    };
+    ImgBox B;
+    const int xl = max(1, pix(xmin) - 1), xh = min(A.npix, pix(xmax) + 1);
+    const int yl = max(1, pix(ymin) - 1), yh = min(A.npix, pix(ymax) + 1);
+    B.x0 = xl;
+    B.y0 = yl;
+    B.nx = max(0, xh - xl + 1);
+    B.ny = max(0, yh - yl + 1);
+    B.ok = B.nx * B.ny <= IMG_TILE;
+    return B;
+}
+
+// a run's sums into the side's LDS tile when the pixel lies in its box,
+// else straight into the image
+__device__ __forceinline__ void img_put(const ImgArgs &A, double (*tl)[IMG_TILE], const ImgBox &B,
+                                        const ImgAcc &c) {
+    if (c.px < 0) return;
+    const int ix = c.px / A.npix + 1, iy = c.px % A.npix + 1;
+    const int dx = ix - B.x0, dy = iy - B.y0;
+    if (B.ok && dx >= 0 && dx < B.nx && dy >= 0 && dy < B.ny) {
+        const int e = dx * B.ny + dy;
+        atomicAdd(&tl[0][e], c.f);
+        if (A.scat_how == 2) {
+            atomicAdd(&tl[1][e], c.q);
+            atomicAdd(&tl[2][e], c.u);
+            atomicAdd(&tl[3][e], c.v);
+        }
+    } else {
+        img_add(A, c);
+    }
+}
+
+// one block (one wave) per polar cell; lane l takes samples l, l+64, ...
+__global__ __launch_bounds__(64, 3) void k_image(ImgArgs A) {
+    __shared__ double tile[2][4][IMG_TILE];
+    const int np = A.nphi;
+    const long long cell = blockIdx.x;
+    const int lane = threadIdx.x;
     const int i = (int)(cell / np) + 1, j = (int)(cell % np) + 1;
     const double Ri = A.R[i - 1], Ri1 = A.R[i];
     const double w2 = ((((2.0 * MCX_PI) * (A.au * A.au)) * (Ri1 - Ri)) / (double)(float)np);
@@ -1100,8 +1120,13 @@ __global__ __launch_bounds__(64) void k_image(ImgArgs A) {
     const double fpix = (double)(float)A.npix;
     const double par = (A.pa * MCX_PI) / 180.0;
     const uint32_t key0 = (uint32_t)A.seed, key1 = (uint32_t)(A.seed >> 32);
+    const double th0 = (MCX_PI * (double)(j - 1)) / (double)np, th1 = (MCX_PI * (double)j) / (double)np;
+    const ImgBox box[2] = {img_box(A, Ri, Ri1, th0 + par, th1 + par, fpix),
+                           img_box(A, Ri, Ri1, -th1 + par, -th0 + par, fpix)};
+    for (int e = lane; e < 2 * 4 * IMG_TILE; e += 64) (&tile[0][0][0])[e] = 0.0;
+    __syncthreads();
     ImgAcc acc[2] = {{-1, 0.0, 0.0, 0.0, 0.0}, {-1, 0.0, 0.0, 0.0, 0.0}};
-    for (int k = kbeg; k < kend; k++) {
+    for (int k = lane; k < A.nint; k += 64) {
         double ranR, ranPhi;
         philox_block(key0, key1, (uint64_t)k, (uint64_t)cell, ranR, ranPhi);
         const double R = Ri + (Ri1 - Ri) * ranR;
@@ -1173,14 +1198,38 @@ __global__ __launch_bounds__(64) void k_image(ImgArgs A) {
             y = (fpix * (y + A.Rmax)) / (2.0 * A.Rmax) + 1.0;
             const int ix = (int)x, iy = (int)y;
             const int px = (ix <= A.npix && iy <= A.npix && ix > 0 && iy > 0) ? (ix - 1) * A.npix + (iy - 1) : -1;
-            if (side)
-                img_deposit(A, acc[1], px, flux / den, fQ / den, -(fU / den), -(fV / den));
-            else
-                img_deposit(A, acc[0], px, flux / den, fQ / den, fU / den, fV / den);
+            ImgAcc &c = acc[side];
+            if (px != c.px) {
+                img_put(A, tile[side], box[side], c);
+                c.px = px;
+                c.f = c.q = c.u = c.v = 0.0;
+            }
+            if (px >= 0) {
+                c.f += flux / den;
+                c.q += fQ / den;
+                c.u += side ? -(fU / den) : fU / den;
+                c.v += side ? -(fV / den) : fV / den;
+            }
         }
     }
-    img_flush_wave(A, acc[0]);
-    img_flush_wave(A, acc[1]);
+    img_put(A, tile[0], box[0], acc[0]);
+    img_put(A, tile[1], box[1], acc[1]);
+    __syncthreads();
+    for (int side = 0; side < 2; side++) {
+        const ImgBox &B = box[side];
+        if (!B.ok) continue;
+        for (int e = lane; e < B.nx * B.ny; e += 64) {
+            const size_t px = (size_t)(B.x0 + e / B.ny - 1) * A.npix + (B.y0 + e % B.ny - 1);
+            const double f = tile[side][0][e];
+            if (f != 0.0) atomicAdd(&A.im[px], f);
+            if (A.scat_how == 2) {
+                const double q = tile[side][1][e], u = tile[side][2][e], v = tile[side][3][e];
+                if (q != 0.0) atomicAdd(&A.imQ[px], q);
+                if (u != 0.0) atomicAdd(&A.imU[px], u);
+                if (v != 0.0) atomicAdd(&A.imV[px], v);
+            }
+        }
+    }
 }
 
 // ===========================================================================
@@ -1727,20 +1776,10 @@ extern "C" int mcx_image(mcx_ctx *ctx, const mcx_obs *o, const double *T, const 
     A.imU = dim + 2 * npx;
     A.imV = dim + 3 * npx;
     const long long ncellp = (long long)(ro.nimg - 1) * ro.nphi;
-    // chunks of samples per cell: about 8192 waves over the whole image
-    const long long want = 8192LL * 64;
-    long long nchunk = ncellp > 0 ? std::min<long long>(io->nint, std::max<long long>(1, (want + ncellp - 1) / ncellp))
-                                  : 1;
-    const int clen = (int)((io->nint + nchunk - 1) / nchunk);
-    nchunk = (io->nint + clen - 1) / clen;
-    A.nchunk = (int)nchunk;
-    A.clen = clen;
-    const long long nthreads = ncellp * nchunk;
-    if (ncellp > 0 && (nthreads + 63) / 64 > 0x7fffffffLL)
-        return fail(ctx, MCX_EINVAL, "image: too many ring cells and samples for one launch");
+    if (ncellp > 0x7fffffffLL) return fail(ctx, MCX_EINVAL, "image: too many ring cells for one launch");
     if (ncellp > 0) {
         HIPCHK(ctx, hipEventRecord(ctx->ev[0], st));
-        hipLaunchKernelGGL(k_image, dim3((unsigned)((nthreads + 63) / 64)), dim3(64), 0, st, A);
+        hipLaunchKernelGGL(k_image, dim3((unsigned)ncellp), dim3(64), 0, st, A);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(ctx->ev[1], st));
         HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));
