@@ -6,8 +6,9 @@
  * driver (Main.f:498-558), and the steps either side of it: the diffusion
  * approximation (Diffuse), the vertical-structure iteration (TempAverage,
  * DiskStructure, compareCells), the ray-traced SED (Observe SPECTRUM,
- * TraceMono) and the dust opacities the pass reads (ComputePart, the DHS
- * particle from optical constants).  The reference passes everything through module
+ * TraceMono), the ray-traced image (Observe IMAGE, MakeImage) and the dust
+ * opacities the pass reads (ComputePart, the DHS particle from optical
+ * constants).  The reference passes everything through module
  * globals (Modules.f:141-150); here the caller hands the same quantities over
  * once as plain host arrays (mcx_set_tables, mcx_set_model) and the library
  * keeps them resident in HBM.
