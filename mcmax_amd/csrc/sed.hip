@@ -20,6 +20,35 @@
 #include "mcx_internal.h"
 
 #define NPHIS 180 // observer directions of the scattered field (TraceMono.f:1309)
+
+// Per-call temporaries of the SED and image calls come from the stream-ordered
+// pool (hipMallocAsync / hipFreeAsync on the context's stream): a call makes
+// about forty of them, and hipFree would synchronise the device for each.
+// Every host read of their contents follows a stream synchronisation.
+template <typename T>
+static int palloc(mcx_ctx *ctx, std::vector<void *> &list, T **p, size_t n) {
+    void *q = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMallocAsync(&q, n * sizeof(T), ctx->stream);
+    if (e != hipSuccess)
+        return fail(ctx, MCX_ENOMEM, "hipMallocAsync of " + std::to_string(n * sizeof(T)) + " bytes failed");
+    list.push_back(q);
+    *p = (T *)q;
+    return 0;
+}
+
+template <typename T>
+static int pupload(mcx_ctx *ctx, std::vector<void *> &list, T **p, const T *h, size_t n) {
+    int r = palloc(ctx, list, p, n);
+    if (r) return r;
+    HIPCHK(ctx, hipMemcpyAsync(*p, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return 0;
+}
+
+static void pfree_list(mcx_ctx *ctx, std::vector<void *> &list) {
+    for (void *q : list) (void)hipFreeAsync(q, ctx->stream);
+    list.clear();
+}
 #define NSTAR_RAYS 30
 
 struct SedGeom {
@@ -1411,9 +1440,10 @@ static int sed_core(mcx_ctx *ctx, const mcx_obs *o, const double *T, const doubl
     const bool mc = scattering && (nmc + nmcs) != 0;
     std::vector<void *> tmp;
     struct Guard {
+        mcx_ctx *c;
         std::vector<void *> &l;
-        ~Guard() { free_list(l); }
-    } guard{tmp};
+        ~Guard() { pfree_list(c, l); }
+    } guard{ctx, tmp};
     int r;
     // ---- geometry of the observation -----------------------------------
     std::vector<int> nrg(nt), rgoff(nt);
@@ -1469,14 +1499,14 @@ static int sed_core(mcx_ctx *ctx, const mcx_obs *o, const double *T, const doubl
     const int nphi = o->nphi, nray = nimg * nphi;
     int *d_nrg, *d_rgoff;
     double *d_thetrg, *d_fracirg, *d_dirs, *d_pisin, *d_Rimg, *d_thet;
-    if ((r = dupload(ctx, tmp, &d_nrg, nrg.data(), nrg.size()))) return r;
-    if ((r = dupload(ctx, tmp, &d_rgoff, rgoff.data(), rgoff.size()))) return r;
-    if ((r = dupload(ctx, tmp, &d_thetrg, thetrg.data(), thetrg.size()))) return r;
-    if ((r = dupload(ctx, tmp, &d_fracirg, fracirg.data(), fracirg.size()))) return r;
-    if ((r = dupload(ctx, tmp, &d_dirs, dirs.data(), dirs.size()))) return r;
-    if ((r = dupload(ctx, tmp, &d_pisin, pisin.data(), pisin.size()))) return r;
-    if ((r = dupload(ctx, tmp, &d_Rimg, Rimg.data(), Rimg.size()))) return r;
-    if ((r = dupload(ctx, tmp, &d_thet, ctx->h_thet.data(), ctx->h_thet.size()))) return r;
+    if ((r = pupload(ctx, tmp, &d_nrg, nrg.data(), nrg.size()))) return r;
+    if ((r = pupload(ctx, tmp, &d_rgoff, rgoff.data(), rgoff.size()))) return r;
+    if ((r = pupload(ctx, tmp, &d_thetrg, thetrg.data(), thetrg.size()))) return r;
+    if ((r = pupload(ctx, tmp, &d_fracirg, fracirg.data(), fracirg.size()))) return r;
+    if ((r = pupload(ctx, tmp, &d_dirs, dirs.data(), dirs.size()))) return r;
+    if ((r = pupload(ctx, tmp, &d_pisin, pisin.data(), pisin.size()))) return r;
+    if ((r = pupload(ctx, tmp, &d_Rimg, Rimg.data(), Rimg.size()))) return r;
+    if ((r = pupload(ctx, tmp, &d_thet, ctx->h_thet.data(), ctx->h_thet.size()))) return r;
     SedGeom G;
     G.nr = nr;
     G.nt = nt;
@@ -1495,9 +1525,9 @@ static int sed_core(mcx_ctx *ctx, const mcx_obs *o, const double *T, const doubl
     // ---- image rays ------------------------------------------------------
     int *d_cnt, *d_hit;
     long long *d_off;
-    if ((r = dalloc(ctx, tmp, &d_cnt, (size_t)nray))) return r;
-    if ((r = dalloc(ctx, tmp, &d_hit, (size_t)nray))) return r;
-    if ((r = dalloc(ctx, tmp, &d_off, (size_t)nray))) return r;
+    if ((r = palloc(ctx, tmp, &d_cnt, (size_t)nray))) return r;
+    if ((r = palloc(ctx, tmp, &d_hit, (size_t)nray))) return r;
+    if ((r = palloc(ctx, tmp, &d_off, (size_t)nray))) return r;
     const unsigned rb = (unsigned)((nray + 127) / 128);
     hipLaunchKernelGGL(k_paths, dim3(rb), dim3(128), 0, st, G, d_Rimg, nimg, nphi, o->angle, 0, d_cnt,
                        (const long long *)nullptr, (int *)nullptr, (unsigned char *)nullptr,
@@ -1517,12 +1547,12 @@ static int sed_core(mcx_ctx *ctx, const mcx_obs *o, const double *T, const doubl
     int *seg_c;
     unsigned char *seg_irg, *seg_k, *seg_j1, *seg_j2;
     double *seg_v;
-    if ((r = dalloc(ctx, tmp, &seg_c, (size_t)nseg))) return r;
-    if ((r = dalloc(ctx, tmp, &seg_irg, (size_t)nseg))) return r;
-    if ((r = dalloc(ctx, tmp, &seg_k, (size_t)nseg))) return r;
-    if ((r = dalloc(ctx, tmp, &seg_j1, (size_t)nseg))) return r;
-    if ((r = dalloc(ctx, tmp, &seg_j2, (size_t)nseg))) return r;
-    if ((r = dalloc(ctx, tmp, &seg_v, (size_t)nseg))) return r;
+    if ((r = palloc(ctx, tmp, &seg_c, (size_t)nseg))) return r;
+    if ((r = palloc(ctx, tmp, &seg_irg, (size_t)nseg))) return r;
+    if ((r = palloc(ctx, tmp, &seg_k, (size_t)nseg))) return r;
+    if ((r = palloc(ctx, tmp, &seg_j1, (size_t)nseg))) return r;
+    if ((r = palloc(ctx, tmp, &seg_j2, (size_t)nseg))) return r;
+    if ((r = palloc(ctx, tmp, &seg_v, (size_t)nseg))) return r;
     hipLaunchKernelGGL(k_paths, dim3(rb), dim3(128), 0, st, G, d_Rimg, nimg, nphi, o->angle, 1, d_cnt,
                        (const long long *)d_off, seg_c, seg_irg, seg_k, seg_v, seg_j1, seg_j2, d_hit);
     HIPCHK(ctx, hipGetLastError());
@@ -1534,38 +1564,38 @@ static int sed_core(mcx_ctx *ctx, const mcx_obs *o, const double *T, const doubl
     // ---- per-wavelength buffers -------------------------------------------
     double *d_T = nullptr;
     if (T) {
-        if ((r = dupload(ctx, tmp, &d_T, T, (size_t)ncell))) return r;
+        if ((r = pupload(ctx, tmp, &d_T, T, (size_t)ncell))) return r;
     } else {
         d_T = ctx->d_T;
     }
     double *Kabs, *Kext, *Ksca, *Alb, *emis, *Eem, *EmisDis, *vism, *cum, *tot, *Fmix, *MIF11, *MIF12;
-    if ((r = dalloc(ctx, tmp, &Kabs, (size_t)ncell))) return r;
-    if ((r = dalloc(ctx, tmp, &Kext, (size_t)ncell))) return r;
-    if ((r = dalloc(ctx, tmp, &Ksca, (size_t)ncell))) return r;
-    if ((r = dalloc(ctx, tmp, &Alb, (size_t)ncell))) return r;
-    if ((r = dalloc(ctx, tmp, &emis, (size_t)ncell))) return r;
-    if ((r = dalloc(ctx, tmp, &Eem, (size_t)ncell))) return r;
-    if ((r = dalloc(ctx, tmp, &EmisDis, (size_t)ncell))) return r;
-    if ((r = dalloc(ctx, tmp, &vism, (size_t)ncell))) return r;
-    if ((r = dalloc(ctx, tmp, &cum, (size_t)ncell))) return r;
-    if ((r = dalloc(ctx, tmp, &tot, 2))) return r;
+    if ((r = palloc(ctx, tmp, &Kabs, (size_t)ncell))) return r;
+    if ((r = palloc(ctx, tmp, &Kext, (size_t)ncell))) return r;
+    if ((r = palloc(ctx, tmp, &Ksca, (size_t)ncell))) return r;
+    if ((r = palloc(ctx, tmp, &Alb, (size_t)ncell))) return r;
+    if ((r = palloc(ctx, tmp, &emis, (size_t)ncell))) return r;
+    if ((r = palloc(ctx, tmp, &Eem, (size_t)ncell))) return r;
+    if ((r = palloc(ctx, tmp, &EmisDis, (size_t)ncell))) return r;
+    if ((r = palloc(ctx, tmp, &vism, (size_t)ncell))) return r;
+    if ((r = palloc(ctx, tmp, &cum, (size_t)ncell))) return r;
+    if ((r = palloc(ctx, tmp, &tot, 2))) return r;
     const int nclass = ctx->nclass;
-    if ((r = dalloc(ctx, tmp, &Fmix, (size_t)nclass * 6 * 180))) return r;
-    if ((r = dalloc(ctx, tmp, &MIF11, (size_t)nclass))) return r;
-    if ((r = dalloc(ctx, tmp, &MIF12, (size_t)nclass))) return r;
+    if ((r = palloc(ctx, tmp, &Fmix, (size_t)nclass * 6 * 180))) return r;
+    if ((r = palloc(ctx, tmp, &MIF11, (size_t)nclass))) return r;
+    if ((r = palloc(ctx, tmp, &MIF12, (size_t)nclass))) return r;
     const size_t nsf = (size_t)nr * NRG * 91 * 2;
     double *sf, *sQ, *sU, *sV;
-    if ((r = dalloc(ctx, tmp, &sf, nsf))) return r;
-    if ((r = dalloc(ctx, tmp, &sQ, nsf))) return r;
-    if ((r = dalloc(ctx, tmp, &sU, nsf))) return r;
-    if ((r = dalloc(ctx, tmp, &sV, nsf))) return r;
+    if ((r = palloc(ctx, tmp, &sf, nsf))) return r;
+    if ((r = palloc(ctx, tmp, &sQ, nsf))) return r;
+    if ((r = palloc(ctx, tmp, &sU, nsf))) return r;
+    if ((r = palloc(ctx, tmp, &sV, nsf))) return r;
     double *image, *scatim, *imageQ, *imageU = nullptr, *imageV = nullptr;
-    if ((r = dalloc(ctx, tmp, &image, (size_t)nray))) return r;
-    if ((r = dalloc(ctx, tmp, &scatim, (size_t)nray))) return r;
-    if ((r = dalloc(ctx, tmp, &imageQ, (size_t)nray))) return r;
+    if ((r = palloc(ctx, tmp, &image, (size_t)nray))) return r;
+    if ((r = palloc(ctx, tmp, &scatim, (size_t)nray))) return r;
+    if ((r = palloc(ctx, tmp, &imageQ, (size_t)nray))) return r;
     if (ro) {
-        if ((r = dalloc(ctx, tmp, &imageU, (size_t)nray))) return r;
-        if ((r = dalloc(ctx, tmp, &imageV, (size_t)nray))) return r;
+        if ((r = palloc(ctx, tmp, &imageU, (size_t)nray))) return r;
+        if ((r = palloc(ctx, tmp, &imageV, (size_t)nray))) return r;
     }
     std::vector<double> h_img(nray), h_sim(nray), h_imQ(nray);
     const unsigned cb = (unsigned)((ncell + 255) / 256);
@@ -1729,19 +1759,20 @@ extern "C" int mcx_image(mcx_ctx *ctx, const mcx_obs *o, const double *T, const 
     hipStream_t st = ctx->stream;
     std::vector<void *> tmp;
     struct Guard {
+        mcx_ctx *c;
         std::vector<void *> &l;
-        ~Guard() { free_list(l); }
-    } guard{tmp};
+        ~Guard() { pfree_list(c, l); }
+    } guard{ctx, tmp};
     const size_t npx = (size_t)io->npix * io->npix;
     double *dR, *dI, *dQ, *dU, *dV, *dim;
-    if ((r = dupload(ctx, tmp, &dR, ro.R.data(), ro.R.size()))) return r;
-    if ((r = dupload(ctx, tmp, &dI, ro.I.data(), (size_t)nray))) return r;
-    if ((r = dupload(ctx, tmp, &dQ, ro.Q.data(), (size_t)nray))) return r;
-    if ((r = dupload(ctx, tmp, &dU, ro.U.data(), (size_t)nray))) return r;
-    if ((r = dupload(ctx, tmp, &dV, ro.V.data(), (size_t)nray))) return r;
-    if ((r = dalloc(ctx, tmp, &dim, 4 * npx))) return r;
+    if ((r = pupload(ctx, tmp, &dR, ro.R.data(), ro.R.size()))) return r;
+    if ((r = pupload(ctx, tmp, &dI, ro.I.data(), (size_t)nray))) return r;
+    if ((r = pupload(ctx, tmp, &dQ, ro.Q.data(), (size_t)nray))) return r;
+    if ((r = pupload(ctx, tmp, &dU, ro.U.data(), (size_t)nray))) return r;
+    if ((r = pupload(ctx, tmp, &dV, ro.V.data(), (size_t)nray))) return r;
+    if ((r = palloc(ctx, tmp, &dim, 4 * npx))) return r;
     double *dfac;
-    if ((r = dalloc(ctx, tmp, &dfac, 5 * (size_t)std::max(nray, 1)))) return r;
+    if ((r = palloc(ctx, tmp, &dfac, 5 * (size_t)std::max(nray, 1)))) return r;
     if (nray > 0) {
         hipLaunchKernelGGL(k_image_rays, dim3((unsigned)((nray + 255) / 256)), dim3(256), 0, st, nray,
                            ctx->tb.scat_how, dI, dQ, dU, dV, dfac, dfac + nray, dfac + 2 * (size_t)nray,
