@@ -295,3 +295,30 @@ def test_observation_file_image(tiny_model, tmp_path):
     p.write_text("type=VISIBILITY\n")
     with pytest.raises(observe.ObservationError):
         observe.parse_observation_file(str(p), m)
+
+
+def test_rimage_profile_closed_form(tiny_model, tmp_path):
+    """RImage's flux column for rays of constant I is the trapezoid sum of
+    2 pi R I AU^2 over the rings, exact for that linear integrand:
+    1e23 I pi AU^2 (R_i^2 - R_0^2) / d^2 at ring i+1; the intensity column
+    is the ray at azimuth nphi/2 times the pixel area scaling."""
+    m = tiny_model
+    R = np.array([0.0, 0.5, 1.5, 4.0, 9.0])
+    nphi = 6
+    rays = np.zeros((4, len(R), nphi))
+    rays[0] = 2.0
+    rays[0, :, nphi // 2 - 1] = 3.0   # the column RImage prints
+    p = tmp_path / "r.dat"
+    output.write_rimage(str(p), m, R, rays, 1.0, 1.0, 20.0, 100)
+    rows = [[float(x) for x in line.split()] for line in open(p)]
+    d2 = m.distance_cm ** 2
+    scaling = (1e23 / d2) * (20.0 ** 2 * AU ** 2) / 100 ** 2
+    assert [r[0] for r in rows] == list(R)
+    # the ray at nphi/2 carries 3, the other five 2: mean 13/6 per azimuth;
+    # row k holds R_k and the flux inside R_{k+1} (the last row, R_n and the
+    # total)
+    C = 1e23 * (13.0 / 6.0) * math.pi * AU ** 2 / d2
+    for k in range(len(R) - 1):
+        assert rows[k][1] == pytest.approx(C * (R[k + 1] ** 2 - R[0] ** 2), rel=1e-13), k
+    assert rows[-1][1] == pytest.approx(C * (R[-1] ** 2 - R[0] ** 2), rel=1e-13)
+    assert all(r[2] == pytest.approx(3.0 * scaling, rel=1e-15) for r in rows)
