@@ -776,6 +776,86 @@ extern "C" int mcx_compute_parts(mcx_ctx *ctx, int32_t nlam, const double *lam, 
     }
     const int ncall = (int)calls.size();
 
+    // ---- homogeneous spheres known before any layered result --------------
+    // The too-large (x > 10^4) and solid grains take MeerhoffMie throughout
+    // (ComputePart.f:316-330): a few hundred long serial chains on a few
+    // CUs, run on a second stream of default priority while the layered
+    // spheres use the rest of the device.
+    auto make_item = [&](const Group &g, long long &offd, long long &offc, HomItem &it) -> int {
+        const mcx_part_spec &p = specs[g.s];
+        const double rmie = (double)g.r1, lmie0 = lam[g.il];
+        const double lmie = (rmie / lmie0 < 5000.0) ? lmie0 : rmie / 5000.0;
+        it.m_re = (double)p.e1[(size_t)g.l * nlam + g.il];
+        it.m_im = -(double)p.e2[(size_t)g.l * nlam + g.il];
+        const double rtox = 2.0 * pie / lmie;
+        it.x = rtox * rmie;
+        it.fakt = lmie * lmie / (2.0 * pie);
+        it.nmax = (int)(it.x + 4.05 * pow(it.x, 1.0 / 3.0) + 20.0);
+        it.nfi = it.nmax + 60;
+        const double zabs = it.x * hypot(it.m_re, it.m_im);
+        it.nD = (int)(zabs + 4.05 * pow(zabs, 1.0 / 3.0) + 70.0);
+        if (it.nD < it.nmax)
+            return fail(ctx, MCX_EINVAL, "species " + std::to_string(g.s) +
+                                             ": |m| x too small for the homogeneous-sphere series (ComputePart.f:2946-2950)");
+        it.perx = 1.0 / it.x;
+        it.sinx = sin(it.x);
+        it.cosx = cos(it.x);
+        it.off_psi = offd;
+        offd += it.nfi + 1;
+        it.off_D = offc;
+        offc += it.nD + 1;
+        it.off_ab = offc;
+        offc += 2 * ((long long)it.nmax + 1);
+        return 0;
+    };
+    struct Side {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        ~Side() {
+            if (st) (void)hipStreamSynchronize(st);
+            if (ev) (void)hipEventDestroy(ev);
+            if (st) (void)hipStreamDestroy(st);
+        }
+    } side;
+    std::vector<HomItem> items;
+    long long offd_a = 0, offc_a = 0;
+    for (Group &g : groups) {
+        if (!g.hom_only) continue;
+        HomItem it{};
+        if ((rc = make_item(g, offd_a, offc_a, it))) return rc;
+        g.item = (int)items.size();
+        items.push_back(it);
+    }
+    // capacity for every group that could need a homogeneous sphere
+    const size_t item_cap = std::max<size_t>(groups.size(), 1);
+    HomItem *ditems = nullptr;
+    HomRes *dhres = nullptr;
+    double *dhout = nullptr, *du = nullptr;
+    if ((rc = dalloc(ctx, tmp.v, &ditems, item_cap))) return rc;
+    if ((rc = dalloc(ctx, tmp.v, &dhres, item_cap))) return rc;
+    if ((rc = dalloc(ctx, tmp.v, &dhout, item_cap * 4 * NA))) return rc;
+    if ((rc = dupload(ctx, tmp.v, &du, u, NA))) return rc;
+    size_t max_scratch = 0;
+    if (!items.empty()) {
+        const int n = (int)items.size();
+        HIPCHK(ctx, hipMemcpyAsync(ditems, items.data(), sizeof(HomItem) * n, hipMemcpyHostToDevice, st));
+        double *scrd = nullptr;
+        cxd *scrc = nullptr;
+        if ((rc = dalloc(ctx, tmp.v, &scrd, (size_t)offd_a))) return rc;
+        if ((rc = dalloc(ctx, tmp.v, &scrc, (size_t)offc_a))) return rc;
+        max_scratch = (size_t)offd_a * 8 + (size_t)offc_a * 16;
+        HIPCHK(ctx, hipStreamCreate(&side.st));
+        HIPCHK(ctx, hipEventCreateWithFlags(&side.ev, hipEventDisableTiming));
+        // the side stream starts after the uploads on st
+        HIPCHK(ctx, hipEventRecord(side.ev, st));
+        HIPCHK(ctx, hipStreamWaitEvent(side.st, side.ev, 0));
+        k_hom_coef<<<(n + 63) / 64, 64, 0, side.st>>>(ditems, 0, n, scrd, scrc, dhres);
+        HIPCHK(ctx, hipGetLastError());
+        k_hom_angles<<<n, 64, 0, side.st>>>(ditems, 0, scrc, du, dhres, dhout);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventRecord(side.ev, side.st));
+    }
+
     // ---- layered spheres, in batches that fit the scratch budget ----------
     size_t freeb = 0, totb = 0;
     HIPCHK(ctx, hipMemGetInfo(&freeb, &totb));
@@ -788,7 +868,6 @@ extern "C" int mcx_compute_parts(mcx_ctx *ctx, int32_t nlam, const double *lam, 
     if ((rc = dalloc(ctx, tmp.v, &dres, calls.size()))) return rc;
     if ((rc = dalloc(ctx, tmp.v, &dout, calls.size() * 4 * NA))) return rc;
     if ((rc = dupload(ctx, tmp.v, &dmu, mu, NH))) return rc;
-    size_t max_scratch = 0;
     cxd *scr = nullptr;
     size_t scr_bytes = 0;
     for (int c0 = 0; c0 < ncall;) {
@@ -829,60 +908,35 @@ extern "C" int mcx_compute_parts(mcx_ctx *ctx, int32_t nlam, const double *lam, 
     }
     scr = nullptr;
 
-    // ---- homogeneous spheres where the reference uses them ----------------
-    std::vector<HomItem> items;
-    long long offd = 0, offc = 0;
+    // ---- homogeneous spheres of the layered-sphere failures ---------------
+    // (the ones of the too-large and solid grains already run on st2)
+    long long offd_b = 0, offc_b = 0;
+    const int nA = (int)items.size();
     for (Group &g : groups) {
-        bool need = g.hom_only;
+        if (g.hom_only) continue;
+        bool need = false;
         for (int i = 0; i < g.ncall && !need; i++) need = hres_d[g.first + i].err != 0;
         if (!need) continue;
-        const mcx_part_spec &p = specs[g.s];
         HomItem it{};
-        const double rmie = (double)g.r1, lmie0 = lam[g.il];
-        const double lmie = (rmie / lmie0 < 5000.0) ? lmie0 : rmie / 5000.0;
-        it.m_re = (double)p.e1[(size_t)g.l * nlam + g.il];
-        it.m_im = -(double)p.e2[(size_t)g.l * nlam + g.il];
-        const double rtox = 2.0 * pie / lmie;
-        it.x = rtox * rmie;
-        it.fakt = lmie * lmie / (2.0 * pie);
-        it.nmax = (int)(it.x + 4.05 * pow(it.x, 1.0 / 3.0) + 20.0);
-        it.nfi = it.nmax + 60;
-        const double zabs = it.x * hypot(it.m_re, it.m_im);
-        it.nD = (int)(zabs + 4.05 * pow(zabs, 1.0 / 3.0) + 70.0);
-        if (it.nD < it.nmax)
-            return fail(ctx, MCX_EINVAL, "species " + std::to_string(g.s) +
-                                             ": |m| x too small for the homogeneous-sphere series (ComputePart.f:2946-2950)");
-        it.perx = 1.0 / it.x;
-        it.sinx = sin(it.x);
-        it.cosx = cos(it.x);
-        it.off_psi = offd;
-        offd += it.nfi + 1;
-        it.off_D = offc;
-        offc += it.nD + 1;
-        it.off_ab = offc;
-        offc += 2 * ((long long)it.nmax + 1);
+        if ((rc = make_item(g, offd_b, offc_b, it))) return rc;
         g.item = (int)items.size();
         items.push_back(it);
     }
     const int nitem = (int)items.size();
-    HomItem *ditems = nullptr;
-    HomRes *dhres = nullptr;
-    double *dhout = nullptr, *du = nullptr;
-    if ((rc = dupload(ctx, tmp.v, &ditems, items.data(), items.size()))) return rc;
-    if ((rc = dalloc(ctx, tmp.v, &dhres, items.size()))) return rc;
-    if ((rc = dalloc(ctx, tmp.v, &dhout, items.size() * 4 * NA))) return rc;
-    if ((rc = dupload(ctx, tmp.v, &du, u, NA))) return rc;
-    if (nitem) {
+    if (nitem > nA) {
+        HIPCHK(ctx, hipMemcpyAsync(ditems + nA, items.data() + nA, sizeof(HomItem) * (nitem - nA),
+                                   hipMemcpyHostToDevice, st));
         double *scrd = nullptr;
         cxd *scrc = nullptr;
-        if ((rc = dalloc(ctx, tmp.v, &scrd, (size_t)offd))) return rc;
-        if ((rc = dalloc(ctx, tmp.v, &scrc, (size_t)offc))) return rc;
-        max_scratch = std::max(max_scratch, (size_t)offd * 8 + (size_t)offc * 16);
-        k_hom_coef<<<(nitem + 63) / 64, 64, 0, st>>>(ditems, 0, nitem, scrd, scrc, dhres);
+        if ((rc = dalloc(ctx, tmp.v, &scrd, (size_t)offd_b))) return rc;
+        if ((rc = dalloc(ctx, tmp.v, &scrc, (size_t)offc_b))) return rc;
+        max_scratch = std::max(max_scratch, (size_t)offd_b * 8 + (size_t)offc_b * 16);
+        k_hom_coef<<<(nitem - nA + 63) / 64, 64, 0, st>>>(ditems, nA, nitem - nA, scrd, scrc, dhres);
         HIPCHK(ctx, hipGetLastError());
-        k_hom_angles<<<nitem, 64, 0, st>>>(ditems, 0, scrc, du, dhres, dhout);
+        k_hom_angles<<<nitem - nA, 64, 0, st>>>(ditems, nA, scrc, du, dhres, dhout);
         HIPCHK(ctx, hipGetLastError());
     }
+    if (nA) HIPCHK(ctx, hipStreamWaitEvent(st, side.ev, 0));
 
     // ---- the terms of ComputePart's sums, in its order ------------------
     const int nblock = nspec * nlam;
