@@ -72,7 +72,8 @@ class Iteration:
         while True:
             prev = cur
             t0 = time.perf_counter()
-            rc = tr.transport(self.nphot, seed=self.seed + len(self.history))
+            rc = tr.transport(self.nphot, seed=self.seed + len(self.history),
+                              maxinteract=int(getattr(m, "maxinteract", 0)))
             tal = tr.tallies()
             self.last_tallies = tal
             T, dT, dEJv = tr.temperature()

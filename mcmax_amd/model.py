@@ -398,6 +398,9 @@ def build_model(kw, basedir=".", tr=None, device=0):
     m.Ksca = np.maximum(m.Ksca, 0.0)
     m.Kext = m.Kabs + m.Ksca  # Init.f:3428-3436
     finish_opacities(m, p)
+    # keyword maxinteract != 0 switches on the reference's interaction
+    # roulette (Init.f:637-639, MCRadiation.f:568-571); 0 leaves it off
+    m.maxinteract = int(p.get("maxinteract", 0))
     m.randomwalk = bool(p.get("randomwalk", False))
     m.factRW = float(p.get("factrw", 10.0))
     # Init.f:2266-2272: without any abundance keyword every species weighs the same

@@ -231,3 +231,28 @@ def test_parallel_statistics_dhs_species(gpu, workdir, oracle_lib):
     m = configs.load_config("tiny", workdir + "/gpu_tiny_dhs_thin", tr=gpu, **_dhs_overrides(3))
     assert m.ng == 3
     _statistics(gpu, m, oracle_lib, 100000)
+
+
+def test_serial_parity_interaction_roulette(gpu, workdir, oracle_lib):
+    """Keyword maxinteract != 0: after every interaction the packet dies with
+    probability 1/maxinteract, the survivors carry E maxinteract/(maxinteract-1)
+    (MCRadiation.f:567-571).  One lane against the oracle: the same packets
+    die, and every counter but the midplane-split step count agrees."""
+    from mcmax_amd import configs
+    m = configs.load_config("tiny", workdir + "/gpu_tiny_roulette", mdust=1e-3, maxinteract=40)
+    assert m.maxinteract == 40
+    gpu.set_model(m)
+    n = 300
+    g = transport.mc_radiation(gpu, n, seed=17, nlanes=1, maxinteract=m.maxinteract)
+    o = oracle_lib.Oracle(m).mcradiation(n, rng="philox", seed=17, maxinteract=m.maxinteract)
+    assert o["nmaxinteract"] > 10
+    assert g["nmaxinteract"] == o["nmaxinteract"]
+    for k in COUNTERS:
+        if k == "nsteps":
+            assert abs(g[k] - o[k]) <= 0.005 * o[k], (k, g[k], o[k])
+        else:
+            assert g[k] == o[k], (k, g[k], o[k])
+    assert np.array_equal(g["Ni"], o["Ni"])
+    # killed packets take their energy with them: less escapes than L*
+    assert g["nescape"] + g["nmaxinteract"] == n
+    assert g["Eescape"] < m.Lstar

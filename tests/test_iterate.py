@@ -18,7 +18,7 @@ class FakeTransport:
     def set_model(self, m, tables=True):
         self.calls.append(("set_model", tables))
 
-    def transport(self, nphot, seed=1):
+    def transport(self, nphot, seed=1, maxinteract=0):
         self.calls.append(("transport", seed))
         return 0
 
