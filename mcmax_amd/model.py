@@ -341,6 +341,10 @@ def build_model(kw, basedir=".", tr=None, device=0):
     m.Rin, m.Rout = float(p["rin"]), float(p["rout"])
     if int(p.get("ntspan", 0)) * int(p.get("ntlev", 0)) != 0:
         raise SetupError("ntspan/ntlev refinement is not supported")
+    if bool(p.get("radpress", False)):
+        # the pass would also have to tally the radiation force per cell
+        # (MCRadiation.f:842-880) for DiskStructure
+        raise SetupError("radpress (radiation pressure from the Monte Carlo pass) is not supported")
     m.Theta, m.theta_av, m.thet = theta_grid(int(p["ntheta"]))
     m.R, m.R_av = r_grid(int(p["nrad"]), m.Rin, m.Rout, m.Rstar_cm)
     m.nr, m.nt = len(m.R) - 1, len(m.Theta) - 1

@@ -117,3 +117,11 @@ def test_setup_rejects_unsupported_options(workdir):
     from mcmax_amd import configs
     with pytest.raises(model.SetupError):
         configs.load_config("tiny", os.path.join(workdir, "bad"), gridrefine=True)
+
+
+def test_radiation_pressure_is_refused(tmp_path):
+    """radpress needs the pass's radiation-force tally, which is not built:
+    the setup refuses it rather than run without it."""
+    from mcmax_amd import configs, model
+    with pytest.raises(model.SetupError):
+        configs.load_config("tiny", str(tmp_path), radpress=True)
