@@ -90,16 +90,26 @@ typedef struct {
     uint64_t seed;                   /* Philox4x32-10 key; packet p uses the
                                         counter stream (block, p)           */
     int32_t maxinteract;             /* >0: roulette, MCRadiation.f:568-571 */
-    int32_t nlanes;                  /* packets in flight at once.  0: the
-                                        device's resident lanes, at most
-                                        nphot/64 (Bjorkman-Wood slices are
-                                        taken in arrival order; below ~2% in
-                                        flight the temperatures keep the
-                                        serial statistics); 1: a single lane
-                                        (packets in order, as the serial
-                                        reference loop MCRadiation.f:320)   */
-    int64_t bitmap_bytes_max;        /* cap for the distinct-packet bitmap
-                                        behind Ni (0: 32 GiB)               */
+    int32_t nlanes;                  /* 1: a single lane, packets in order,
+                                        each Bjorkman-Wood absorption takes
+                                        its slice of the cell's running
+                                        total, as the serial reference loop
+                                        (MCRadiation.f:320, Radiation.f:173-
+                                        258).  Otherwise the staged schedule
+                                        below, with at most this many
+                                        packets in flight (0: the device's
+                                        resident lanes); its results do not
+                                        depend on the concurrency            */
+    int64_t stage0;                  /* staged schedule: packets run in
+                                        stages [P_k, P_k+1), P_1 = stage0
+                                        (0: 64), P_k+1 = stage_ratio P_k;
+                                        packet p of a stage takes its slice
+                                        in cell c at S_k(c) p / P_k plus its
+                                        own earlier deposits in c, S_k(c) the
+                                        Eabs of the packets before the stage
+                                        (the serial loop's running total in
+                                        expectation).  A pass is then
+                                        bit-reproducible for a given seed   */
     int32_t randomwalk;              /* keyword randomwalk (RNDW): modified
                                         random walk in thick cells          */
     double factRW;                   /* keyword factRW (Init.f:205: 10)     */
@@ -108,7 +118,12 @@ typedef struct {
                                         this many lanes need one; -1 picks
                                         the default, 0 never (results do not
                                         depend on it)                       */
+    int32_t stage_ratio;             /* 0: 2; -1: the ratio grows as
+                                        min(16, 1 + sqrt(P_k / stage0))     */
+    int32_t precision;               /* MCX_FP64 (the reference's real*8)   */
 } mcx_run;
+
+#define MCX_FP64 0
 
 /* Pass tallies and statistics (MCRadiation.f:656-715). */
 typedef struct {
@@ -133,6 +148,9 @@ typedef struct {
                                         when 50, 90, 99, 99.9 and 100% of
                                         the packets had finished (last
                                         launch of the pass; -1: unknown)    */
+    int64_t nvisit_overflow;         /* cell entries a packet's visit table
+                                        could not hold (Ni may count such a
+                                        packet twice; 0 in every test)      */
 } mcx_tallies;
 
 /* Temperature products of Main.f:498-549 and the void cell (Main.f:546). */

@@ -41,8 +41,9 @@ class Grid(C.Structure):
 class Run(C.Structure):
     _fields_ = [
         ("nphot", C.c_int64), ("seed", C.c_uint64), ("maxinteract", C.c_int32),
-        ("nlanes", C.c_int32), ("bitmap_bytes_max", C.c_int64), ("randomwalk", C.c_int32),
-        ("factRW", C.c_double), ("coop_lanes", C.c_int32),
+        ("nlanes", C.c_int32), ("stage0", C.c_int64), ("randomwalk", C.c_int32),
+        ("factRW", C.c_double), ("coop_lanes", C.c_int32), ("stage_ratio", C.c_int32),
+        ("precision", C.c_int32),
     ]
 
 
@@ -55,7 +56,7 @@ class Tallies(C.Structure):
         ("nsteps", C.c_int64), ("nraar", C.c_int64), ("nmaxinteract", C.c_int64),
         ("ngeomfail", C.c_int64), ("tautot", C.c_double), ("Eescape", C.c_double),
         ("nrw", C.c_int64), ("noverflow", C.c_int64), ("thick", _ip),
-        ("maxsteps", C.c_int64), ("t_done", C.c_double * 5),
+        ("maxsteps", C.c_int64), ("t_done", C.c_double * 5), ("nvisit_overflow", C.c_int64),
     ]
 
 

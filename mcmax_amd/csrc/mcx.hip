@@ -11,17 +11,22 @@
 //  * the Lucy path-length tallies EJv, EJv2 and the absorbed energy Eabs are
 //    integer fixed-point sums (per-cell units), accumulated with 64-bit
 //    integer atomics: the sums do not depend on the order of additions;
-//  * Bjorkman-Wood reemission takes each absorption's slice of the cell's
-//    cumulative absorbed energy from the value the Eabs fetch-add returns,
-//    i.e. every absorption corrects its own interval of the cell's energy,
-//    exactly as in the serial loop (Radiation.f:173-258);
+//  * Bjorkman-Wood reemission (Radiation.f:173-258): a one-lane pass takes
+//    each absorption's slice of the cell's cumulative absorbed energy from
+//    the value the Eabs fetch-add returns, exactly as the serial loop; a
+//    concurrent pass runs the packets in stages [P_k, P_k+1) and packet p
+//    takes its slice at S_k(c) p/P_k + (its own earlier deposits in c),
+//    S_k the cell's Eabs after the packets before the stage -- the serial
+//    loop's cumulative in expectation, taken from the completed prefix --
+//    so every history depends only on its own draws and the stage's
+//    snapshot, and a pass is bit-reproducible whatever the scheduling;
 //  * reemission wavelengths are drawn by binary search in per-species
 //    cumulative tables int_0^lambda B_nu(T) kappa dnu (one row per integer
 //    T), instead of rebuilding the O(nlam) difference spectrum per
 //    absorption (Radiation.f:187-213, 966-989);
-//  * Ni (distinct packets per cell, MCRadiation.f:943-946) comes from a
-//    cell x packet bitmap: a packet sets its bit when it enters a cell, and a
-//    popcount per cell after the pass gives the exact count;
+//  * Ni (distinct packets per cell, MCRadiation.f:943-946): each lane keeps
+//    a hash table of the cells its current packet has entered (with the
+//    packet's own absorbed energy there); the first entry adds 1 to Ni;
 //  * the modified random walk (RandomWalk.f:47-147) reads its diffusion
 //    coefficients from a table per distinct species mix and integer T, built
 //    once per model on the device, instead of recomputing DiffCoeffCell
@@ -36,10 +41,10 @@ struct TransportArgs {
     unsigned long long *next_pid;
     long long pid_end;
     long long pid_base;
+    long long Pk;  // packets before the stage (the snapshot's count; 0: none)
     long long nphot;
     unsigned long long seed;
     int maxinteract;
-    int nwords;
     int serial;    // only lane 0 of the grid takes packets (in packet order)
     int coop;      // wave-cooperative lookups when at most this many lanes need one
     int rb_mask;   // draws per lane ring minus one (a power of two minus one)
@@ -138,6 +143,27 @@ __device__ double dev_increaseT_w(const double *__restrict__ kpc, int jw, double
     return dev_determineT(kpc, E1, Tstart);
 }
 
+// T of energy per gram E1: the bracket kp(i) <= E1 <= kp(i+1) and its T^4
+// interpolation, the value increaseT and determineT both reach whatever
+// their start; the window kpc[jw..jw+2] in (w0, w1, w2) is tried first.
+__device__ __forceinline__ double t_interp(int i, double kp0, double kp1, double E1) {
+    float a4 = fpow4(i), b4 = fpow4(i + 1);
+    float d4 = b4 - a4;
+    return qroot((double)a4 + ((double)d4 * (E1 - kp0)) / (kp1 - kp0));
+}
+
+__device__ double dev_T_win(const double *__restrict__ kpc, int jw, double w0, double w1, double w2, double E1) {
+    if (w0 <= E1 && w1 >= E1) return t_interp(jw, w0, w1, E1);
+    if (w1 <= E1 && w2 >= E1) return t_interp(jw + 1, w1, w2, E1);
+    return dev_determineT(kpc, E1, (double)((E1 > w2) ? jw + 2 : jw));
+}
+
+// the window start for a temperature guess
+__device__ __forceinline__ int win_start(double g) {
+    int jw = (g >= 1.0) ? (int)(g / 1.0) : 1;
+    if (jw > TMAXD - 2) jw = TMAXD - 2;
+    return jw;
+}
 
 // First bin b in [0, n) with cum(b) >= Lr; n when none.
 template <typename F>
@@ -173,9 +199,19 @@ __device__ __forceinline__ void pick_lambda(int nbins, double Lr, F cum, int &l1
 struct Packet {
     double x, y, z, vx, vy, vz, E, wl1;
     double tau0, tau, nEJv;
-    int i, jj, l1, edgeNr, ntrace, lastc, nstep;
+    int i, jj, l1, edgeNr, ntrace, nstep;
     bool onEdge, scatt, pol;
 };
+
+// Fixed-point Eabs units of energy E (2^EABS_BITS per nominal packet energy).
+__device__ __forceinline__ unsigned long long eabs_q(double E, double invE, unsigned long long *scnt) {
+    double qd = rint(E * invE);
+    if (!(qd < 0x1p62)) {
+        atomicAdd(&scnt[C_NOVERFLOW], 1ull);
+        qd = 0x1p62;
+    }
+    return (unsigned long long)qd;
+}
 
 // Fixed-point Eabs add; returns the cell's total before the add.
 __device__ __forceinline__ unsigned long long add_eabs(const DevTally &ty, int c, double E, double invE,
@@ -440,8 +476,7 @@ struct RingRng {
     }
 };
 
-#define INFLIGHT_DIV 64 // default cap on packets in flight: nphot / 64
-#define NPL 13          // per-thread LDS doubles (odd stride spreads the banks)
+#define NPL 19          // per-thread LDS doubles, 18 used (odd stride spreads the banks)
 #define COOP_LANES 6 // default: a wave serves lookups together when at most this many lanes need one
 
 // MODE 0: one dust species; 1: several, with per-class reemission tables;
@@ -488,6 +523,11 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
 #define P_IC pl[10]  // ic: the cell these entries belong to (-1: none)
 #define P_TC pl[11]  // 2*class + C%thick of cell ic
 #define P_KDIT pl[12] // kd_iT
+#define P_FS pl[13]   // p / P_k of the packet (0 in the first stage)
+#define P_BASE pl[14] // S_k(c) p / P_k of the visited cell, in Eabs units
+#define P_OWN pl[15]  // bits of the packet's own Eabs units in the visited cell
+#define P_VSLOT pl[16] // its slot in the lane's visit table (-1: none; +2^VIS_LOG2: own changed)
+#define P_TG pl[17]   // a guess of the visited cell's T as the packet sees it
     for (int k = threadIdx.x; k <= nr; k += blockDim.x) sR2[k] = md.R2[k];
     for (int k = threadIdx.x; k <= nt; k += blockDim.x) {
         sT2[k] = md.T2[k];
@@ -548,6 +588,13 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     P_KDIT = -1.0;
     P_TRW = 0.0;
     P_MASS = 0.0;
+    P_FS = 0.0;
+    P_BASE = 0.0;
+    P_OWN = 0.0;
+    P_VSLOT = -1.0;
+    P_TG = 0.0;
+    // this lane's visit table
+    uint4 *const vt = ty.vtab + ((size_t)(blockIdx.x * blockDim.x + threadIdx.x) << VIS_LOG2);
     double2 k1 = make_double2(0.0, 0.0), k2 = make_double2(0.0, 0.0);
     TallyAcc acc;
     acc.c = -1;
@@ -593,8 +640,13 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                         pid = mine;
                         g.pid = (unsigned long long)pid;
                         g.head = g.fill = 0;
-                        p.lastc = -1;
                         p.nstep = 0;
+                        // a new packet: its own visit table entries, its own
+                        // view of the cells (P_k = 0 in the first stage)
+                        cc = -1;
+                        P_IC = -1.0;
+                        P_VSLOT = -1.0;
+                        P_FS = (A.Pk > 0) ? (double)pid / (double)A.Pk : 0.0;
                         st = S_EMIT;
                     }
                 }
@@ -696,10 +748,21 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
             const bool newc = (c != cc), newk = (c != kc || p.l1 != kl);
             double nd = 0.0, n1 = 0.0, n2 = 0.0;
             double2 nk1 = k1, nk2 = k2;
+            uint4 ve = make_uint4(0u, 0u, 0u, 0u);
+            unsigned vh = 0u;
+            unsigned long long nS = 0ull;
+            float nTg = 0.f;
             if (newc) {
                 nd = md.dens[c];
                 n1 = ty.invU1[c];
                 n2 = ty.invU2[c];
+                // c's slot of the visit table, and the stage's snapshot
+                vh = ((unsigned)c * 0x9E3779B1u) >> (32 - VIS_LOG2);
+                ve = vt[vh];
+                if (!A.serial) {
+                    nS = ty.Esnap[c];
+                    nTg = ty.Tst[c];
+                }
             }
             if (newk) {
                 nk1 = md.KT[(size_t)c * nlam + p.l1];
@@ -710,6 +773,15 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
             bool ok = exit_face(sR2, sT2, nt, p.x, p.y, p.z, p.vx, p.vy, p.vz, p.i, p.jj, p.onEdge, p.edgeNr, v,
                                 inext, jnext);
             if (newc) {
+                // the previous visit's own energy back to its slot
+                {
+                    const int vs = (int)P_VSLOT;
+                    if (vs >= (1 << VIS_LOG2)) {
+                        const unsigned long long o = (unsigned long long)__double_as_longlong(P_OWN);
+                        vt[vs - (1 << VIS_LOG2)] = make_uint4((unsigned)(pid + 1), (unsigned)cc, (unsigned)o,
+                                                              (unsigned)(o >> 32));
+                    }
+                }
                 acc_flush(ty, acc);
                 cc = c;
                 acc.c = c;
@@ -717,17 +789,41 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 c_dens = nd;
                 c_u1 = n1;
                 c_u2 = n2;
+                // this packet's entry for c: found, or inserted (its first
+                // entry into c: Ni += 1, MCRadiation.f:943-946)
+                const unsigned tag = (unsigned)(pid + 1);
+                int slot = -1;
+                unsigned long long own = 0ull;
+                for (int k = 0; k < VIS_PROBE; k++) {
+                    if (ve.x != tag) {
+                        vt[vh] = make_uint4(tag, (unsigned)c, 0u, 0u);
+                        atomicAdd(&ty.Ni[c], 1);
+                        slot = (int)vh;
+                        break;
+                    }
+                    if (ve.y == (unsigned)c) {
+                        own = ((unsigned long long)ve.w << 32) | ve.z;
+                        slot = (int)vh;
+                        break;
+                    }
+                    vh = (vh + 1u) & ((1u << VIS_LOG2) - 1u);
+                    ve = vt[vh];
+                }
+                if (slot < 0) {
+                    // table full along the probe: counted as a first entry
+                    CNT(C_NVISOVF);
+                    atomicAdd(&ty.Ni[c], 1);
+                }
+                P_VSLOT = (double)slot;
+                P_OWN = __longlong_as_double((long long)own);
+                P_BASE = (double)nS * P_FS;
+                P_TG = (nTg > 0.f) ? (double)(nTg * __powf((float)P_FS, 0.2f)) : 0.0;
             }
             if (newk) {
                 k1 = nk1;
                 k2 = nk2;
                 kc = c;
                 kl = p.l1;
-            }
-            if (c != p.lastc) {
-                long long rel = pid - A.pid_base;
-                atomicOr(&ty.bitmap[(size_t)c * A.nwords + (size_t)(rel >> 5)], 1u << (rel & 31));
-                p.lastc = c;
             }
             const double wl2 = 1.0 - p.wl1;
             double Kabs = k1.x * p.wl1 + k2.x * wl2;
@@ -798,6 +894,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 P_MASS = md.mass[c];
                 P_TC = A.rw.on ? (double)(A.rw.thick[c] | (A.rw.cls[c] << 1)) : 0.0;
                 P_KDIT = -1.0;
+                if (!A.serial) P_TRW = -1.0; // not known yet for this cell
             }
             const int l1 = p.l1, l2 = p.l1 + 1;
             const double wl2 = 1.0 - p.wl1;
@@ -818,31 +915,60 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 double Ex = (p.E * (KabsL - 0.0)) / KabsL;
                 const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
                 const double *Kp0 = tb.Kp; // species 0
-                // The cell's T after this lane's previous absorption in it is
-                // the likely start: fetch that window of the tables together
-                // with the cell's energy (one round trip instead of several).
-                int jw = (c_Tl >= 1.0) ? (int)(c_Tl / 1.0) : -1;
-                if (jw > TMAXD - 2) jw = TMAXD - 2;
-                double w0 = 0.0, w1 = 0.0, w2 = 0.0, q0 = 0.0, q1 = 0.0, q2 = 0.0;
-                if (jw >= 1) {
-                    w0 = kpc[jw];
-                    w1 = kpc[jw + 1];
-                    w2 = kpc[jw + 2];
-                    if (ONE) {
-                        q0 = Kp0[jw];
-                        q1 = Kp0[jw + 1];
-                        q2 = Kp0[jw + 2];
-                    }
-                }
                 const double mass = P_MASS;
-                const double Thint = __longlong_as_double((long long)__hip_atomic_load(
-                    &ty.Tcell[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-                unsigned long long q;
-                unsigned long long old = add_eabs(ty, c, Ex, A.invE, q, scnt);
-                if (jw < 1) {
-                    jw = (int)(Thint / 1.0);
-                    if (jw < 1) jw = 1;
+                int jw;
+                double w0, w1, w2, q0 = 0.0, q1 = 0.0, q2 = 0.0, T0 = 0.0, T1;
+                if (A.serial) {
+                    // The cell's T after this lane's previous absorption in
+                    // it is the likely start: fetch that window of the tables
+                    // together with the cell's energy (one round trip
+                    // instead of several).
+                    jw = (c_Tl >= 1.0) ? (int)(c_Tl / 1.0) : -1;
                     if (jw > TMAXD - 2) jw = TMAXD - 2;
+                    w0 = w1 = w2 = 0.0;
+                    if (jw >= 1) {
+                        w0 = kpc[jw];
+                        w1 = kpc[jw + 1];
+                        w2 = kpc[jw + 2];
+                        if (ONE) {
+                            q0 = Kp0[jw];
+                            q1 = Kp0[jw + 1];
+                            q2 = Kp0[jw + 2];
+                        }
+                    }
+                    const double Thint = __longlong_as_double((long long)__hip_atomic_load(
+                        &ty.Tcell[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                    unsigned long long q;
+                    unsigned long long old = add_eabs(ty, c, Ex, A.invE, q, scnt);
+                    if (jw < 1) {
+                        jw = (int)(Thint / 1.0);
+                        if (jw < 1) jw = 1;
+                        if (jw > TMAXD - 2) jw = TMAXD - 2;
+                        w0 = kpc[jw];
+                        w1 = kpc[jw + 1];
+                        w2 = kpc[jw + 2];
+                        if (ONE) {
+                            q0 = Kp0[jw];
+                            q1 = Kp0[jw + 1];
+                            q2 = Kp0[jw + 2];
+                        }
+                    }
+                    if (old != 0ull) T0 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)old * A.unitE) / mass, Thint);
+                    T1 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)(old + q) * A.unitE) / mass, T0);
+                    atomicMax(&ty.Tcell[c], (unsigned long long)__double_as_longlong(T1));
+                    P_TRW = (T1 > Thint) ? T1 : Thint; // the cell's T after this update
+                } else {
+                    // staged: the slice (S_k p/P_k + own, + q) of this
+                    // packet's view of the cell; the add itself returns nothing
+                    // anyone waits for
+                    const unsigned long long q = eabs_q(Ex, A.invE, scnt);
+                    atomicAdd(&ty.Eabs[c], q);
+                    const unsigned long long o0 = (unsigned long long)__double_as_longlong(P_OWN), o1 = o0 + q;
+                    P_OWN = __longlong_as_double((long long)o1);
+                    if (P_VSLOT >= 0.0 && P_VSLOT < (double)(1 << VIS_LOG2)) P_VSLOT += (double)(1 << VIS_LOG2);
+                    const double base = P_BASE;
+                    const double Eb = (base + (double)o0) * A.unitE, E1 = (base + (double)o1) * A.unitE;
+                    jw = win_start((c_Tl >= 1.0) ? c_Tl : P_TG);
                     w0 = kpc[jw];
                     w1 = kpc[jw + 1];
                     w2 = kpc[jw + 2];
@@ -851,13 +977,11 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                         q1 = Kp0[jw + 1];
                         q2 = Kp0[jw + 2];
                     }
+                    if (Eb != 0.0) T0 = dev_T_win(kpc, jw, w0, w1, w2, Eb / mass);
+                    T1 = dev_T_win(kpc, jw, w0, w1, w2, E1 / mass);
+                    P_TRW = T1;
                 }
-                double T0 = 0.0;
-                if (old != 0ull) T0 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)old * A.unitE) / mass, Thint);
-                double T1 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)(old + q) * A.unitE) / mass, T0);
                 c_Tl = T1;
-                atomicMax(&ty.Tcell[c], (unsigned long long)__double_as_longlong(T1));
-                P_TRW = (T1 > Thint) ? T1 : Thint; // the cell's T after this update
                 int iT0 = (int)(T0 / 1.0), iT1 = (int)(T1 / 1.0);
                 double e0 = T0 - (double)(float)iT0;
                 double e1 = T1 - (double)(float)iT1;
@@ -924,9 +1048,23 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
             } else {
                 CNT(C_NSCAT);
                 p.scatt = true;
-                if ((int)P_TC & 1)
-                    P_TRW = __longlong_as_double((long long)__hip_atomic_load(&ty.Tcell[c], __ATOMIC_RELAXED,
-                                                                               __HIP_MEMORY_SCOPE_AGENT));
+                if ((int)P_TC & 1) {
+                    if (A.serial) {
+                        P_TRW = __longlong_as_double((long long)__hip_atomic_load(&ty.Tcell[c], __ATOMIC_RELAXED,
+                                                                                   __HIP_MEMORY_SCOPE_AGENT));
+                    } else if (P_TRW < 0.0) {
+                        // the cell's T as this packet sees it (random-walk test)
+                        const double Eb =
+                            (P_BASE + (double)(unsigned long long)__double_as_longlong(P_OWN)) * A.unitE;
+                        double Tv = 0.0;
+                        if (Eb != 0.0) {
+                            const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
+                            const int jw = win_start((c_Tl >= 1.0) ? c_Tl : P_TG);
+                            Tv = dev_T_win(kpc, jw, kpc[jw], kpc[jw + 1], kpc[jw + 2], Eb / P_MASS);
+                        }
+                        P_TRW = Tv;
+                    }
+                }
                 if (tb.scat_how == 1) {
                     random_direction(g, p.vx, p.vy, p.vz);
                     st = S_AFTER;
@@ -1200,13 +1338,23 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 p.y = p.y + dmin * p.vy;
                 p.z = p.z + dmin * p.vz;
                 const double ejv = ((p.E * v) * AU) * kd.y;
-                unsigned long long q;
-                const unsigned long long old = add_eabs(ty, c, ejv * c_dens, A.invE, q, scnt);
                 acc_ejv(acc, ejv, c_u1, scnt);
                 p.nEJv = p.nEJv + ejv; // the module-level nEJv of RandomWalk.f:115
-                const double T1 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)(old + q) * A.unitE) / P_MASS, Thint);
-                atomicMax(&ty.Tcell[c], (unsigned long long)__double_as_longlong(T1));
-                P_TRW = (T1 > Thint) ? T1 : Thint;
+                if (A.serial) {
+                    unsigned long long q;
+                    const unsigned long long old = add_eabs(ty, c, ejv * c_dens, A.invE, q, scnt);
+                    const double T1 =
+                        dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)(old + q) * A.unitE) / P_MASS, Thint);
+                    atomicMax(&ty.Tcell[c], (unsigned long long)__double_as_longlong(T1));
+                    P_TRW = (T1 > Thint) ? T1 : Thint;
+                } else {
+                    const unsigned long long q = eabs_q(ejv * c_dens, A.invE, scnt);
+                    atomicAdd(&ty.Eabs[c], q);
+                    const unsigned long long o1 = (unsigned long long)__double_as_longlong(P_OWN) + q;
+                    P_OWN = __longlong_as_double((long long)o1);
+                    if (P_VSLOT >= 0.0 && P_VSLOT < (double)(1 << VIS_LOG2)) P_VSLOT += (double)(1 << VIS_LOG2);
+                    P_TRW = dev_T_win(kpc, jw, w0, w1, w2, ((P_BASE + (double)o1) * A.unitE) / P_MASS);
+                }
                 tautot += v / dmin;
                 // leave in a direction of the forward hemisphere (MCRadiation.f:919-925)
                 double vx, vy, vz;
@@ -1261,6 +1409,11 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
 #undef P_IC
 #undef P_TC
 #undef P_KDIT
+#undef P_FS
+#undef P_BASE
+#undef P_OWN
+#undef P_VSLOT
+#undef P_TG
 }
 
 // ---------------------------------------------------------------------------
@@ -1371,33 +1524,39 @@ __global__ void k_diffcoeff(int nclass, int nlam, int ng, int scattering, const 
     KD[idx] = make_double2(i2 / i1, i3 / i2);
 }
 
-__global__ void k_popcount(int ncell, int nwords, const unsigned *__restrict__ bitmap, int *Ni) {
-    int c = blockIdx.x;
+// the stage's snapshot: Eabs of the packets before it, and the temperature
+// it gives each cell (a start for the temperature searches of the stage)
+__global__ void k_stage_snap(int ncell, const unsigned long long *__restrict__ Eabs, const double *__restrict__ KpCell,
+                             const double *__restrict__ mass, double unitE, unsigned long long *Esnap, float *Tst) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
-    const unsigned *row = bitmap + (size_t)c * nwords;
-    int s = 0;
-    for (int k = threadIdx.x; k < nwords; k += blockDim.x) s += __popc(row[k]);
-    __shared__ int red[256];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) Ni[c] += red[0];
+    const unsigned long long e = Eabs[c];
+    Esnap[c] = e;
+    Tst[c] = e ? (float)dev_determineT(KpCell + (size_t)c * (TMAXD + 1), ((double)e * unitE) / mass[c], 1.0) : 0.f;
 }
 
+__global__ void k_fill_u4(size_t n, uint4 *p) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        p[k] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+
+// raw tallies from the fixed-point sums; Tbw = the cell's Bjorkman-Wood T
+// after the pass, T(Eabs/m) (the serial loop's last increaseT)
 __global__ void k_finalize(int ncell, const unsigned long long *EJvq, const unsigned long long *EJv2q,
                            const unsigned long long *Eabsq, const double *invU1, const double *invU2, double unitE,
-                           const unsigned long long *Tbits, double *EJv, double *EJv2, double *Eabs, double *Tbw) {
+                           const double *__restrict__ KpCell, const double *__restrict__ mass, double *EJv,
+                           double *EJv2, double *Eabs, double *Tbw) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     EJv[c] = (double)EJvq[c] / invU1[c];
     const unsigned long long *L = EJv2q + 3 * (size_t)c;
     double s = ((double)L[2] * 0x1p64 + (double)L[1] * 0x1p32) + (double)L[0];
     EJv2[c] = s / invU2[c];
-    Eabs[c] = (double)Eabsq[c] * unitE;
-    Tbw[c] = __longlong_as_double((long long)Tbits[c]);
+    const unsigned long long e = Eabsq[c];
+    Eabs[c] = (double)e * unitE;
+    Tbw[c] = e ? dev_determineT(KpCell + (size_t)c * (TMAXD + 1), ((double)e * unitE) / mass[c], 1.0) : 0.0;
 }
 
 // Main.f:498-549 per cell; void cell temperature Main.f:546-548.
@@ -1519,8 +1678,8 @@ extern "C" int mcx_set_tables(mcx_ctx *ctx, const mcx_tables *t) {
     free_list(ctx->run_alloc);
     free_list(ctx->mod_alloc);
     free_list(ctx->tab_alloc);
-    ctx->bitmap_bytes = 0;
-    ctx->ty.bitmap = nullptr;
+    ctx->vtab_lanes = 0;
+    ctx->ty.vtab = nullptr;
     ctx->have_model = ctx->have_pass = false;
     const int nlam = t->nlam, ng = t->ng, nb = nlam - 1;
     ctx->nlam = nlam;
@@ -1643,8 +1802,8 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     free_list(ctx->run_alloc);
     free_list(ctx->mod_alloc);
-    ctx->bitmap_bytes = 0;
-    ctx->ty.bitmap = nullptr;
+    ctx->vtab_lanes = 0;
+    ctx->ty.vtab = nullptr;
     ctx->have_model = ctx->have_pass = false;
     const int nr = gr->nr, nt = gr->nt, ncell = nr * nt, ng = ctx->ng, nlam = ctx->nlam;
     std::vector<double> R2(nr + 1), T2(nt + 1), mass(ncell);
@@ -1784,6 +1943,8 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
     if ((r = dalloc(ctx, ctx->mod_alloc, &ty.counters, G_NSLOTS))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ty.tautot, 1))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Ni, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Esnap, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Tst, ncell))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_EJv, ncell))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_EJv2, ncell))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Eabs, ncell))) return r;
@@ -1797,33 +1958,28 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
     return MCX_OK;
 }
 
+// the end of the stage that starts at packet P: ratio x P (ratio >= 2), or
+// with ratio -1 the growing ratio min(16, 1 + sqrt(P / stage0)); at most N
+static long long stage_next(long long P, long long N, long long stage0, int ratio) {
+    double r = (ratio >= 2) ? (double)ratio : 1.0 + sqrt((double)P / (double)stage0);
+    if (r > 16.0) r = 16.0;
+    const double nx = ceil((double)P * r);
+    return (nx >= (double)N) ? N : (long long)nx;
+}
+
 extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     if (!ctx || !run) return MCX_EINVAL;
     if (!ctx->have_model) return fail(ctx, MCX_ESTATE, "mcx_set_model first");
-    if (run->nphot <= 0) return fail(ctx, MCX_EINVAL, "nphot must be positive");
+    if (run->nphot <= 0 || run->nphot >= (1ll << 31)) return fail(ctx, MCX_EINVAL, "nphot must be in 1 .. 2^31-1");
+    if (run->precision != MCX_FP64) return fail(ctx, MCX_EINVAL, "unknown precision mode");
+    if (run->stage0 < 0 || run->stage_ratio < -1 || run->stage_ratio == 1)
+        return fail(ctx, MCX_EINVAL, "stage0 >= 0 and stage_ratio -1, 0 or >= 2 required");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const DevModel &md = ctx->md;
     DevTally &ty = ctx->ty;
     const int ncell = md.ncell, nlam = ctx->nlam;
     hipStream_t s = ctx->stream;
-    // distinct-packet bitmap: packets are run in batches whose bitmap fits
-    long long cap = run->bitmap_bytes_max > 0 ? run->bitmap_bytes_max : (32ll << 30);
-    long long per_packet_bytes = (long long)ncell; // ncell bits per 8 packets -> ncell/8 bytes per packet
-    long long batch = run->nphot;
-    long long max_batch = (cap * 8ll) / (per_packet_bytes > 0 ? per_packet_bytes : 1);
-    max_batch = (max_batch / 32) * 32;
-    if (max_batch < 32) max_batch = 32;
-    if (batch > max_batch) batch = max_batch;
-    int nwords = (int)((batch + 31) / 32);
-    size_t need = (size_t)ncell * (size_t)nwords * sizeof(unsigned);
-    if (need > ctx->bitmap_bytes) {
-        HIPCHK(ctx, hipStreamSynchronize(s));
-        free_list(ctx->run_alloc);
-        ty.bitmap = nullptr;
-        int r = dalloc(ctx, ctx->run_alloc, &ty.bitmap, need / sizeof(unsigned));
-        if (r) return r;
-        ctx->bitmap_bytes = need;
-    }
+    const bool serial = (run->nlanes == 1);
     DevRW rw = ctx->rw;
     rw.on = 0;
     if (run->randomwalk) {
@@ -1843,7 +1999,43 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     const double E0 = ctx->Lstar / ((double)(float)run->nphot * ((1.0 - 0.0) - 0.0));
     ctx->E0 = E0;
     ctx->unitE = ldexp(E0, -EABS_BITS);
+    // the kernel for the model's species (one; several with class tables; several)
+    void (*kern)(TransportArgs) = (ctx->ng == 1) ? k_transport<0> : md.CUMmix ? k_transport<1> : k_transport<2>;
+    // launch shape: a full device of resident wavefronts, or one lane
+    int threads = serial ? 64 : 256; // one wavefront: lane 0 traces, the others serve its lookups
+    size_t lds = sizeof(double) * ((size_t)(md.nr + 1) + 2 * (size_t)(md.nt + 1) + 2 * 181 + (size_t)(nlam - 1) +
+                                   NPL * (size_t)threads);
+    if (rw.on) lds += sizeof(double) * ((size_t)(md.nr + 1) + (size_t)(md.nt + 1));
+    // rings of Philox draws: 16 per thread when they fit beside the rest
+    int rb = 16;
+    while (rb > 4 && lds + sizeof(double) * (size_t)rb * threads > ctx->lds_max) rb >>= 1;
+    lds += sizeof(double) * (size_t)rb * threads;
+    if (lds > ctx->lds_max)
+        return fail(ctx, MCX_EINVAL, "grid too large for the kernel's LDS tables (" + std::to_string(lds) + " bytes)");
+    int max_blocks = 1;
+    if (!serial) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        max_blocks = per_cu * ctx->sm_count;
+        if (run->nlanes > 1) max_blocks = std::max(1, std::min(max_blocks, (run->nlanes + threads - 1) / threads));
+    }
+    // the lanes' visit tables (cleared per pass: tags are packet numbers)
+    const size_t lanes = (size_t)max_blocks * threads;
+    if (lanes > ctx->vtab_lanes) {
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        free_list(ctx->run_alloc);
+        ty.vtab = nullptr;
+        ctx->vtab_lanes = 0;
+        int r = dalloc(ctx, ctx->run_alloc, &ty.vtab, lanes << VIS_LOG2);
+        if (r) return r;
+        ctx->vtab_lanes = lanes;
+    }
+    ty.Ni = ctx->d_Ni;
+    ty.Esnap = ctx->d_Esnap;
+    ty.Tst = ctx->d_Tst;
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
+    hipLaunchKernelGGL(k_fill_u4, dim3(1024), dim3(256), 0, s, lanes << VIS_LOG2, ty.vtab);
     HIPCHK(ctx, hipMemsetAsync(ty.EJv, 0, sizeof(unsigned long long) * ncell, s));
     HIPCHK(ctx, hipMemsetAsync(ty.EJv2, 0, sizeof(unsigned long long) * 3 * ncell, s));
     HIPCHK(ctx, hipMemsetAsync(ty.Eabs, 0, sizeof(unsigned long long) * ncell, s));
@@ -1856,37 +2048,6 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     hipLaunchKernelGGL(k_cell_units, dim3((ncell + 255) / 256), dim3(256), 0, s, md.nr, md.nt, nlam, md.R2, md.KT, E0,
                        (double *)ty.invU1, (double *)ty.invU2);
     HIPCHK(ctx, hipGetLastError());
-    // the kernel for the model's species (one; several with class tables; several)
-    void (*kern)(TransportArgs) = (ctx->ng == 1) ? k_transport<0> : md.CUMmix ? k_transport<1> : k_transport<2>;
-    // launch shape: a full device of resident wavefronts, or one lane
-    int threads = 256, blocks = 1;
-    if (run->nlanes == 1) threads = 64; // one wavefront: lane 0 traces, the others serve its lookups
-    size_t lds = sizeof(double) * ((size_t)(md.nr + 1) + 2 * (size_t)(md.nt + 1) + 2 * 181 + (size_t)(nlam - 1) +
-                                   NPL * (size_t)threads);
-    if (rw.on) lds += sizeof(double) * ((size_t)(md.nr + 1) + (size_t)(md.nt + 1));
-    // rings of Philox draws: 16 per thread when they fit beside the rest
-    int rb = 16;
-    while (rb > 4 && lds + sizeof(double) * (size_t)rb * threads > ctx->lds_max) rb >>= 1;
-    lds += sizeof(double) * (size_t)rb * threads;
-    if (lds > ctx->lds_max)
-        return fail(ctx, MCX_EINVAL, "grid too large for the kernel's LDS tables (" + std::to_string(lds) + " bytes)");
-    if (run->nlanes == 1) {
-        blocks = 1;
-    } else {
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) != hipSuccess || per_cu < 1)
-            per_cu = 1;
-        blocks = per_cu * ctx->sm_count;
-        // Packets in flight at once take their Bjorkman-Wood slices in
-        // arrival order rather than packet order.  With at most nphot/64 of
-        // them in flight the temperatures keep the serial loop's statistics;
-        // more concurrency biases thick unresolved cells.
-        long long cap = (run->nlanes > 1) ? run->nlanes : run->nphot / INFLIGHT_DIV;
-        long long cap_blocks = (cap + threads - 1) / threads;
-        if (cap_blocks < 1) cap_blocks = 1;
-        if (cap_blocks < blocks) blocks = (int)cap_blocks;
-        if (blocks < 1) blocks = 1;
-    }
     TransportArgs A;
     A.tb = ctx->tb;
     A.md = md;
@@ -1898,34 +2059,47 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     A.maxinteract = run->maxinteract;
     A.invE = ldexp(1.0, EABS_BITS) / E0;
     A.unitE = ctx->unitE;
-    A.nwords = nwords;
-    A.serial = (run->nlanes == 1);
+    A.serial = serial ? 1 : 0;
     A.coop = (run->coop_lanes < 0) ? COOP_LANES : run->coop_lanes;
     A.rb_mask = rb - 1;
+    // stages: [0, stage0), then each ends at ratio x its start; one launch
+    // for the whole pass when one lane runs it
+    const long long N = run->nphot;
+    const long long s0 = run->stage0 > 0 ? run->stage0 : 64;
+    const int ratio = run->stage_ratio == 0 ? 2 : run->stage_ratio;
     float kms_total = 0.f;
-    for (long long base = 0; base < run->nphot; base += batch) {
-        long long end = base + batch;
-        if (end > run->nphot) end = run->nphot;
-        A.pid_base = base;
-        A.pid_end = end;
-        unsigned long long b0 = (unsigned long long)base;
-        HIPCHK(ctx, hipMemcpyAsync(ctx->d_next, &b0, sizeof(b0), hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipMemsetAsync(ty.bitmap, 0, (size_t)ncell * nwords * sizeof(unsigned), s));
+    // MCX_STAGE_LOG set: one line per stage on stderr (its packets and time)
+    static const bool stage_log = getenv("MCX_STAGE_LOG") != nullptr;
+    long long Pa = 0, Pb = serial ? N : std::min(N, s0);
+    while (Pa < N) {
+        if (!serial) {
+            hipLaunchKernelGGL(k_stage_snap, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell, ty.Eabs, md.KpCell,
+                               md.mass, ctx->unitE, ctx->d_Esnap, ctx->d_Tst);
+            HIPCHK(ctx, hipGetLastError());
+        }
+        A.Pk = serial ? 0 : Pa;
+        A.pid_base = Pa;
+        A.pid_end = Pb;
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, ctx->d_next, (unsigned long long)Pa);
         HIPCHK(ctx, hipMemsetAsync(ty.counters + G_DONE, 0, sizeof(unsigned long long), s));
         HIPCHK(ctx, hipMemsetAsync(ty.counters + G_TLINE, 0, sizeof(unsigned long long) * 6, s));
+        const int blocks =
+            serial ? 1 : (int)std::max(1ll, std::min((long long)max_blocks, (Pb - Pa + threads - 1) / threads));
         HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, s, A);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(ctx->ev[2], s));
-        hipLaunchKernelGGL(k_popcount, dim3(ncell), dim3(256), 0, s, ncell, nwords, ty.bitmap, ctx->d_Ni);
-        HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventSynchronize(ctx->ev[2]));
         float ms = 0.f;
         HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
         kms_total += ms;
+        if (stage_log) fprintf(stderr, "[mcx] stage [%lld, %lld): %.1f ms\n", Pa, Pb, ms);
+        Pa = Pb;
+        Pb = stage_next(Pb, N, s0, ratio);
     }
     hipLaunchKernelGGL(k_finalize, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell, ty.EJv, ty.EJv2, ty.Eabs,
-                       ty.invU1, ty.invU2, ctx->unitE, ty.Tcell, ctx->d_EJv, ctx->d_EJv2, ctx->d_Eabs, ctx->d_Tbw);
+                       ty.invU1, ty.invU2, ctx->unitE, md.KpCell, md.mass, ctx->d_EJv, ctx->d_EJv2, ctx->d_Eabs,
+                       ctx->d_Tbw);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ctx->ev[3], s));
     HIPCHK(ctx, hipEventSynchronize(ctx->ev[3]));
@@ -2000,6 +2174,7 @@ extern "C" int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *o) {
     o->tautot = tt;
     o->Eescape = Eesc;
     o->maxsteps = (int64_t)cnt[G_MAXSTEPS];
+    o->nvisit_overflow = (int64_t)cnt[C_NVISOVF];
     for (int k = 0; k < 5; k++) {
         const long long t0 = (long long)cnt[G_TLINE], t1 = (long long)cnt[G_TLINE + 1 + k];
         o->t_done[k] = (t1 != 0 && ctx->wclk_khz > 0) ? (double)(t1 - t0) / (double)ctx->wclk_khz : -1.0;

@@ -26,7 +26,14 @@
 #define EABS_BITS 24  // one nominal packet energy maps to 2^24
 
 enum { C_NEMIT = 0, C_NINTERACT, C_NABS, C_NSCAT, C_NESCAPE, C_NSTARHIT, C_NSTEPS, C_NRAAR, C_NMAXINT,
-       C_NGEOM, C_NOVERFLOW, C_NRW, C_NCOUNT };
+       C_NGEOM, C_NOVERFLOW, C_NRW, C_NVISOVF, C_NCOUNT };
+
+// per-lane table of the cells the lane's current packet has entered (Ni's
+// distinct-packet count and the packet's own absorbed energy per cell):
+// 2^VIS_LOG2 slots of {tag = packet + 1, cell, own energy lo, hi}, linear
+// probing over at most VIS_PROBE slots
+#define VIS_LOG2 10
+#define VIS_PROBE 64
 
 // slots of DevTally::counters beyond the event counters: the longest packet
 // history (cell steps), packets finished in the current launch, and wall-clock
@@ -71,8 +78,11 @@ struct DevTally {
     unsigned long long *EJv2;   // [ncell*3] limbs of weight 2^0, 2^32, 2^64
     unsigned long long *Eabs;   // [ncell]
     const double *invU1, *invU2;
-    unsigned long long *Tcell;  // [ncell] bits of the latest BW T (monotone)
-    unsigned *bitmap;           // [ncell*nwords]
+    unsigned long long *Tcell;  // [ncell] bits of the latest BW T (monotone; one-lane passes)
+    int *Ni;                    // [ncell] distinct packets per cell
+    uint4 *vtab;                // [lanes << VIS_LOG2] visited cells of each lane's packet
+    const unsigned long long *Esnap; // [ncell] Eabs of the packets before the stage
+    const float *Tst;           // [ncell] T(Esnap/m) at the stage's start
     unsigned long long *specE;  // [2][(NANG+1)*nlam]
     unsigned *ispec;            // [2][(NANG+1)*nlam]
     unsigned long long *counters;
@@ -124,7 +134,9 @@ struct mcx_ctx {
     double Tvoid = 1.0;
     // run
     DevTally ty{};
-    size_t bitmap_bytes = 0;
+    size_t vtab_lanes = 0;      // lanes the visit table holds
+    unsigned long long *d_Esnap = nullptr;
+    float *d_Tst = nullptr;
     int *d_Ni = nullptr;
     double *d_EJv = nullptr, *d_EJv2 = nullptr, *d_Eabs = nullptr, *d_Tbw = nullptr;
     double *d_T = nullptr, *d_dT = nullptr, *d_dEJv = nullptr;

@@ -72,16 +72,18 @@ class Transport:
         self._check(self.lib.mcx_set_model(self.ctx, C.byref(g)))
         self.model = m
 
-    def transport(self, nphot, seed=1, nlanes=0, maxinteract=0, bitmap_bytes_max=0, randomwalk=None,
-                  factRW=None, coop_lanes=-1):
+    def transport(self, nphot, seed=1, nlanes=0, maxinteract=0, randomwalk=None, factRW=None, coop_lanes=-1,
+                  stage0=0, stage_ratio=0):
         """One MC pass (mcx_transport); tallies stay on the device.  The
-        random-walk switches default to the model's keywords."""
+        random-walk switches default to the model's keywords.  nlanes=1 runs
+        the serial loop on one lane; otherwise the staged schedule (stage0,
+        stage_ratio; 0 picks 64 and 2)."""
         m = self.model
         rw = bool(getattr(m, "randomwalk", False)) if randomwalk is None else bool(randomwalk)
         f = float(getattr(m, "factRW", 10.0)) if factRW is None else float(factRW)
         r = _lib.Run(nphot=int(nphot), seed=int(seed), maxinteract=int(maxinteract), nlanes=int(nlanes),
-                     bitmap_bytes_max=int(bitmap_bytes_max), randomwalk=int(rw), factRW=f,
-                     coop_lanes=int(coop_lanes))
+                     stage0=int(stage0), randomwalk=int(rw), factRW=f, coop_lanes=int(coop_lanes),
+                     stage_ratio=int(stage_ratio), precision=0)
         rc = self.lib.mcx_transport(self.ctx, C.byref(r))
         if rc not in (0, _lib.MCX_EGEOM):
             self._check(rc)
@@ -106,7 +108,8 @@ class Transport:
         for k in ("spec", "scatspec", "ispec", "iscatspec"):
             out[k] = out[k].reshape(NANGLE + 1, m.nlam)
         for k in ("nemit", "ninteract", "nabs", "nscat", "nescape", "nstarhit", "nsteps", "nraar",
-                  "nmaxinteract", "ngeomfail", "tautot", "Eescape", "nrw", "noverflow", "maxsteps"):
+                  "nmaxinteract", "ngeomfail", "tautot", "Eescape", "nrw", "noverflow", "maxsteps",
+                  "nvisit_overflow"):
             out[k] = getattr(t, k)
         out["t_done_ms"] = list(t.t_done)
         return out
@@ -287,10 +290,11 @@ def make_obs(angle_deg=45.0, nphi=45, nj0=2, nphot=1000, nphotstar=1000, nexits=
                tracescat=int(tracescat), pfstop=pfstop, seed=seed)
 
 
-def mc_radiation(tr, nphot, seed=1, nlanes=0, maxinteract=0, coop_lanes=-1):
+def mc_radiation(tr, nphot, seed=1, nlanes=0, maxinteract=0, coop_lanes=-1, stage0=0, stage_ratio=0):
     """MCRadiation + temperature inversion; returns a dict with the tallies
     and T, dT, dEJv."""
-    rc = tr.transport(nphot, seed=seed, nlanes=nlanes, maxinteract=maxinteract, coop_lanes=coop_lanes)
+    rc = tr.transport(nphot, seed=seed, nlanes=nlanes, maxinteract=maxinteract, coop_lanes=coop_lanes,
+                      stage0=stage0, stage_ratio=stage_ratio)
     out = tr.tallies()
     out["T"], out["dT"], out["dEJv"] = tr.temperature()
     out["rc"] = rc

@@ -377,6 +377,13 @@ typedef struct {
     double *KDext, *KDabs;
     int32_t *iTD;
     double ry[1000], rphi[1000];
+    /* staged schedule (orc_mcradiation): fixed-point Eabs, the stage's
+       snapshot, the current packet's own deposits per cell */
+    int staged;
+    uint64_t *Eq, *Sq, *own;
+    int64_t *own_pkt;
+    double fS, unitE, invE;
+    int64_t overflow;
 } ctx_t;
 
 /* Radiation.f:1007-1024 */
@@ -708,13 +715,17 @@ static double min_dist(const orc_model *m, const phot_t *p)
     return orc_min_dist(m, p->x, p->y, p->z, p->i, p->jj);
 }
 
+/* the staged schedule's cell state (defined after increaseT) */
+static double staged_T(const ctx_t *C, int c, int64_t pkt);
+static void staged_deposit(ctx_t *C, int c, int64_t pkt, double E, double *T0, double *T1);
+
 /* RandomWalk.f:47-147 (tcontact, no QHP/LRF) */
 static int random_walk(ctx_t *C, phot_t *p, double dmin)
 {
     const orc_model *m = C->m;
     orc_tally *t = C->t;
     int c = p->i * C->nt + p->jj;
-    int iT = (int)(t->T[c] / 1.0);
+    int iT = (int)((C->staged ? staged_T(C, c, p->nr) : t->T[c]) / 1.0);
     if (iT == 0) iT = 1;
     if (iT > TMAX) iT = TMAX;
     if (iT > TMAX - 1) iT = TMAX - 1; /* DiffCoeffCell clamps its argument */
@@ -735,11 +746,16 @@ static int random_walk(ctx_t *C, phot_t *p, double dmin)
     p->y = p->y + dmin * p->vy;
     p->z = p->z + dmin * p->vz;
     double EJv = (p->E * v) * AU_C;
-    t->Eabs[c] = t->Eabs[c] + (EJv * Kabs) * m->dens[c];
+    if (C->staged) {
+        double T0, T1;
+        staged_deposit(C, c, p->nr, (EJv * Kabs) * m->dens[c], &T0, &T1);
+    } else {
+        t->Eabs[c] = t->Eabs[c] + (EJv * Kabs) * m->dens[c];
+    }
     t->EJv[c] = t->EJv[c] + EJv * Kabs;
     C->nEJv = C->nEJv + EJv * Kabs;
     if (t->nev1) t->nev1[c]++;
-    t->T[c] = increaseT(C, c, t->Eabs[c] / C->mass[c], t->T[c]);
+    if (!C->staged) t->T[c] = increaseT(C, c, t->Eabs[c] / C->mass[c], t->T[c]);
     t->tautot = t->tautot + v / dmin;
     return 1;
 }
@@ -918,6 +934,47 @@ static double increaseT(const ctx_t *C, int c, double E1, double Tcell)
     return determineT_ctx(C, c, E1, Tcell);
 }
 
+/* ---- the staged schedule (see orc_mcradiation in mcx_oracle.h) -------- */
+/* T of energy per gram E1: the value increaseT and determineT both reach
+ * (the bracket Kp(i) <= E1 <= Kp(i+1) and its T^4 interpolation), whatever
+ * their starting temperature */
+static double T_of(const ctx_t *C, int c, double E1) { return determineT_ctx(C, c, E1, 1.0); }
+
+static uint64_t own_of(const ctx_t *C, int c, int64_t pkt) { return (C->own_pkt[c] == pkt) ? C->own[c] : 0; }
+
+/* E_b = (S(c) p/P_k + own_p(c)) unitE */
+static double staged_E(const ctx_t *C, int c, uint64_t own)
+{
+    return ((double)C->Sq[c] * C->fS + (double)own) * C->unitE;
+}
+
+/* the cell T packet pkt sees (0 before any energy, as the serial loop's
+ * C%T before the cell's first absorption) */
+static double staged_T(const ctx_t *C, int c, int64_t pkt)
+{
+    double Eb = staged_E(C, c, own_of(C, c, pkt));
+    return (Eb != 0.0) ? T_of(C, c, Eb / C->mass[c]) : 0.0;
+}
+
+/* deposit E in cell c: the device's fixed-point add (2^24 units per E0),
+ * then T before and after the deposit as packet pkt sees them */
+static void staged_deposit(ctx_t *C, int c, int64_t pkt, double E, double *T0, double *T1)
+{
+    double qd = rint(E * C->invE);
+    if (!(qd < 0x1p62)) {
+        C->overflow++;
+        qd = 0x1p62;
+    }
+    uint64_t q = (uint64_t)qd;
+    uint64_t o = own_of(C, c, pkt);
+    C->Eq[c] += q;
+    C->own_pkt[c] = pkt;
+    C->own[c] = o + q;
+    double Eb = staged_E(C, c, o), E1 = staged_E(C, c, o + q);
+    *T0 = (Eb != 0.0) ? T_of(C, c, Eb / C->mass[c]) : 0.0;
+    *T1 = T_of(C, c, E1 / C->mass[c]);
+}
+
 /* Radiation.f:506-534 */
 static void rotateStokes(phot_t *p, double x, double y, double z)
 {
@@ -1034,9 +1091,14 @@ static void interact(ctx_t *C, phot_t *p)
         p->scatt = 0;
         p->pol = 0;
         p->Q = 0.0; p->U = 0.0; p->V = 0.0;
-        t->Eabs[c] = t->Eabs[c] + (Edust * (KabsTotL - 0.0)) / KabsTotL;
-        double T0 = t->T[c];
-        double T1 = increaseT(C, c, t->Eabs[c] / C->mass[c], t->T[c]);
+        double T0, T1;
+        if (C->staged) {
+            staged_deposit(C, c, p->nr, (Edust * (KabsTotL - 0.0)) / KabsTotL, &T0, &T1);
+        } else {
+            t->Eabs[c] = t->Eabs[c] + (Edust * (KabsTotL - 0.0)) / KabsTotL;
+            T0 = t->T[c];
+            T1 = increaseT(C, c, t->Eabs[c] / C->mass[c], t->T[c]);
+        }
         int iT0 = (int)(T0 / 1.0), iT1 = (int)(T1 / 1.0);
         double epsT0 = T0 - (double)(float)iT0 * 1.0;
         double epsT1 = T1 - (double)(float)iT1 * 1.0;
@@ -1070,7 +1132,7 @@ static void interact(ctx_t *C, phot_t *p)
             kp = 0.0 / KabsTotL;
         }
         emit(C, p, C->spec, kp);
-        t->T[c] = T1;
+        if (!C->staged) t->T[c] = T1;
     } else {
         t->nscat++;
         p->scatt = 1;
@@ -1242,6 +1304,52 @@ static int interleaved_loop(ctx_t *C, const orc_run *run, const double *wangle, 
     return err;
 }
 
+/* the end of the stage that starts at P: ratio x P (ratio >= 2), or with
+ * ratio -1 the growing ratio min(16, 1 + sqrt(P / stage0)) (2 for the first
+ * stages, 16 once P >= 225 stage0); at most N */
+int64_t orc_stage_next(int64_t P, int64_t N, int64_t stage0, int ratio)
+{
+    double r = (ratio >= 2) ? (double)ratio : 1.0 + sqrt((double)P / (double)stage0);
+    if (r > 16.0) r = 16.0;
+    double nx = ceil((double)P * r);
+    return (nx >= (double)N) ? N : (int64_t)nx;
+}
+
+/* one packet of the loop MCRadiation.f:320-648 (goto 3 on a star hit) */
+static int run_packet(ctx_t *C, const orc_run *run, phot_t *pp, int64_t ip, const double *wangle,
+                      const double *wlam)
+{
+    orc_tally *t = C->t;
+    const int64_t Nphot = run->nphot;
+    phot_t p = *pp; /* the reference's phot persists from packet to packet */
+    p.nr = ip;
+    if (run->rng == 1) { C->g.pid = (uint64_t)(ip - 1); C->g.k = 0; }
+    if (t->pkt_steps) t->pkt_steps[ip - 1] = -t->nsteps;
+label3:
+    C->nEJv = 0.0;
+    emit_star(C, &p, Nphot);
+    double tau = -log(ran2(&C->g));
+    for (;;) {
+        int escape, hitstar;
+        if (trace2d(C, &p, tau, &escape, &hitstar) != 0) { *pp = p; return -1; }
+        if (hitstar) { t->nstarhit++; goto label3; }
+        if (escape) {
+            tally_escape(C, &p, wangle, wlam);
+            break;
+        }
+        t->ninteract++;
+        interact(C, &p);
+        if (run->maxinteract > 0) {
+            if (ran2(&C->g) < (1.0 / (double)(float)run->maxinteract)) { t->nmaxinteract++; break; }
+            p.E = p.E * (double)((float)run->maxinteract / (float)(run->maxinteract - 1));
+        }
+        tau = -log(ran2(&C->g));
+    }
+    if (t->pkt_steps) t->pkt_steps[ip - 1] += t->nsteps;
+    *pp = p;
+    return 0;
+}
+
 int orc_mcradiation(const orc_model *m, const orc_run *run, orc_ran1_state *rs, orc_tally *t)
 {
     ctx_t C;
@@ -1306,38 +1414,42 @@ int orc_mcradiation(const orc_model *m, const orc_run *run, orc_ran1_state *rs, 
     t->Eescape = 0.0;
     int err = 0;
     int64_t Nphot = run->nphot;
+    phot_t p;
+    memset(&p, 0, sizeof(p));
     if (run->inflight > 1) {
         if (run->rng != 1) return -4;
         err = interleaved_loop(&C, run, wangle, wlam);
-    } else {
-        phot_t p;
-        memset(&p, 0, sizeof(p));
-        for (int64_t ip = 1; ip <= Nphot && !err; ip++) {
-            p.nr = ip;
-            if (run->rng == 1) { C.g.pid = (uint64_t)(ip - 1); C.g.k = 0; }
-            if (t->pkt_steps) t->pkt_steps[ip - 1] = -t->nsteps;
-        label3:
-            C.nEJv = 0.0;
-            emit_star(&C, &p, Nphot);
-            double tau = -log(ran2(&C.g));
-            for (;;) {
-                int escape, hitstar;
-                if (trace2d(&C, &p, tau, &escape, &hitstar) != 0) { err = -1; break; }
-                if (hitstar) { t->nstarhit++; goto label3; }
-                if (escape) {
-                    tally_escape(&C, &p, wangle, wlam);
-                    break;
-                }
-                t->ninteract++;
-                interact(&C, &p);
-                if (run->maxinteract > 0) {
-                    if (ran2(&C.g) < (1.0 / (double)(float)run->maxinteract)) { t->nmaxinteract++; break; }
-                    p.E = p.E * (double)((float)run->maxinteract / (float)(run->maxinteract - 1));
-                }
-                tau = -log(ran2(&C.g));
+    } else if (run->staged) {
+        if (run->rng != 1) return -4;
+        C.staged = 1;
+        C.Eq = (uint64_t *)calloc(ncell, sizeof(uint64_t));
+        C.Sq = (uint64_t *)calloc(ncell, sizeof(uint64_t));
+        C.own = (uint64_t *)calloc(ncell, sizeof(uint64_t));
+        C.own_pkt = (int64_t *)calloc(ncell, sizeof(int64_t));
+        const double E0 = m->Lstar / ((double)(float)Nphot * ((1.0 - 0.0) - 0.0));
+        C.invE = ldexp(1.0, 24) / E0;
+        C.unitE = ldexp(E0, -24);
+        const int64_t s0 = run->stage0 > 0 ? run->stage0 : 64;
+        const int ratio = run->stage_ratio == 0 ? 2 : run->stage_ratio; /* -1: the growing ratio */
+        int64_t Pa = 0, Pb = s0 < Nphot ? s0 : Nphot;
+        while (Pa < Nphot && !err) {
+            memcpy(C.Sq, C.Eq, sizeof(uint64_t) * (size_t)ncell);
+            for (int64_t n = 0; n < Pb - Pa && !err; n++) {
+                /* staged == 2: the stage's packets in reverse order */
+                const int64_t ip = (run->staged == 2) ? Pb - n : Pa + 1 + n;
+                C.fS = (Pa > 0) ? (double)(ip - 1) / (double)Pa : 0.0;
+                err = run_packet(&C, run, &p, ip, wangle, wlam);
             }
-            if (t->pkt_steps) t->pkt_steps[ip - 1] += t->nsteps;
+            Pa = Pb;
+            Pb = orc_stage_next(Pb, Nphot, s0, ratio);
         }
+        for (int c = 0; c < ncell; c++) {
+            t->Eabs[c] = (double)C.Eq[c] * C.unitE;
+            t->T[c] = (C.Eq[c] != 0) ? T_of(&C, c, t->Eabs[c] / C.mass[c]) : 0.0;
+        }
+        free(C.Eq); free(C.Sq); free(C.own); free(C.own_pkt);
+    } else {
+        for (int64_t ip = 1; ip <= Nphot && !err; ip++) err = run_packet(&C, run, &p, ip, wangle, wlam);
     }
     free(C.KabsTot); free(C.KscaTot); free(C.mass); free(C.KabsL); free(C.KextL);
     free(C.spec); free(C.lastphotnr);

@@ -85,6 +85,12 @@ typedef struct {
     int64_t inflight;       /* > 1 (Philox only): advance this many packets
                                in turn, one trace2d call each (diagnostic
                                of interleaved schedules; 0/1: serial)    */
+    int staged;             /* 1 (Philox only): the staged Bjorkman-Wood
+                               schedule of the device's concurrent pass
+                               (orc_mcradiation below); 2: the same with
+                               each stage's packets in reverse order     */
+    int64_t stage0;         /* packets of the first stage                 */
+    int stage_ratio;        /* each later stage ends at ratio x its start */
 } orc_run;
 
 typedef struct {
@@ -133,8 +139,29 @@ void orc_phi_tables(double *cos2phi, double *sin2phi);
 
 /* ---- the transport pass and the temperature inversion ---------------- */
 /* Restates MCRadiation (MCRadiation.f:6-887) for the thermal-contact, plain
- * stellar emission path.  Returns 0, or a negative error code.             */
+ * stellar emission path.  Returns 0, or a negative error code.
+ *
+ * run->staged selects the schedule of the device's concurrent pass instead
+ * of the serial loop (not a reference code path; parity unpinned against
+ * the reference beyond the statistical tests against the serial loop).
+ * Packets 0..N-1 run in stages [P_k, P_k+1): P_1 = stage0, P_k+1 =
+ * ratio P_k (capped at N; stage0 0 means 64, ratio 0 means 2; with ratio -1
+ * the ratio grows as min(16, 1 + sqrt(P_k / stage0)), orc_stage_next).  In the serial loop the absorbed energy a cell
+ * holds when packet p arrives is, in expectation over the (exchangeable)
+ * packets before it, p/N of the cell's pass total; the staged schedule
+ * takes that expectation from the completed prefix instead of the
+ * realisation: packet p of a stage starting at P_k sees, in cell c,
+ *     E_b = (S_k(c) p / P_k + own_p(c)) unitE
+ * where S_k(c) is the fixed-point Eabs of packets 0..P_k-1 (0 in the first
+ * stage) and own_p(c) packet p's own earlier deposits in c.  Bjorkman-Wood
+ * reemission then uses T0 = T(E_b/m), T1 = T((E_b + e)/m), the random-walk
+ * test T(E_b/m).  Eabs is kept in the device's fixed point (2^24 units per
+ * nominal packet energy E0 = L/N).  Each history then depends only on its
+ * own draws and the stage's snapshot, so the order in which the packets of
+ * a stage run does not matter. */
 int orc_mcradiation(const orc_model *m, const orc_run *run, orc_ran1_state *rs, orc_tally *t);
+/* the end of the stage starting at packet P (the staged schedule above) */
+int64_t orc_stage_next(int64_t P, int64_t N, int64_t stage0, int ratio);
 
 /* Restates Main.f:498-558 (with determineT, Radiation.f:629-702):
  * EJv, EJv2 are the raw pass tallies; Tbw the Bjorkman-Wood cell T.

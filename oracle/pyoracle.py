@@ -31,7 +31,7 @@ class OrcModel(C.Structure):
 
 class OrcRun(C.Structure):
     _fields_ = [("nphot", C.c_int64), ("rng", C.c_int), ("seed", C.c_uint64), ("maxinteract", C.c_int),
-                ("inflight", C.c_int64)]
+                ("inflight", C.c_int64), ("staged", C.c_int), ("stage0", C.c_int64), ("stage_ratio", C.c_int)]
 
 
 class OrcTally(C.Structure):
@@ -351,7 +351,11 @@ class Oracle:
     def min_dist(self, x, y, z, i, jj):
         return self.lib.orc_min_dist(C.byref(self.om), x, y, z, int(i), int(jj))
 
-    def mcradiation(self, nphot, rng="philox", seed=1, maxinteract=0, per_packet=False, inflight=0):
+    def mcradiation(self, nphot, rng="philox", seed=1, maxinteract=0, per_packet=False, inflight=0, staged=False,
+                    stage0=64, stage_ratio=2):
+        """One pass: the serial loop, or with staged=True the staged
+        Bjorkman-Wood schedule of the device's concurrent pass (orc_run in
+        mcx_oracle.h)."""
         m = self.m
         n = m.nr * m.nt
         nb = (NANGLE + 1) * m.nlam
@@ -365,7 +369,8 @@ class Oracle:
             out["pkt_steps"] = np.zeros(int(nphot), dtype=np.int64)
             t.pkt_steps = out["pkt_steps"].ctypes.data_as(C.POINTER(C.c_int64))
         run = OrcRun(nphot=int(nphot), rng=0 if rng == "ran1" else 1, seed=int(seed), maxinteract=int(maxinteract),
-                     inflight=int(inflight))
+                     inflight=int(inflight), staged=int(staged), stage0=int(stage0),
+                     stage_ratio=int(stage_ratio))
         rc = self.lib.orc_mcradiation(C.byref(self.om), C.byref(run), C.byref(self.ran1), C.byref(t))
         if rc != 0:
             raise RuntimeError(f"oracle transport failed: {rc}")
