@@ -39,51 +39,93 @@ def algorithmic_bytes(t, ng, nlam):
     return total, (Bx, Ba, Bs, Brw)
 
 
-def measured_traffic(config, nphot, ng):
-    """HBM bytes per k_transport launch for this workload, from the rocprofv3
-    FETCH_SIZE/WRITE_SIZE runs summarised in profiles/ (scaled by a plain copy
-    of known size); None when no measurement of this workload exists."""
+def measured_traffic(config, nphot, ng, schedule):
+    """HBM bytes per k_transport launch (one stage) for this workload, from
+    the rocprofv3 FETCH_SIZE/WRITE_SIZE runs summarised in profiles/ (the
+    pass's dispatches summed, scaled by a plain copy of known size, divided
+    by the pass's launches); None when no measurement of this workload and
+    stage schedule exists."""
     path = os.path.join(ROOT, "profiles", "k_transport_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    if t.get("config") != config or int(t.get("nphot", -1)) != int(nphot) or int(t.get("ng", -1)) != int(ng):
+    if (t.get("config") != config or int(t.get("nphot", -1)) != int(nphot) or int(t.get("ng", -1)) != int(ng)
+            or t.get("schedule") != schedule):
         return None
     return t.get("bytes_per_launch")
 
 
-def cpu_baseline(m, pass_steps, nphot, seconds=15.0):
-    """Oracle (FP64 C restatement of the reference loop) on one host core, on
-    a bounded sample of the same workload: a pilot of 5000 packets sizes a
-    sample of about `seconds` of CPU time.  Packet histories are
-    heavy-tailed, so a sample's packets/s overstates a full pass; the rate is
-    taken per cell step and applied to the cell steps of the measured pass
-    (pass_steps for nphot packets)."""
+def _cpu_worker(args):
+    """one host process: the oracle's serial loop on batches of the model
+    until `seconds` of its own CPU time (fresh seeds per batch)"""
+    m, seconds, seed0, n = args
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     o = pyoracle.Oracle(m)
-    n = 5000
-    t0 = time.perf_counter()
-    o.mcradiation(n, rng="philox", seed=99)
-    dt = time.perf_counter() - t0
-    n2 = max(5000, int(n * seconds / max(dt, 1e-3)))
-    # batches of n2 packets (fresh seeds) until about `seconds` of CPU time:
-    # a pilot that met a long history sizes the batch too small
-    steps, dt, npk, b = 0, 0.0, 0, 0
-    while dt < 0.8 * seconds and b < 8:
-        t0 = time.perf_counter()
-        r = o.mcradiation(n2, rng="philox", seed=98 - b)
-        dt += time.perf_counter() - t0
+    steps, npk, b = 0, 0, 0
+    t0 = time.process_time()
+    while time.process_time() - t0 < seconds and b < 64:
+        r = o.mcradiation(n, rng="philox", seed=seed0 + b)
         steps += r["nsteps"]
-        npk += n2
+        npk += n
         b += 1
-    steps_per_s = steps / dt
+    return steps, npk, time.process_time() - t0
+
+
+def host_cores():
+    """the host cores this process may use (at most 16: the box's CPU share
+    per GPU), and the CPU's model name"""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    name = "?"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return max(1, min(16, n)), n, name
+
+
+def cpu_baseline(m, pass_steps, nphot, seconds=15.0):
+    """The oracle (FP64 C restatement of the reference's serial loop) on the
+    GPU box's host cores, on a bounded sample of the same workload: one
+    process per core, each an independent serial pass over batches of
+    packets (the reference's nruns sub-runs, MCRadiation.f:192, 269-668)
+    for about `seconds` of CPU time.  Packet histories are heavy-tailed,
+    so a sample's packets/s overstates a full pass: the rate is taken per
+    cell step and applied to the cell steps of the measured pass
+    (pass_steps for nphot packets)."""
+    import multiprocessing as mp
+    cores, avail, name = host_cores()
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    o = pyoracle.Oracle(m)
+    t0 = time.perf_counter()
+    o.mcradiation(2000, rng="philox", seed=99)
+    dt = time.perf_counter() - t0
+    n = max(2000, int(2000 * 0.5 * seconds / max(dt, 1e-3)) // 8)
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_cpu_worker, [(m, seconds, 1000 * (k + 1), n) for k in range(cores)])
+    wall = time.perf_counter() - t0
+    steps = sum(r[0] for r in res)
+    npk = sum(r[1] for r in res)
+    cpu1 = sum(r[0] / r[2] for r in res) / cores
+    steps_per_s = steps / wall
     pass_s = pass_steps / steps_per_s
-    return {"value": nphot / pass_s, "unit": "packets/s", "cores": 1, "kind": "port",
-            "sample": f"{npk} packets of the same model on one host core ({b} batches): {steps} cell steps in "
-                      f"{dt:.1f} s ({steps_per_s:.3e} steps/s, {npk / dt:.0f} packets/s in the sample); at that "
-                      f"rate the measured pass ({pass_steps} cell steps) takes {pass_s:.0f} s"}
+    return {"value": nphot / pass_s, "unit": "packets/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} host processes ({name}; {avail} cores visible) each running the oracle's serial "
+                      f"loop on batches of {n} packets of the same model for ~{seconds:.0f} s CPU: {npk} packets, "
+                      f"{steps} cell steps in {wall:.1f} s wall ({steps_per_s:.3e} steps/s; {cpu1:.3e} steps/s per "
+                      f"core); at that rate the measured pass ({pass_steps} cell steps) takes {pass_s:.1f} s",
+            "one_core_packets_per_s": nphot / (pass_steps / cpu1)}
 
 
 def dust_bench(a, work, lam_stride=10):
@@ -129,6 +171,20 @@ def dust_bench(a, work, lam_stride=10):
         out["cpu_baseline"] = {"value": n / dt, "unit": "rows/s", "cores": 1, "kind": "port",
                                "sample": f"every species at {len(sel)} of {len(lam)} wavelengths ({n} rows) "
                                          f"in {dt:.1f} s on one host core"}
+    return out
+
+
+def stage_schedule(n, ratio=0, s0=64):
+    """the stage ends of a concurrent pass (the library's schedule,
+    mcx_transport): s0, then ratio x (ratio 0: 2; -1: 1 + sqrt(P/s0), at
+    most 16)"""
+    r0 = 2 if ratio == 0 else ratio
+    P, out = min(s0, n), [min(s0, n)]
+    while P < n:
+        r = r0 if r0 >= 2 else min(16.0, 1.0 + math.sqrt(P / s0))
+        nx = math.ceil(P * r)
+        P = n if nx >= n else nx
+        out.append(P)
     return out
 
 
@@ -335,7 +391,10 @@ def main():
     value = a.nphot * steps / wall
     kern_s = sum(kms) / len(kms) * 1e-3
     nbytes, (Bx, Ba, Bs, Brw) = algorithmic_bytes(tal, m.ng, m.nlam)
-    achieved = nbytes / kern_s
+    # one launch per stage: bytes per launch / average launch duration
+    launches = len(stage_schedule(a.nphot, a.stage_ratio))
+    achieved = (nbytes / launches) / (kern_s / launches)
+    schedule = f"stage0 64, ratio {a.stage_ratio if a.stage_ratio else 2}"
     out = {
         "metric": METRIC, "value": value, "unit": "packets/s", "n_gpus": 1, "steps": steps,
         "warmup": warm, "steps_requested": a.steps, "warmup_requested": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -345,8 +404,11 @@ def main():
                    "nphot": a.nphot, "nr": m.nr, "nt": m.nt, "nlam": m.nlam, "ng": m.ng,
                    "randomwalk": not a.plain, "lanes": a.lanes or "default"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic(a.config, a.nphot, m.ng),
-                     "kernel": "k_transport", "kernel_ms": kern_s * 1e3,
+                     "frac": achieved / HBM_PEAK,
+                     "traffic": measured_traffic(a.config, a.nphot, m.ng, schedule),
+                     "kernel": "k_transport", "launches": launches, "schedule": schedule,
+                     "kernel_ms_per_launch": kern_s * 1e3 / launches,
+                     "bytes_per_launch": nbytes / launches, "kernel_ms": kern_s * 1e3,
                      "bytes_per_event": {"step": Bx, "absorption": Ba, "scattering": Bs, "rw_step": Brw,
                                          "escape": 8},
                      "events": {k: int(tal[k]) for k in ("nsteps", "nabs", "nscat", "nrw", "nescape")}},

@@ -1068,6 +1068,49 @@ static void scatangle(ctx_t *C, phot_t *p, const double *M, double MIF11, double
     if (i != 1 && i != 180) p->pol = 1;
 }
 
+/* the Bjorkman-Wood reemission of an absorption in cell c that took the cell
+ * from T0 to T1: the difference spectrum (B(T1) - B(T0)) KabsTot and emit()
+ * (Radiation.f:173-258, 940-1001) */
+static void reemit(ctx_t *C, phot_t *p, int c, double T0, double T1, double KabsTotL)
+{
+    const orc_model *m = C->m;
+    const int nlam = C->nlam, ng = C->ng;
+    const double *Ka = C->KabsTot + (size_t)c * nlam;
+    int iT0 = (int)(T0 / 1.0), iT1 = (int)(T1 / 1.0);
+    double epsT0 = T0 - (double)(float)iT0 * 1.0;
+    double epsT1 = T1 - (double)(float)iT1 * 1.0;
+    if (iT0 >= TMAX - 1) iT0 = TMAX - 2;
+    if (iT1 >= TMAX - 1) iT1 = TMAX - 1;
+    if (iT0 < 1) iT0 = 1;
+    if (iT1 < 2) iT1 = 2;
+    double kp = 0.0;
+    const double *B = m->BB;
+    const double *wc = m->w + (size_t)c * ng;
+    if (iT0 == iT1) {
+        for (int l = 0; l < nlam; l++)
+            C->spec[l] = (B[(size_t)(iT0 + 1) * nlam + l] - B[(size_t)iT0 * nlam + l]) * Ka[l];
+        for (int ii = 0; ii < ng; ii++) {
+            const double *Kp = m->Kp + (size_t)ii * (TMAX + 1);
+            kp = kp + ((Kp[iT0 + 1] - Kp[iT0]) * wc[ii]) * 1.0;
+        }
+    } else {
+        for (int l = 0; l < nlam; l++)
+            C->spec[l] = (((epsT1 * B[(size_t)(iT1 + 1) * nlam + l] + (1.0 - epsT1) * B[(size_t)iT1 * nlam + l]) -
+                           epsT0 * B[(size_t)(iT0 + 1) * nlam + l]) -
+                          (1.0 - epsT0) * B[(size_t)iT0 * nlam + l]) * Ka[l];
+        for (int ii = 0; ii < ng; ii++) {
+            const double *Kp = m->Kp + (size_t)ii * (TMAX + 1);
+            kp = kp + (((((epsT1 * Kp[iT1 + 1] + (1.0 - epsT1) * Kp[iT1]) - epsT0 * Kp[iT0 + 1]) -
+                         (1.0 - epsT0) * Kp[iT0]) * wc[ii]) * 1.0);
+        }
+    }
+    if (kp == 0.0) {
+        for (int l = 0; l < nlam; l++) C->spec[l] = 0.0;
+        kp = 0.0 / KabsTotL;
+    }
+    emit(C, p, C->spec, kp);
+}
+
 /* Radiation.f:41-355, thermal contact, no QHP/gas heating, multiwav off. */
 static void interact(ctx_t *C, phot_t *p)
 {
@@ -1099,39 +1142,7 @@ static void interact(ctx_t *C, phot_t *p)
             T0 = t->T[c];
             T1 = increaseT(C, c, t->Eabs[c] / C->mass[c], t->T[c]);
         }
-        int iT0 = (int)(T0 / 1.0), iT1 = (int)(T1 / 1.0);
-        double epsT0 = T0 - (double)(float)iT0 * 1.0;
-        double epsT1 = T1 - (double)(float)iT1 * 1.0;
-        if (iT0 >= TMAX - 1) iT0 = TMAX - 2;
-        if (iT1 >= TMAX - 1) iT1 = TMAX - 1;
-        if (iT0 < 1) iT0 = 1;
-        if (iT1 < 2) iT1 = 2;
-        double kp = 0.0;
-        const double *B = m->BB;
-        const double *wc = m->w + (size_t)c * ng;
-        if (iT0 == iT1) {
-            for (int l = 0; l < nlam; l++)
-                C->spec[l] = (B[(size_t)(iT0 + 1) * nlam + l] - B[(size_t)iT0 * nlam + l]) * Ka[l];
-            for (int ii = 0; ii < ng; ii++) {
-                const double *Kp = m->Kp + (size_t)ii * (TMAX + 1);
-                kp = kp + ((Kp[iT0 + 1] - Kp[iT0]) * wc[ii]) * 1.0;
-            }
-        } else {
-            for (int l = 0; l < nlam; l++)
-                C->spec[l] = (((epsT1 * B[(size_t)(iT1 + 1) * nlam + l] + (1.0 - epsT1) * B[(size_t)iT1 * nlam + l]) -
-                               epsT0 * B[(size_t)(iT0 + 1) * nlam + l]) -
-                              (1.0 - epsT0) * B[(size_t)iT0 * nlam + l]) * Ka[l];
-            for (int ii = 0; ii < ng; ii++) {
-                const double *Kp = m->Kp + (size_t)ii * (TMAX + 1);
-                kp = kp + (((((epsT1 * Kp[iT1 + 1] + (1.0 - epsT1) * Kp[iT1]) - epsT0 * Kp[iT0 + 1]) -
-                             (1.0 - epsT0) * Kp[iT0]) * wc[ii]) * 1.0);
-            }
-        }
-        if (kp == 0.0) {
-            for (int l = 0; l < nlam; l++) C->spec[l] = 0.0;
-            kp = 0.0 / KabsTotL;
-        }
-        emit(C, p, C->spec, kp);
+        reemit(C, p, c, T0, T1, KabsTotL);
         if (!C->staged) t->T[c] = T1;
     } else {
         t->nscat++;
@@ -3070,4 +3081,77 @@ void orc_make_image(int nimg, int nphi, const double *Rimg, const double *rays, 
             }
         }
 #undef AT
+}
+
+/* ---- samplers in isolation (test pins) -------------------------------- */
+static void sampler_ctx(ctx_t *C, const orc_model *m, orc_tally *t, uint64_t seed)
+{
+    memset(C, 0, sizeof(*C));
+    memset(t, 0, sizeof(*t));
+    C->m = m;
+    C->nr = m->nr; C->nt = m->nt; C->nlam = m->nlam; C->ng = m->ng; C->NT = m->nt + 1;
+    C->t = t;
+    C->g.mode = 1;
+    C->g.seed = seed;
+    C->KabsL = (double *)calloc(m->ng, sizeof(double));
+    C->KextL = (double *)calloc(m->ng, sizeof(double));
+    C->spec = (double *)calloc(m->nlam, sizeof(double));
+}
+
+static void sampler_free(ctx_t *C)
+{
+    free(C->KabsL); free(C->KextL); free(C->spec); free(C->KabsTot);
+}
+
+int orc_sample_reemission(const orc_model *m, int c, double T0, double T1, uint64_t seed, int n, int32_t *ilam1,
+                          double *wl1)
+{
+    ctx_t C;
+    orc_tally t;
+    sampler_ctx(&C, m, &t, seed);
+    const int nlam = m->nlam, ng = m->ng;
+    C.KabsTot = (double *)calloc((size_t)(c + 1) * nlam, sizeof(double));
+    double *Ka = C.KabsTot + (size_t)c * nlam, kab = 0.0;
+    for (int l = 0; l < nlam; l++) {
+        double ka = 0.0;
+        for (int ii = 0; ii < ng; ii++) ka = ka + (m->Kabs[(size_t)ii * nlam + l] * m->w[(size_t)c * ng + ii]) * 1.0;
+        Ka[l] = ka;
+        if (l == 0) kab = ka;
+    }
+    for (int k = 0; k < n; k++) {
+        phot_t p;
+        memset(&p, 0, sizeof(p));
+        C.g.pid = (uint64_t)k;
+        C.g.k = 0;
+        reemit(&C, &p, c, T0, T1, kab);
+        ilam1[k] = p.ilam1;
+        wl1[k] = p.wl1;
+    }
+    sampler_free(&C);
+    return 0;
+}
+
+int orc_sample_scatangle(const orc_model *m, int s, int l, double E, double Q, double U, double V, int pol,
+                         uint64_t seed, int n, double *cost, double *phi)
+{
+    ctx_t C;
+    orc_tally t;
+    sampler_ctx(&C, m, &t, seed);
+    const size_t o = (size_t)s * m->nlam + l;
+    for (int k = 0; k < n; k++) {
+        phot_t p;
+        memset(&p, 0, sizeof(p));
+        p.vz = 1.0;
+        p.Sx = 1.0;
+        p.E = E; p.Q = Q; p.U = U; p.V = V;
+        p.pol = pol;
+        C.g.pid = (uint64_t)k;
+        C.g.k = 0;
+        scatangle(&C, &p, m->F + o * 6 * 180, m->IF11[o], m->IF12[o]);
+        cost[k] = p.vz;
+        /* S started along x and turned by phi about the incoming z */
+        if (phi) phi[k] = atan2(p.Sy, p.Sx);
+    }
+    sampler_free(&C);
+    return 0;
 }

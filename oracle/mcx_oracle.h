@@ -160,6 +160,17 @@ void orc_phi_tables(double *cos2phi, double *sin2phi);
  * own draws and the stage's snapshot, so the order in which the packets of
  * a stage run does not matter. */
 int orc_mcradiation(const orc_model *m, const orc_run *run, orc_ran1_state *rs, orc_tally *t);
+/* The samplers alone (test pins): n reemission wavelengths of an
+ * absorption in cell c that took it from T0 to T1 (emit's ilam1, wl1,
+ * Radiation.f:173-258, 940-1001); n scattering angles (cos theta against
+ * the incoming direction z, and the azimuth phi of the turned reference
+ * direction, which starts along x; phi may be NULL) off species s at
+ * wavelength l with Stokes (E, Q, U, V) (scatangle, Radiation.f:381-495).  Draw k of both uses the
+ * Philox stream (seed, k). */
+int orc_sample_reemission(const orc_model *m, int c, double T0, double T1, uint64_t seed, int n, int32_t *ilam1,
+                          double *wl1);
+int orc_sample_scatangle(const orc_model *m, int s, int l, double E, double Q, double U, double V, int pol,
+                         uint64_t seed, int n, double *cost, double *phi);
 /* the end of the stage starting at packet P (the staged schedule above) */
 int64_t orc_stage_next(int64_t P, int64_t N, int64_t stage0, int ratio);
 

@@ -281,6 +281,10 @@ def load():
                                      [C.c_int, C.c_int, C.c_double, C.c_double] + [_dp] * 5 +
                                      [C.POINTER(C.c_long)])
     lib.orc_compute_part.restype = C.c_int
+    lib.orc_sample_reemission.argtypes = [C.POINTER(OrcModel), C.c_int, C.c_double, C.c_double, C.c_uint64,
+                                          C.c_int, _ip, _dp]
+    lib.orc_sample_scatangle.argtypes = [C.POINTER(OrcModel), C.c_int, C.c_int] + [C.c_double] * 4 + [
+        C.c_int, C.c_uint64, C.c_int, _dp, _dp]
     _lib = lib
     return lib
 
@@ -312,6 +316,22 @@ class Oracle:
                            rndw=int(m.randomwalk), factRW=float(m.factRW), gsym=p(k["gsym"]),
                            thet=p(k["thet"]))
         self.ran1 = Ran1State(idum=-42, iy=0)
+
+    def sample_reemission(self, c, T0, T1, n, seed=1):
+        """n reemission wavelengths (ilam1 1-based, wl1) of an absorption in
+        cell c taking it from T0 to T1."""
+        il = np.zeros(n, dtype=np.int32)
+        wl = np.zeros(n)
+        self.lib.orc_sample_reemission(C.byref(self.om), int(c), float(T0), float(T1), int(seed), int(n), p(il), p(wl))
+        return il, wl
+
+    def sample_scatangle(self, s, l, n, E=1.0, Q=0.0, U=0.0, V=0.0, pol=False, seed=1, with_phi=False):
+        """cos theta of n scatterings off species s at wavelength index l
+        (and the azimuths phi of the scattering with with_phi)."""
+        out, ph = np.zeros(n), np.zeros(n)
+        self.lib.orc_sample_scatangle(C.byref(self.om), int(s), int(l), float(E), float(Q), float(U), float(V),
+                                      int(pol), int(seed), int(n), p(out), p(ph))
+        return (out, ph) if with_phi else out
 
     def random_walk_tables(self):
         """(y, phi, thick) of InitRandomWalk for this model."""

@@ -8,7 +8,9 @@ production dust (ten DHS species, m = 1.7 + 0.03i, 0.05-1000 um, MRN
 weights) at the production dust mass 1e-3 Msun; the dust opacities come from
 the oracle's ComputePart restatement (bit-identical to the device's).
 
-    python tests/golden/make_staged_fixtures.py
+    python tests/golden/make_staged_fixtures.py [small] [large]
+
+(small: 48 seeds of 20000 packets; large: 16 seeds of 200000 packets)
 """
 import hashlib
 import os
@@ -27,6 +29,8 @@ from mcmax_amd import configs  # noqa: E402
 
 NPHOT = 20000
 SEEDS = list(range(1001, 1049))
+# the same model at 2e5 packets, 16 seeds
+NPHOT_L, SEEDS_L = 200000, list(range(2001, 2017))
 DHS10 = {"abun01": 1.0, "computepart01:file": "dhs.dat", "computepart01:amin": 0.05,
          "computepart01:amax": 1000.0, "computepart01:apow": 3.5, "computepart01:fmax": 0.8,
          "computepart01:ngrains": 10, "computepart01:blend": False}
@@ -55,13 +59,13 @@ def model_hash(m):
     return h.hexdigest()
 
 
-def run_case(name, overrides, workdir):
+def run_case(name, overrides, workdir, nphot=NPHOT, seeds=SEEDS, tag=""):
     m = configs.load_config("tiny", os.path.join(workdir, name), tr=OracleDust(), **overrides)
     o = pyoracle.Oracle(m)
     keys = ("T", "Ni", "spec", "ispec", "ninteract", "nabs", "nsteps", "Eescape")
     acc = {k: [] for k in keys}
-    for s in SEEDS:
-        r = o.mcradiation(NPHOT, rng="philox", seed=s)
+    for s in seeds:
+        r = o.mcradiation(nphot, rng="philox", seed=s)
         r["T"], r["dT"], _ = o.temperature(r)
         for k in keys:
             acc[k].append(r[k])
@@ -70,12 +74,16 @@ def run_case(name, overrides, workdir):
     # statistics only need single precision (the spectrum's values exceed
     # float32's range)
     out["T"] = out["T"].astype(np.float32)
-    out.update(nphot=NPHOT, seeds=np.asarray(SEEDS), model_hash=model_hash(m))
-    np.savez_compressed(os.path.join(HERE, f"serial_{name}.npz"), **out)
+    out.update(nphot=nphot, seeds=np.asarray(seeds), model_hash=model_hash(m))
+    np.savez_compressed(os.path.join(HERE, f"serial_{name}{tag}.npz"), **out)
 
 
 if __name__ == "__main__":
     pyoracle.build()
     work = tempfile.mkdtemp(prefix="mcx_golden_")
+    which = sys.argv[1:] or ["small", "large"]
     for name, ov in CASES.items():
-        run_case(name, ov, work)
+        if "small" in which:
+            run_case(name, ov, work)
+        if "large" in which:
+            run_case(name, ov, work, NPHOT_L, SEEDS_L, "_2e5")
