@@ -300,12 +300,52 @@ def image_bench(a, work, lam0=1.6):
     return out
 
 
+def precision_study(a, work):
+    """The precision study: the config with --mdust (default 1e-2 Msun:
+    midplane radial tau ~1e5), `--steps` pairs of passes with the same seeds
+    in FP64 and in FP32 (MCX_FP32); per-cell T error and MC SED error of the
+    FP32 passes against the FP64 passes, and the time of each."""
+    import numpy as np
+    from mcmax_amd import configs, transport
+    tr = transport.Transport(0)
+    m = configs.load_config(a.config, work, tr=tr, randomwalk=not a.plain, mdust=a.mdust)
+    tr.set_model(m)
+    res = {0: [], 1: []}
+    times = {0: [], 1: []}
+    for s in range(a.steps):
+        for prec in (0, 1):
+            g = transport.mc_radiation(tr, a.nphot, seed=a.seed + s, precision=prec, max_ms=1e3 * a.budget)
+            res[prec].append(g)
+            times[prec].append(g["kernel_ms"])
+            print(f"[bench] precision {prec} seed {a.seed + s}: {g['kernel_ms']:.0f} ms, maxsteps {g['maxsteps']}",
+                  file=sys.stderr, flush=True)
+    tr.close()
+    T64 = np.array([r["T"][1:] for r in res[0]])
+    T32 = np.array([r["T"][1:] for r in res[1]])
+    ok = np.array([r["Ni"][1:] for r in res[0] + res[1]]).min(axis=0) > 20
+    rel = np.abs(T32.mean(0) / T64.mean(0) - 1)[ok]
+    S64 = np.array([r["spec"].sum(axis=0) for r in res[0]]).mean(0)
+    S32 = np.array([r["spec"].sum(axis=0) for r in res[1]]).mean(0)
+    n64 = np.array([r["ispec"].sum(axis=0) for r in res[0]]).sum(0)
+    well = n64 >= 10000
+    sed = np.abs(S32[well] / S64[well] - 1)
+    return {"metric": "FP32 transport: per-cell T and MC SED error against FP64 passes with the same seeds",
+            "value": float(np.median(rel)), "unit": "median |dT|/T", "higher_is_better": False,
+            "steps": a.steps, "nphot": a.nphot, "data": "synthetic",
+            "config": {"workload": f"{a.config} disk model at mdust {a.mdust} Msun", "nr": m.nr, "nt": m.nt,
+                       "ng": m.ng, "randomwalk": not a.plain},
+            "T_rel_p90": float(np.percentile(rel, 90)), "T_rel_max": float(rel.max()), "cells": int(ok.sum()),
+            "sed_max_rel_well_sampled": float(sed.max()) if well.any() else None, "sed_bins": int(well.sum()),
+            "ms_fp64": times[0], "ms_fp32": times[1]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nphot", type=int, default=10_000_000)
+    ap.add_argument("--mdust", type=float, default=1e-2, help="dust mass of --precision-study (Msun)")
     ap.add_argument("--config", default="production")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu", action="store_true")
@@ -318,6 +358,10 @@ def main():
                     help="measure the dust-opacity generator on the model's computepart particle instead")
     ap.add_argument("--image", action="store_true",
                     help="measure an IMAGE observation (ray images and MakeImage's spread) instead")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
+                    help="transport precision mode (fp32: MCX_FP32 direction arithmetic)")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="FP32 against FP64 passes with the same seeds (per-cell T and MC SED errors) instead")
     ap.add_argument("--stage-ratio", type=int, default=0,
                     help="staged schedule: stage ratio (0: the library's default 2; -1: growing ratio)")
     ap.add_argument("--budget", type=float, default=400.0,
@@ -352,6 +396,10 @@ def main():
     if a.image:
         print(json.dumps(image_bench(a, work)))
         return
+    if a.precision_study:
+        print(json.dumps(precision_study(a, work)))
+        return
+    prec = 1 if a.precision == "fp32" else 0
     tr = transport.Transport(0)
     m = configs.load_config(a.config, work, tr=tr, randomwalk=not a.plain)
     print(f"[bench] model {a.config}: {m.summary()}", file=sys.stderr, flush=True)
@@ -366,7 +414,8 @@ def main():
         if w > 0 and time.perf_counter() - t_start > 0.25 * a.budget:
             break
         t1 = time.perf_counter()
-        tr.transport(a.nphot, seed=a.seed + 1000 + w, nlanes=a.lanes, stage_ratio=a.stage_ratio)
+        tr.transport(a.nphot, seed=a.seed + 1000 + w, nlanes=a.lanes, stage_ratio=a.stage_ratio, precision=prec,
+                     max_ms=1e3 * a.budget)
         tr.temperature()
         longest = max(longest, time.perf_counter() - t1)
         warm += 1
@@ -378,7 +427,8 @@ def main():
         if s > 0 and time.perf_counter() - t_start + longest > a.budget:
             break
         t1 = time.perf_counter()
-        rc = tr.transport(a.nphot, seed=a.seed + s, nlanes=a.lanes, stage_ratio=a.stage_ratio)
+        rc = tr.transport(a.nphot, seed=a.seed + s, nlanes=a.lanes, stage_ratio=a.stage_ratio, precision=prec,
+                          max_ms=1e3 * a.budget)
         k, _ = tr.timing()
         kms.append(k)
         tr.temperature()
@@ -398,7 +448,7 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "packets/s", "n_gpus": 1, "steps": steps,
         "warmup": warm, "steps_requested": a.steps, "warmup_requested": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64" if prec == 0 else "f32 directions / f64 geometry", "data": "synthetic",
         "config": {"workload": f"{a.config} disk model, one MC pass + T inversion per step"
                                + ("" if a.plain else ", modified random walk (factRW=10)"),
                    "nphot": a.nphot, "nr": m.nr, "nt": m.nt, "nlam": m.nlam, "ng": m.ng,

@@ -44,6 +44,8 @@ extern "C" {
 #define MCX_EDEVICE (-2)  /* HIP runtime error                           */
 #define MCX_ENOMEM (-3)   /* device allocation failed                    */
 #define MCX_ESTATE (-4)   /* call out of order (e.g. no model set)       */
+#define MCX_ETIME (-6)    /* the pass reached its time limit (mcx_run.max_ms)
+                             with packets unfinished; tallies partial   */
 #define MCX_EGEOM (-5)    /* packets lost to "nothing to hit"
                              (MCRadiation.f:1223-1232 stops there)       */
 
@@ -120,10 +122,23 @@ typedef struct {
                                         depend on it)                       */
     int32_t stage_ratio;             /* 0: 2; -1: the ratio grows as
                                         min(16, 1 + sqrt(P_k / stage0))     */
-    int32_t precision;               /* MCX_FP64 (the reference's real*8)   */
+    int32_t precision;               /* MCX_FP64: the reference's real*8
+                                        throughout.  MCX_FP32: directions,
+                                        their rotations and the Stokes
+                                        vector, and the optical-depth draws
+                                        in single precision (hardware sin,
+                                        cos, log, rsqrt); positions, the
+                                        crossing quadratics, optical-depth
+                                        sums, tallies and temperatures FP64 */
+    double max_ms;                   /* > 0: time limit of the pass (the
+                                        reference's maxruntime, MCRadiation.f:
+                                        323-342, as a hard stop): past it
+                                        every lane abandons its packet and
+                                        the call returns MCX_ETIME          */
 } mcx_run;
 
 #define MCX_FP64 0
+#define MCX_FP32 1
 
 /* Pass tallies and statistics (MCRadiation.f:656-715). */
 typedef struct {
@@ -151,6 +166,7 @@ typedef struct {
     int64_t nvisit_overflow;         /* cell entries a packet's visit table
                                         could not hold (Ni may count such a
                                         packet twice; 0 in every test)      */
+    int64_t ntimeout;                /* packets abandoned at the time limit */
 } mcx_tallies;
 
 /* Temperature products of Main.f:498-549 and the void cell (Main.f:546). */

@@ -12,7 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmcx.so")
 
-MCX_OK, MCX_EINVAL, MCX_EDEVICE, MCX_ENOMEM, MCX_ESTATE, MCX_EGEOM = 0, -1, -2, -3, -4, -5
+MCX_OK, MCX_EINVAL, MCX_EDEVICE, MCX_ENOMEM, MCX_ESTATE, MCX_EGEOM, MCX_ETIME = 0, -1, -2, -3, -4, -5, -6
 TMAX = 10000
 NANGLE = 30
 
@@ -43,7 +43,7 @@ class Run(C.Structure):
         ("nphot", C.c_int64), ("seed", C.c_uint64), ("maxinteract", C.c_int32),
         ("nlanes", C.c_int32), ("stage0", C.c_int64), ("randomwalk", C.c_int32),
         ("factRW", C.c_double), ("coop_lanes", C.c_int32), ("stage_ratio", C.c_int32),
-        ("precision", C.c_int32),
+        ("precision", C.c_int32), ("max_ms", C.c_double),
     ]
 
 
@@ -56,7 +56,7 @@ class Tallies(C.Structure):
         ("nsteps", C.c_int64), ("nraar", C.c_int64), ("nmaxinteract", C.c_int64),
         ("ngeomfail", C.c_int64), ("tautot", C.c_double), ("Eescape", C.c_double),
         ("nrw", C.c_int64), ("noverflow", C.c_int64), ("thick", _ip),
-        ("maxsteps", C.c_int64), ("t_done", C.c_double * 5), ("nvisit_overflow", C.c_int64),
+        ("maxsteps", C.c_int64), ("t_done", C.c_double * 5), ("nvisit_overflow", C.c_int64), ("ntimeout", C.c_int64),
     ]
 
 

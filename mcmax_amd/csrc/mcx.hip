@@ -45,6 +45,7 @@ struct TransportArgs {
     long long nphot;
     unsigned long long seed;
     int maxinteract;
+    unsigned long long deadline; // wall_clock64() past which lanes abandon their packets (0: none)
     int serial;    // only lane 0 of the grid takes packets (in packet order)
     int coop;      // wave-cooperative lookups when at most this many lanes need one
     int rb_mask;   // draws per lane ring minus one (a power of two minus one)
@@ -333,10 +334,11 @@ __device__ __forceinline__ void acc_ejv2(TallyAcc &t, double x2, double invU2, u
 //                                                 integer bin (Radiation.f:187-213)
 //  L_DIFF  f = ((((e1 R1[n+b] + (1-e1) R1[b]) - e0 R0[n+b]) - (1-e0) R0[b]) ws) 1
 //                                                 reemission, four Planck rows
-//  L_SCAT  f = e1 R1[b] + e0 R0[b] > t             scattering angle, E F11 + Q F12
+//  L_SCAT  f = e1 C11[b] + e0 C12[b] > t          scattering angle, E F11 + Q F12
 //                                                 (Radiation.f:436-452)
-// R0, R1 point at cumulative rows; for L_SCAT R1 = C11, R0 = C12 of the
-// chosen Mueller matrix and e1 = E, e0 = Q.  With several species
+// R0, R1 point at cumulative rows; for L_SCAT R1 is the chosen (species,
+// lambda) block of SC (records of 8: C11, C12, then F11 ... F44) and
+// e1 = E, e0 = Q.  With several species
 // (ng > 1) the reemission spectrum is the cell's mixture, sum_s w_s times
 // species s's difference spectrum (Radiation.f:199-214 with KabsTot): R0, R1
 // are species 0's rows, species s's lie (TMAX+1) n further per species, and
@@ -351,7 +353,7 @@ struct Lookup {
 };
 
 __device__ __forceinline__ double lk_eval(const Lookup &q, int b, int ng) {
-    if (q.kind == L_SCAT) return q.e1 * q.R1[b] + q.e0 * q.R0[b];
+    if (q.kind == L_SCAT) return q.e1 * q.R1[8 * b] + q.e0 * q.R1[8 * b + 1];
     if (ng <= 1) {
         if (q.kind == L_SAME) return ((q.R0[q.n + b] - q.R0[b]) * q.ws) * 1.0;
         const double a2 = 1.0 - q.e1, a4 = 1.0 - q.e0;
@@ -476,12 +478,21 @@ struct RingRng {
     }
 };
 
-#define NPL 19          // per-thread LDS doubles, 18 used (odd stride spreads the banks)
+#define NPL 23          // per-thread LDS doubles, 22 used (odd stride spreads the banks)
 #define COOP_LANES 6 // default: a wave serves lookups together when at most this many lanes need one
 
 // MODE 0: one dust species; 1: several, with per-class reemission tables;
 // 2: several, their tables summed at every reemission probe
-template <int MODE>
+// the direction draw and the optical-depth draw of the kernel's precision
+#define RDIR(gen, X, Y, Z)                                  \
+    do {                                                    \
+        if constexpr (F32) random_direction_f(gen, X, Y, Z); \
+        else random_direction(gen, X, Y, Z);                \
+    } while (0)
+#define TAUDRAW(u) (F32 ? tau_draw_f(u) : -log(u))
+
+// F32: the direction arithmetic in single precision (precision mode MCX_FP32)
+template <int MODE, bool F32>
 __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     extern __shared__ double lds[];
     const DevTables &tb = A.tb;
@@ -528,6 +539,10 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
 #define P_OWN pl[15]  // bits of the packet's own Eabs units in the visited cell
 #define P_VSLOT pl[16] // its slot in the lane's visit table (-1: none; +2^VIS_LOG2: own changed)
 #define P_TG pl[17]   // a guess of the visited cell's T as the packet sees it
+#define P_JW pl[18]   // the visited cell's table window kpc[jw..jw+2] (-1: none) ...
+#define P_W0 pl[19]   // ... and its three values
+#define P_W1 pl[20]
+#define P_W2 pl[21]
     for (int k = threadIdx.x; k <= nr; k += blockDim.x) sR2[k] = md.R2[k];
     for (int k = threadIdx.x; k <= nt; k += blockDim.x) {
         sT2[k] = md.T2[k];
@@ -593,6 +608,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     P_OWN = 0.0;
     P_VSLOT = -1.0;
     P_TG = 0.0;
+    P_JW = -1.0;
     // this lane's visit table
     uint4 *const vt = ty.vtab + ((size_t)(blockIdx.x * blockDim.x + threadIdx.x) << VIS_LOG2);
     double2 k1 = make_double2(0.0, 0.0), k2 = make_double2(0.0, 0.0);
@@ -653,6 +669,24 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
             }
         }
         if (__all(st == S_DONE)) break;
+        // the pass's time limit: past it every lane gives up its packet
+        // (counted) and takes no other, so the launch ends
+        if (A.deadline && wall_clock64() > A.deadline) {
+            if (st != S_DONE) {
+                CNT(C_NTIMEOUT);
+                // the first abandoned packet's state, for the error message
+                if (atomicCAS(&ty.counters[G_TOREP], 0ull, 1ull) == 0ull) {
+                    double *d = ty.torep;
+                    d[0] = (double)pid; d[1] = (double)st; d[2] = (double)p.nstep; d[3] = (double)p.i;
+                    d[4] = (double)p.jj; d[5] = p.x; d[6] = p.y; d[7] = p.z; d[8] = p.vx; d[9] = p.vy;
+                    d[10] = p.vz; d[11] = (double)p.onEdge; d[12] = (double)p.edgeNr; d[13] = p.tau0;
+                    d[14] = p.tau; d[15] = (double)p.ntrace;
+                }
+                st = S_DONE;
+            }
+            acc_flush(ty, acc);
+            break;
+        }
 
         // ---- rng service: the wavefront computes, 64 blocks per pass, the
         //      next RB/4 Philox blocks of every lane whose ring is at most
@@ -689,16 +723,16 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
         if (st == S_EMIT) {
             p.nEJv = 0.0;
             (void)g.next(); // where_emit
-            random_direction(g, p.x, p.y, p.z);
+            RDIR(g, p.x, p.y, p.z);
             p.x = md.R0 * p.x;
             p.y = md.R0 * p.y;
             p.z = md.R0 * p.z;
             CNT(C_NEMIT);
-            random_direction(g, p.vx, p.vy, p.vz); // inside emit()
+            RDIR(g, p.vx, p.vy, p.vz); // inside emit()
             double Lr = g.next() * tb.Lstar;
             pick_lambda(nbins, Lr, [&](int b) { return sCS[b]; }, p.l1, p.wl1);
             p.E = tb.Lstar / ((double)(float)A.nphot * ((1.0 - 0.0) - 0.0));
-            random_direction(g, p.vx, p.vy, p.vz);
+            RDIR(g, p.vx, p.vy, p.vz);
             if (p.x * p.vx + p.y * p.vy + p.z * p.vz < 0.0) {
                 p.vx = -p.vx;
                 p.vy = -p.vy;
@@ -734,7 +768,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 P_SZ = sz / r;
             }
             p.E = p.E * 1.0;
-            p.tau0 = -log(g.next());
+            p.tau0 = TAUDRAW(g.next());
             p.tau = 0.0;
             p.ntrace = 0;
             st = S_STEP;
@@ -815,6 +849,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     atomicAdd(&ty.Ni[c], 1);
                 }
                 P_VSLOT = (double)slot;
+                P_JW = -1.0;
                 P_OWN = __longlong_as_double((long long)own);
                 P_BASE = (double)nS * P_FS;
                 P_TG = (nTg > 0.f) ? (double)(nTg * __powf((float)P_FS, 0.2f)) : 0.0;
@@ -968,14 +1003,29 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     if (P_VSLOT >= 0.0 && P_VSLOT < (double)(1 << VIS_LOG2)) P_VSLOT += (double)(1 << VIS_LOG2);
                     const double base = P_BASE;
                     const double Eb = (base + (double)o0) * A.unitE, E1 = (base + (double)o1) * A.unitE;
-                    jw = win_start((c_Tl >= 1.0) ? c_Tl : P_TG);
-                    w0 = kpc[jw];
-                    w1 = kpc[jw + 1];
-                    w2 = kpc[jw + 2];
-                    if (ONE) {
-                        q0 = Kp0[jw];
-                        q1 = Kp0[jw + 1];
-                        q2 = Kp0[jw + 2];
+                    // the window of the previous absorption in this visit
+                    // still brackets both energies in a long walk inside the
+                    // cell: no load
+                    if (!ONE && P_JW >= 1.0 && P_W0 <= Eb / mass && E1 / mass <= P_W2) {
+                        jw = (int)P_JW;
+                        w0 = P_W0;
+                        w1 = P_W1;
+                        w2 = P_W2;
+                    } else {
+                        jw = win_start((c_Tl >= 1.0) ? c_Tl : P_TG);
+                        w0 = kpc[jw];
+                        w1 = kpc[jw + 1];
+                        w2 = kpc[jw + 2];
+                        if (ONE) {
+                            q0 = Kp0[jw];
+                            q1 = Kp0[jw + 1];
+                            q2 = Kp0[jw + 2];
+                        } else {
+                            P_JW = (double)jw;
+                            P_W0 = w0;
+                            P_W1 = w1;
+                            P_W2 = w2;
+                        }
                     }
                     if (Eb != 0.0) T0 = dev_T_win(kpc, jw, w0, w1, w2, Eb / mass);
                     T1 = dev_T_win(kpc, jw, w0, w1, w2, E1 / mass);
@@ -1023,7 +1073,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 }
                 // emit(): new direction, then the wavelength
                 CNT(C_NEMIT);
-                random_direction(g, p.vx, p.vy, p.vz);
+                RDIR(g, p.vx, p.vy, p.vz);
                 {
                     double sx = -p.vy, sy = p.vx, sz = 0.0;
                     double r = sqrt(sx * sx + sy * sy + sz * sz);
@@ -1066,7 +1116,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     }
                 }
                 if (tb.scat_how == 1) {
-                    random_direction(g, p.vx, p.vy, p.vz);
+                    RDIR(g, p.vx, p.vy, p.vz);
                     st = S_AFTER;
                 } else {
                     // pick the (species, lambda) Mueller matrix (Radiation.f:273-290)
@@ -1095,30 +1145,37 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                         // (no early exit), so the loads are in flight together;
                         // the first running sum below zero picks, as in the
                         // reference's loop
+                        // the integrals IF11, IF12 of every candidate are
+                        // loaded with it, so the pick costs one round trip
                         bool found = false;
+                        MIF11 = tb.IF11[moff];
+                        MIF12 = tb.IF12[moff];
 #pragma unroll 4
                         for (int s = 0; s < ng; s++) {
-                            const double ws = wc[s], k2 = tb.Ksca[(size_t)s * nlam + l2],
-                                         k1 = tb.Ksca[(size_t)s * nlam + l1];
+                            const size_t o1 = (size_t)s * nlam + l1, o2 = o1 + 1;
+                            const double ws = wc[s], k2 = tb.Ksca[o2], k1 = tb.Ksca[o1];
+                            const double a2 = tb.IF11[o2], b2 = tb.IF12[o2], a1 = tb.IF11[o1], b1 = tb.IF12[o1];
                             double w1 = ws * 1.0;
                             const double w2 = (w1 * wl2) * k2;
                             w1 = (w1 * p.wl1) * k1;
                             if (!found) {
                                 KscaR = KscaR - w2;
                                 if (KscaR < 0.0) {
-                                    moff = (size_t)s * nlam + l2;
+                                    moff = o2;
+                                    MIF11 = a2;
+                                    MIF12 = b2;
                                     found = true;
                                 } else {
                                     KscaR = KscaR - w1;
                                     if (KscaR < 0.0) {
-                                        moff = (size_t)s * nlam + l1;
+                                        moff = o1;
+                                        MIF11 = a1;
+                                        MIF12 = b1;
                                         found = true;
                                     }
                                 }
                             }
                         }
-                        MIF11 = tb.IF11[moff];
-                        MIF12 = tb.IF12[moff];
                     }
                     // scatangle (Radiation.f:381-495)
                     double IF11 = MIF11 * 180.0, IF12 = MIF12 * 180.0;
@@ -1147,11 +1204,33 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                         phi = g.next() * MCX_PI;
                     }
                     double sx = P_SX, sy = P_SY, sz = P_SZ;
-                    rotate_about(sx, sy, sz, p.vx, p.vy, p.vz, phi);
+                    if constexpr (F32) rotate_about_f(sx, sy, sz, p.vx, p.vy, p.vz, phi);
+                    else rotate_about(sx, sy, sz, p.vx, p.vy, p.vz, phi);
                     if (p.pol) {
                         // RotateStokes (Radiation.f:506-534)
                         double P = (P_U / p.E) * (P_U / p.E) + (P_Q / p.E) * (P_Q / p.E);
-                        if (P != 0.0) {
+                        if (P != 0.0 && F32) {
+                            const float fE = (float)p.E, fvx = (float)p.vx, fvy = (float)p.vy, fvz = (float)p.vz;
+                            const float fSX = (float)P_SX, fSY = (float)P_SY, fSZ = (float)P_SZ;
+                            const float fsx = (float)sx, fsy = (float)sy, fsz = (float)sz;
+                            const float fQ = (float)P_Q, fU = (float)P_U;
+                            const float cost = fSX * fsx + fSY * fsy + fSZ * fsz;
+                            // the denominator (v x S)_x can round to zero in single
+                            // precision: then the Stokes vector keeps its frame
+                            const float den = fvy * fSZ - fvz * fSY;
+                            const float sint = (fsx - (fSX * (fvy * fvy + fvz * fvz) - fvx * (fvy * fSY + fvz * fSZ)) *
+                                                          cost) / (den != 0.0f ? den : 1.0f);
+                            const float cos2t = cost * cost - sint * sint;
+                            const float sin2t = (2.0f * sint) * cost;
+                            const float Qn = fQ * cos2t + fU * sin2t;
+                            const float Un = -fQ * sin2t + fU * cos2t;
+                            const float P2 = (Un / fE) * (Un / fE) + (Qn / fE) * (Qn / fE);
+                            const float sPP2 = sqrtf((float)P / P2);
+                            if (den != 0.0f && isfinite(sPP2 * Un) && isfinite(sPP2 * Qn)) {
+                                P_U = (double)(sPP2 * Un);
+                                P_Q = (double)(sPP2 * Qn);
+                            }
+                        } else if (P != 0.0) {
                             double cost = P_SX * sx + P_SY * sy + P_SZ * sz;
                             double sint = (sx - (P_SX * (p.vy * p.vy + p.vz * p.vz) -
                                                  p.vx * (p.vy * P_SY + p.vz * P_SZ)) * cost) /
@@ -1172,8 +1251,8 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     double Fr = 180.0 * (p.E * MIF11 + P_Q * MIF12);
                     Fr = g.next() * Fr;
                     // first angle bin whose cumulative E F11 + Q F12 exceeds Fr
-                    lk.R1 = tb.C11 + moff * 180;
-                    lk.R0 = tb.C12 + moff * 180;
+                    lk.R1 = tb.SC + moff * 180 * 8;
+                    lk.R0 = lk.R1;
                     lk.e1 = p.E;
                     lk.e0 = P_Q;
                     lk.ws = 1.0;
@@ -1236,34 +1315,43 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     theta = (theta * MCX_PI) / 180.0;
                 }
                 if (g.next() < 0.5) theta = -theta;
-                const size_t moff = (size_t)(lk.R1 - tb.C11) / 180;
-                const double *M = tb.F + moff * 6 * 180;
-                const double *F11 = M, *F12 = M + 180, *F22 = M + 360, *F33 = M + 540, *F34 = M + 720,
-                             *F44 = M + 900;
+                // the bin's Mueller elements share the line of its cumulatives
+                const double *Mb = lk.R1 + 8 * (ia - 1) + 2;
+                const double f11 = Mb[0], f12 = Mb[1], f22 = Mb[2], f33 = Mb[3], f34 = Mb[4], f44 = Mb[5];
                 double En = 0.0, Qn = 0.0, Un = 0.0, Vn = 0.0;
-                En = En + F11[ia - 1] * p.E;
-                En = En + F12[ia - 1] * P_Q;
-                Qn = Qn + F12[ia - 1] * p.E;
-                Qn = Qn + F22[ia - 1] * P_Q;
-                Un = Un + F33[ia - 1] * P_U;
-                double Fs = (theta >= 0.0) ? fabs(F34[ia - 1]) : -fabs(F34[ia - 1]);
+                En = En + f11 * p.E;
+                En = En + f12 * P_Q;
+                Qn = Qn + f12 * p.E;
+                Qn = Qn + f22 * P_Q;
+                Un = Un + f33 * P_U;
+                double Fs = (theta >= 0.0) ? fabs(f34) : -fabs(f34);
                 Un = Un + Fs * P_V;
                 Vn = Vn - Fs * P_U;
-                Vn = Vn + F44[ia - 1] * P_V;
+                Vn = Vn + f44 * P_V;
                 if (En != 0.0) {
                     P_Q = (p.E * Qn) / En;
                     P_U = (p.E * Un) / En;
                     P_V = (p.E * Vn) / En;
                 }
-                rotate_about(p.vx, p.vy, p.vz, P_SX, P_SY, P_SZ, theta);
-                double r = sqrt(p.vx * p.vx + p.vy * p.vy + p.vz * p.vz);
-                p.vx /= r;
-                p.vy /= r;
-                p.vz /= r;
-                r = sqrt(P_SX * P_SX + P_SY * P_SY + P_SZ * P_SZ);
-                P_SX /= r;
-                P_SY /= r;
-                P_SZ /= r;
+                if constexpr (F32) {
+                    rotate_about_f(p.vx, p.vy, p.vz, P_SX, P_SY, P_SZ, theta);
+                    normalize_f(p.vx, p.vy, p.vz);
+                    double sx = P_SX, sy = P_SY, sz = P_SZ;
+                    normalize_f(sx, sy, sz);
+                    P_SX = sx;
+                    P_SY = sy;
+                    P_SZ = sz;
+                } else {
+                    rotate_about(p.vx, p.vy, p.vz, P_SX, P_SY, P_SZ, theta);
+                    double r = sqrt(p.vx * p.vx + p.vy * p.vy + p.vz * p.vz);
+                    p.vx /= r;
+                    p.vy /= r;
+                    p.vz /= r;
+                    r = sqrt(P_SX * P_SX + P_SY * P_SY + P_SZ * P_SZ);
+                    P_SX /= r;
+                    P_SY /= r;
+                    P_SZ /= r;
+                }
                 if (ia != 1 && ia != 180) p.pol = true;
             }
             st = S_AFTER;
@@ -1281,7 +1369,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 }
             }
             if (st == S_STEP) {
-                p.tau0 = -log(g.next());
+                p.tau0 = TAUDRAW(g.next());
                 p.tau = 0.0;
                 p.ntrace = 0;
                 // trace2d starts with the random walk in thick cells (MCRadiation.f:916)
@@ -1333,7 +1421,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 // single-precision ratio times 0.75
                 const double yv = (iy == 1) ? 1.0 : (double)((float)(NY_RW - iy + 1) / (float)(NY_RW - 1)) * 0.75;
                 const double v = -((3.0 * log(yv)) * (dmin * dmin)) / (lr * (MCX_PI * MCX_PI));
-                random_direction(g, p.vx, p.vy, p.vz);
+                RDIR(g, p.vx, p.vy, p.vz);
                 p.x = p.x + dmin * p.vx;
                 p.y = p.y + dmin * p.vy;
                 p.z = p.z + dmin * p.vz;
@@ -1359,7 +1447,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 // leave in a direction of the forward hemisphere (MCRadiation.f:919-925)
                 double vx, vy, vz;
                 for (;;) {
-                    random_direction(g, vx, vy, vz);
+                    RDIR(g, vx, vy, vz);
                     const double inp = p.vx * vx + p.vy * vy + p.vz * vz;
                     if (!(inp < 0.0)) break;
                 }
@@ -1414,6 +1502,10 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
 #undef P_OWN
 #undef P_VSLOT
 #undef P_TG
+#undef P_JW
+#undef P_W0
+#undef P_W1
+#undef P_W2
 }
 
 // ---------------------------------------------------------------------------
@@ -1541,6 +1633,8 @@ __global__ void k_fill_u4(size_t n, uint4 *p) {
 }
 
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+
+__global__ void k_clock(unsigned long long *p) { *p = wall_clock64(); }
 
 // raw tallies from the fixed-point sums; Tbw = the cell's Bjorkman-Wood T
 // after the pass, T(Eabs/m) (the serial loop's last increaseT)
@@ -1708,12 +1802,12 @@ extern "C" int mcx_set_tables(mcx_ctx *ctx, const mcx_tables *t) {
             }
         }
     }
-    // scattering-angle cumulatives: sum_{m<=a} pi sin(theta_m) F1x(m)
-    std::vector<double> C11, C12;
+    // scattering-angle cumulatives sum_{m<=a} pi sin(theta_m) F1x(m), each
+    // bin's record beside that bin's Mueller elements (one 64-byte line)
+    std::vector<double> SC;
     const bool full = (t->scat_how == 2);
     if (full) {
-        C11.resize((size_t)ng * nlam * 180);
-        C12.resize((size_t)ng * nlam * 180);
+        SC.resize((size_t)ng * nlam * 180 * 8);
         for (size_t sl = 0; sl < (size_t)ng * nlam; sl++) {
             const double *M = t->F + sl * 6 * 180;
             double a11 = 0.0, a12 = 0.0;
@@ -1722,8 +1816,10 @@ extern "C" int mcx_set_tables(mcx_ctx *ctx, const mcx_tables *t) {
                 double ps = MCX_PI * sin(thet);
                 a11 = a11 + ps * M[a - 1];
                 a12 = a12 + ps * M[180 + a - 1];
-                C11[sl * 180 + a - 1] = a11;
-                C12[sl * 180 + a - 1] = a12;
+                double *r = SC.data() + (sl * 180 + a - 1) * 8;
+                r[0] = a11;
+                r[1] = a12;
+                for (int k = 0; k < 6; k++) r[2 + k] = M[k * 180 + a - 1];
             }
         }
     }
@@ -1766,10 +1862,9 @@ extern "C" int mcx_set_tables(mcx_ctx *ctx, const mcx_tables *t) {
     }
     if (full) {
         UP(tb.F, t->F, (size_t)ng * nlam * 6 * 180);
-        UP(tb.C11, C11.data(), C11.size());
-        UP(tb.C12, C12.data(), C12.size());
+        UP(tb.SC, SC.data(), SC.size());
     } else {
-        tb.F = tb.C11 = tb.C12 = nullptr;
+        tb.F = tb.SC = nullptr;
     }
     UP(ctx->d_Kabs, t->Kabs, (size_t)ng * nlam);
     ctx->d_Ksca = (double *)tb.Ksca;
@@ -1942,6 +2037,7 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
     if ((r = dalloc(ctx, ctx->mod_alloc, &ty.ispec, (size_t)2 * (NANGD + 1) * nlam))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ty.counters, G_NSLOTS))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ty.tautot, 1))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ty.torep, 16))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Ni, ncell))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Esnap, ncell))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Tst, ncell))) return r;
@@ -1971,7 +2067,7 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     if (!ctx || !run) return MCX_EINVAL;
     if (!ctx->have_model) return fail(ctx, MCX_ESTATE, "mcx_set_model first");
     if (run->nphot <= 0 || run->nphot >= (1ll << 31)) return fail(ctx, MCX_EINVAL, "nphot must be in 1 .. 2^31-1");
-    if (run->precision != MCX_FP64) return fail(ctx, MCX_EINVAL, "unknown precision mode");
+    if (run->precision != MCX_FP64 && run->precision != MCX_FP32) return fail(ctx, MCX_EINVAL, "unknown precision mode");
     if (run->stage0 < 0 || run->stage_ratio < -1 || run->stage_ratio == 1)
         return fail(ctx, MCX_EINVAL, "stage0 >= 0 and stage_ratio -1, 0 or >= 2 required");
     HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -2000,15 +2096,21 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     ctx->E0 = E0;
     ctx->unitE = ldexp(E0, -EABS_BITS);
     // the kernel for the model's species (one; several with class tables; several)
-    void (*kern)(TransportArgs) = (ctx->ng == 1) ? k_transport<0> : md.CUMmix ? k_transport<1> : k_transport<2>;
+    void (*kern)(TransportArgs);
+    if (run->precision == MCX_FP32)
+        kern = (ctx->ng == 1) ? k_transport<0, true> : md.CUMmix ? k_transport<1, true> : k_transport<2, true>;
+    else
+        kern = (ctx->ng == 1) ? k_transport<0, false> : md.CUMmix ? k_transport<1, false> : k_transport<2, false>;
     // launch shape: a full device of resident wavefronts, or one lane
     int threads = serial ? 64 : 256; // one wavefront: lane 0 traces, the others serve its lookups
     size_t lds = sizeof(double) * ((size_t)(md.nr + 1) + 2 * (size_t)(md.nt + 1) + 2 * 181 + (size_t)(nlam - 1) +
                                    NPL * (size_t)threads);
     if (rw.on) lds += sizeof(double) * ((size_t)(md.nr + 1) + (size_t)(md.nt + 1));
     // rings of Philox draws: 16 per thread when they fit beside the rest
+    // with two workgroups per CU (half the CU's 160 KiB of LDS), else fewer
     int rb = 16;
-    while (rb > 4 && lds + sizeof(double) * (size_t)rb * threads > ctx->lds_max) rb >>= 1;
+    const size_t lds_two = std::min(ctx->lds_max, (size_t)80 * 1024);
+    while (rb > 4 && lds + sizeof(double) * (size_t)rb * threads > (serial ? ctx->lds_max : lds_two)) rb >>= 1;
     lds += sizeof(double) * (size_t)rb * threads;
     if (lds > ctx->lds_max)
         return fail(ctx, MCX_EINVAL, "grid too large for the kernel's LDS tables (" + std::to_string(lds) + " bytes)");
@@ -2057,6 +2159,14 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     A.nphot = run->nphot;
     A.seed = run->seed;
     A.maxinteract = run->maxinteract;
+    A.deadline = 0ull;
+    if (run->max_ms > 0.0 && ctx->wclk_khz > 0) {
+        unsigned long long now = 0;
+        hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, ty.counters + G_CLOCK);
+        HIPCHK(ctx, hipMemcpyAsync(&now, ty.counters + G_CLOCK, sizeof(now), hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        A.deadline = now + (unsigned long long)(run->max_ms * (double)ctx->wclk_khz);
+    }
     A.invE = ldexp(1.0, EABS_BITS) / E0;
     A.unitE = ctx->unitE;
     A.serial = serial ? 1 : 0;
@@ -2094,6 +2204,11 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
         HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
         kms_total += ms;
         if (stage_log) fprintf(stderr, "[mcx] stage [%lld, %lld): %.1f ms\n", Pa, Pb, ms);
+        if (A.deadline) {
+            unsigned long long nto = 0;
+            HIPCHK(ctx, hipMemcpy(&nto, ty.counters + C_NTIMEOUT, sizeof(nto), hipMemcpyDeviceToHost));
+            if (nto) break;
+        }
         Pa = Pb;
         Pb = stage_next(Pb, N, s0, ratio);
     }
@@ -2110,6 +2225,17 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     ctx->have_pass = true;
     unsigned long long cnt[G_NSLOTS];
     HIPCHK(ctx, hipMemcpy(cnt, ty.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (cnt[C_NTIMEOUT] > 0) {
+        double d[16];
+        HIPCHK(ctx, hipMemcpy(d, ty.torep, sizeof(d), hipMemcpyDeviceToHost));
+        char buf[512];
+        snprintf(buf, sizeof(buf),
+                 "; first abandoned: packet %.0f state %.0f after %.0f cell steps in cell (%.0f, %.0f) at (%.9g, %.9g, "
+                 "%.9g) AU, direction (%.9g, %.9g, %.9g), onEdge %.0f edge %.0f, tau %.6g of %.6g, %.0f crossings",
+                 d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11], d[12], d[14], d[13], d[15]);
+        return fail(ctx, MCX_ETIME,
+                    "time limit: " + std::to_string(cnt[C_NTIMEOUT]) + " packets unfinished" + std::string(buf));
+    }
     if (cnt[C_NGEOM] > 0) return fail(ctx, MCX_EGEOM, std::to_string(cnt[C_NGEOM]) + " packets found nothing to hit");
     return MCX_OK;
 }
@@ -2175,6 +2301,7 @@ extern "C" int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *o) {
     o->Eescape = Eesc;
     o->maxsteps = (int64_t)cnt[G_MAXSTEPS];
     o->nvisit_overflow = (int64_t)cnt[C_NVISOVF];
+    o->ntimeout = (int64_t)cnt[C_NTIMEOUT];
     for (int k = 0; k < 5; k++) {
         const long long t0 = (long long)cnt[G_TLINE], t1 = (long long)cnt[G_TLINE + 1 + k];
         o->t_done[k] = (t1 != 0 && ctx->wclk_khz > 0) ? (double)(t1 - t0) / (double)ctx->wclk_khz : -1.0;

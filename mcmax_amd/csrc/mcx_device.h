@@ -211,6 +211,58 @@ __device__ __forceinline__ void rotate_about(double &x, double &y, double &z, do
     z = y3;
 }
 
+// ---------------------------------------------------------------------------
+// Single-precision counterparts of the direction arithmetic, for transport in
+// FP32 (precision mode MCX_FP32): hardware sin/cos, log and reciprocal square
+// root instead of the FP64 library sequences.  Positions, crossings, optical
+// depths, tallies and temperatures stay FP64.
+// The crossing quadratics take |v| = 1 (leading coefficient 1, and the
+// second root -b of a packet on its own surface), so the FP32 directions
+// become unit vectors in double before they reach the geometry: one FP64
+// reciprocal square root instead of the FP64 trigonometry.
+__device__ __forceinline__ void to_unit(double &x, double &y, double &z, double a, double b, double c) {
+    const double q = 1.0 / sqrt(a * a + b * b + c * c);
+    x = a * q;
+    y = b * q;
+    z = c * q;
+}
+
+template <typename Gen>
+__device__ __forceinline__ void random_direction_f(Gen &g, double &x, double &y, double &z) {
+    float a, b, c, r;
+    for (;;) {
+        a = 2.0f * (float)g.next() - 1.0f;
+        b = 2.0f * (float)g.next() - 1.0f;
+        c = 2.0f * (float)g.next() - 1.0f;
+        r = a * a + b * b + c * c;
+        if (!(r > 1.0f) && r > 0.0f) break;
+    }
+    to_unit(x, y, z, (double)a, (double)b, (double)c);
+}
+
+__device__ __forceinline__ void rotate_about_f(double &x, double &y, double &z, double u_, double v_, double w_,
+                                               double theta) {
+    float sint, cost;
+    __sincosf((float)theta, &sint, &cost);
+    const float u = (float)u_, v = (float)v_, w = (float)w_;
+    const float X = (float)x, Y = (float)y, Z = (float)z;
+    const float u2 = u * u, v2 = v * v, w2 = w * w;
+    const float inp = X * u + Y * v + Z * w;
+    x = (double)(u * inp + (X * (v2 + w2) - u * (v * Y + w * Z)) * cost + (v * Z - w * Y) * sint);
+    y = (double)(v * inp + (Y * (u2 + w2) - v * (u * X + w * Z)) * cost + (w * X - u * Z) * sint);
+    z = (double)(w * inp + (Z * (u2 + v2) - w * (u * X + v * Y)) * cost + (u * Y - v * X) * sint);
+}
+
+// unit vector (in double, see to_unit)
+__device__ __forceinline__ void normalize_f(double &x, double &y, double &z) { to_unit(x, y, z, x, y, z); }
+
+// the optical depth to the next interaction, -ln(u), in FP32 (u rounded to 1
+// in float: the first-order 1 - u)
+__device__ __forceinline__ double tau_draw_f(double u) {
+    const float uf = (float)u;
+    return (uf < 1.0f) ? (double)(-__logf(uf)) : 1.0 - u;
+}
+
 // (real(i))**4 in single precision, as the reference evaluates it in the
 // T^4 interpolation (Radiation.f:551, 698).
 __device__ __forceinline__ float fpow4(int i) {

@@ -26,7 +26,7 @@
 #define EABS_BITS 24  // one nominal packet energy maps to 2^24
 
 enum { C_NEMIT = 0, C_NINTERACT, C_NABS, C_NSCAT, C_NESCAPE, C_NSTARHIT, C_NSTEPS, C_NRAAR, C_NMAXINT,
-       C_NGEOM, C_NOVERFLOW, C_NRW, C_NVISOVF, C_NCOUNT };
+       C_NGEOM, C_NOVERFLOW, C_NRW, C_NVISOVF, C_NTIMEOUT, C_NCOUNT };
 
 // per-lane table of the cells the lane's current packet has entered (Ni's
 // distinct-packet count and the packet's own absorbed energy per cell):
@@ -41,6 +41,8 @@ enum { C_NEMIT = 0, C_NINTERACT, C_NABS, C_NSCAT, C_NESCAPE, C_NSTARHIT, C_NSTEP
 #define G_MAXSTEPS 14
 #define G_DONE 15
 #define G_TLINE 16
+#define G_CLOCK 22
+#define G_TOREP 23 // set by the first packet abandoned at the time limit
 #define G_NSLOTS 24
 
 #define NY_RW 1000 // survival-table length (RandomWalk.f:6-7)
@@ -54,7 +56,9 @@ struct DevTables {
     const double *Kp;        // [ng][TMAX+1]
     const double *Ksca;      // [ng][nlam]
     const double *F;         // [ng][nlam][6][180]
-    const double *C11, *C12; // [ng][nlam][180]
+    const double *SC;        // [ng][nlam][180][8] per angle bin: the cumulatives
+                             // sum pi sin(theta) F11, F12 up to the bin, then
+                             // F11, F12, F22, F33, F34, F44 of the bin (one line)
     const double *IF11, *IF12;
     const double *cos2phi, *sin2phi; // [361]
     const double *pc2, *ps2;         // [181] prefix sums of cos 2phi, sin 2phi (bins 1..k)
@@ -87,6 +91,7 @@ struct DevTally {
     unsigned *ispec;            // [2][(NANG+1)*nlam]
     unsigned long long *counters;
     double *tautot;
+    double *torep;  // [16] the first abandoned packet's state (time limit)
 };
 
 // modified random walk (RandomWalk.f)

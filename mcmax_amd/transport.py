@@ -73,17 +73,19 @@ class Transport:
         self.model = m
 
     def transport(self, nphot, seed=1, nlanes=0, maxinteract=0, randomwalk=None, factRW=None, coop_lanes=-1,
-                  stage0=0, stage_ratio=0):
+                  stage0=0, stage_ratio=0, precision=0, max_ms=0.0):
         """One MC pass (mcx_transport); tallies stay on the device.  The
         random-walk switches default to the model's keywords.  nlanes=1 runs
         the serial loop on one lane; otherwise the staged schedule (stage0,
-        stage_ratio; 0 picks 64 and 2)."""
+        stage_ratio; 0 picks 64 and 2).  precision 1: FP32 direction
+        arithmetic (MCX_FP32).  max_ms > 0: the pass's time limit (MCX_ETIME
+        raises)."""
         m = self.model
         rw = bool(getattr(m, "randomwalk", False)) if randomwalk is None else bool(randomwalk)
         f = float(getattr(m, "factRW", 10.0)) if factRW is None else float(factRW)
         r = _lib.Run(nphot=int(nphot), seed=int(seed), maxinteract=int(maxinteract), nlanes=int(nlanes),
                      stage0=int(stage0), randomwalk=int(rw), factRW=f, coop_lanes=int(coop_lanes),
-                     stage_ratio=int(stage_ratio), precision=0)
+                     stage_ratio=int(stage_ratio), precision=int(precision), max_ms=float(max_ms))
         rc = self.lib.mcx_transport(self.ctx, C.byref(r))
         if rc not in (0, _lib.MCX_EGEOM):
             self._check(rc)
@@ -109,7 +111,7 @@ class Transport:
             out[k] = out[k].reshape(NANGLE + 1, m.nlam)
         for k in ("nemit", "ninteract", "nabs", "nscat", "nescape", "nstarhit", "nsteps", "nraar",
                   "nmaxinteract", "ngeomfail", "tautot", "Eescape", "nrw", "noverflow", "maxsteps",
-                  "nvisit_overflow"):
+                  "nvisit_overflow", "ntimeout"):
             out[k] = getattr(t, k)
         out["t_done_ms"] = list(t.t_done)
         return out
@@ -290,11 +292,12 @@ def make_obs(angle_deg=45.0, nphi=45, nj0=2, nphot=1000, nphotstar=1000, nexits=
                tracescat=int(tracescat), pfstop=pfstop, seed=seed)
 
 
-def mc_radiation(tr, nphot, seed=1, nlanes=0, maxinteract=0, coop_lanes=-1, stage0=0, stage_ratio=0):
+def mc_radiation(tr, nphot, seed=1, nlanes=0, maxinteract=0, coop_lanes=-1, stage0=0, stage_ratio=0, precision=0,
+                 max_ms=0.0):
     """MCRadiation + temperature inversion; returns a dict with the tallies
     and T, dT, dEJv."""
     rc = tr.transport(nphot, seed=seed, nlanes=nlanes, maxinteract=maxinteract, coop_lanes=coop_lanes,
-                      stage0=stage0, stage_ratio=stage_ratio)
+                      stage0=stage0, stage_ratio=stage_ratio, precision=precision, max_ms=max_ms)
     out = tr.tallies()
     out["T"], out["dT"], out["dEJv"] = tr.temperature()
     out["rc"] = rc

@@ -327,7 +327,7 @@ def _multi_seed(gpu, m, n, seeds):
     gpu.set_model(m)
     runs = []
     for s in seeds:
-        g = transport.mc_radiation(gpu, n, seed=s)
+        g = transport.mc_radiation(gpu, n, seed=s, max_ms=120000.0)
         assert g["noverflow"] == 0 and g["ngeomfail"] == 0 and g["nvisit_overflow"] == 0
         assert g["nescape"] == n and abs(g["Eescape"] / m.Lstar - 1) < 1e-9
         runs.append(g)
@@ -398,3 +398,17 @@ def test_staged_statistics_isotropic(gpu, tiny_iso_model, oracle_lib):
     fx = {k: np.asarray(v) for k, v in fx.items()}
     runs = _multi_seed(gpu, m, 20000, range(101, 113))
     _check_stats(staged_stats.compare(runs, fx))
+
+
+def test_time_limit(gpu, tiny_thick_model):
+    """mcx_run.max_ms: a pass past its time limit stops every lane, returns
+    MCX_ETIME and counts the abandoned packets; the next pass runs normally."""
+    from mcmax_amd import _lib
+    gpu.set_model(tiny_thick_model)
+    with pytest.raises(transport.MCXError) as e:
+        gpu.transport(2000000, seed=3, max_ms=1.0)
+    assert e.value.code == _lib.MCX_ETIME
+    t = gpu.tallies()
+    assert t["ntimeout"] > 0 and t["nescape"] < 2000000
+    g = transport.mc_radiation(gpu, 3000, seed=3, max_ms=60000.0)
+    assert g["ntimeout"] == 0 and g["nescape"] == 3000
