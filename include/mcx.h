@@ -194,6 +194,23 @@ int mcx_set_model(mcx_ctx *ctx, const mcx_grid *grid);
  * reference's "nothing to hit" stop (the pass still completes). */
 int mcx_transport(mcx_ctx *ctx, const mcx_run *run);
 
+/* Many disk models in common launches (no counterpart in the reference,
+ * which holds one model in its module globals, Modules.f:141-150, and runs
+ * MCRadiation for it, MCRadiation.f:6).  Each of the nmodels contexts holds
+ * one model (mcx_set_tables + mcx_set_model, grids of any size, same species
+ * case); one pass of run->nphot packets runs for all of them, each stage in
+ * one launch with blocks_per_model workgroups per model (0: the device's
+ * resident workgroups shared out), on the first context's stream.  The
+ * long histories that end each stage then run side by side instead of one
+ * model after another.  Model m uses the Philox key run->seed + m *
+ * seed_stride; its tallies are bit-identical to those of mcx_transport on
+ * its own context with that key.  Read them with mcx_get_tallies /
+ * mcx_temperature on the model's context.  status[m] (may be NULL)
+ * receives model m's code (MCX_OK, MCX_EGEOM, MCX_ETIME); the call returns
+ * the first failure.  One-lane runs (nlanes = 1) take one model. */
+int mcx_batch_transport(mcx_ctx *const *ctxs, int32_t nmodels, const mcx_run *run, int32_t blocks_per_model,
+                        int64_t seed_stride, int32_t *status);
+
 /* Copy the last pass's tallies to the host (any pointer may be NULL). */
 int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *out);
 

@@ -115,7 +115,7 @@ EXPORTS = (
     "mcx_create", "mcx_destroy", "mcx_last_error", "mcx_set_tables", "mcx_set_model",
     "mcx_transport", "mcx_get_tallies", "mcx_temperature", "mcx_last_timing", "mcx_sed",
     "mcx_set_temperature", "mcx_diffuse", "mcx_temp_average", "mcx_disk_structure", "mcx_compare_cells",
-    "mcx_compute_parts", "mcx_image", "mcx_last_image_timing",
+    "mcx_compute_parts", "mcx_image", "mcx_last_image_timing", "mcx_batch_transport",
 )
 
 _lib = None
@@ -140,6 +140,8 @@ def load():
     lib.mcx_set_tables.argtypes = [vp, C.POINTER(Tables)]
     lib.mcx_set_model.argtypes = [vp, C.POINTER(Grid)]
     lib.mcx_transport.argtypes = [vp, C.POINTER(Run)]
+    lib.mcx_batch_transport.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Run), C.c_int32, C.c_int64,
+                                        C.POINTER(C.c_int32)]
     lib.mcx_get_tallies.argtypes = [vp, C.POINTER(Tallies)]
     lib.mcx_temperature.argtypes = [vp, C.POINTER(Temps)]
     lib.mcx_last_timing.argtypes = [vp, _dp, _dp]
@@ -158,7 +160,7 @@ def load():
     for n in ("mcx_set_tables", "mcx_set_model", "mcx_transport", "mcx_get_tallies",
               "mcx_temperature", "mcx_last_timing", "mcx_sed", "mcx_set_temperature", "mcx_diffuse",
               "mcx_temp_average", "mcx_disk_structure", "mcx_compare_cells", "mcx_compute_parts", "mcx_image",
-              "mcx_last_image_timing"):
+              "mcx_last_image_timing", "mcx_batch_transport"):
         getattr(lib, n).restype = C.c_int
     _lib = lib
     return lib

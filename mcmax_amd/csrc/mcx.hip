@@ -33,24 +33,32 @@
 //    (Dtable.f:97-182) whenever a cell's temperature moves.
 #include "mcx_internal.h"
 
-struct TransportArgs {
+// One model of a launch: its tables, grid, tallies, random-walk data, energy
+// units and packet counter.  A launch runs one model (a plain pass) or many
+// (mcx_batch_transport): blocks [m*bpm, (m+1)*bpm) serve model m.
+struct BatchModel {
     DevTables tb;
     DevModel md;
     DevTally ty;
     DevRW rw;
     unsigned long long *next_pid;
+    double invE;   // 2^EABS_BITS / E0
+    double unitE;  // E0 / 2^EABS_BITS
+    unsigned long long seed; // the model's Philox key
+};
+
+struct TransportArgs {
+    const BatchModel *bm; // [models]
+    int bpm;              // blocks per model
     long long pid_end;
     long long pid_base;
     long long Pk;  // packets before the stage (the snapshot's count; 0: none)
     long long nphot;
-    unsigned long long seed;
     int maxinteract;
     unsigned long long deadline; // wall_clock64() past which lanes abandon their packets (0: none)
     int serial;    // only lane 0 of the grid takes packets (in packet order)
     int coop;      // wave-cooperative lookups when at most this many lanes need one
     int rb_mask;   // draws per lane ring minus one (a power of two minus one)
-    double invE;   // 2^EABS_BITS / E0
-    double unitE;  // E0 / 2^EABS_BITS
 };
 
 enum { S_NEED = 0, S_EMIT, S_STEP, S_INTERACT, S_LOOKUP, S_AFTER, S_ESCAPE, S_RW, S_DONE };
@@ -495,9 +503,10 @@ struct RingRng {
 template <int MODE, bool F32>
 __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     extern __shared__ double lds[];
-    const DevTables &tb = A.tb;
-    const DevModel &md = A.md;
-    const DevTally &ty = A.ty;
+    const BatchModel &Bm = A.bm[blockIdx.x / A.bpm];
+    const DevTables &tb = Bm.tb;
+    const DevModel &md = Bm.md;
+    const DevTally &ty = Bm.ty;
     const int nr = md.nr, nt = md.nt, nlam = tb.nlam, ng = tb.ng;
     constexpr bool ONE = MODE == 0, MIX = MODE == 1;
     // species summed per reemission probe (class tables: one row, as for one
@@ -553,9 +562,9 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
         sPS2[k] = tb.ps2[k];
     }
     for (int k = threadIdx.x; k < nbins; k += blockDim.x) sCS[k] = tb.cumStar[k];
-    if (A.rw.on) {
-        for (int k = threadIdx.x; k <= nr; k += blockDim.x) sRau[k] = A.rw.R[k];
-        for (int k = threadIdx.x; k <= nt; k += blockDim.x) sSinT[k] = A.rw.sinT[k];
+    if (Bm.rw.on) {
+        for (int k = threadIdx.x; k <= nr; k += blockDim.x) sRau[k] = Bm.rw.R[k];
+        for (int k = threadIdx.x; k <= nt; k += blockDim.x) sSinT[k] = Bm.rw.sinT[k];
     }
     __syncthreads();
 
@@ -575,8 +584,8 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     RingRng g;
     g.ring = sRing + (size_t)threadIdx.x * RB;
     g.mask = RB - 1u;
-    g.k0 = (uint32_t)A.seed;
-    g.k1 = (uint32_t)(A.seed >> 32);
+    g.k0 = (uint32_t)Bm.seed;
+    g.k1 = (uint32_t)(Bm.seed >> 32);
     g.pid = 0;
     g.head = g.fill = 0;
     // requests of the wave's lanes whose ring runs low (rng service)
@@ -643,7 +652,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
             if (mask) {
                 int leader = __ffsll((long long)mask) - 1;
                 unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(A.next_pid, (unsigned long long)__popcll(mask));
+                if (lane == leader) base = atomicAdd(Bm.next_pid, (unsigned long long)__popcll(mask));
                 unsigned lo = __shfl((unsigned)base, leader);
                 unsigned hi = __shfl((unsigned)(base >> 32), leader);
                 base = ((unsigned long long)hi << 32) | lo;
@@ -927,7 +936,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
             if ((double)c != P_IC) {
                 P_IC = (double)c;
                 P_MASS = md.mass[c];
-                P_TC = A.rw.on ? (double)(A.rw.thick[c] | (A.rw.cls[c] << 1)) : 0.0;
+                P_TC = (double)((Bm.rw.on ? Bm.rw.thick[c] : 0) | (md.cls[c] << 1));
                 P_KDIT = -1.0;
                 if (!A.serial) P_TRW = -1.0; // not known yet for this cell
             }
@@ -948,7 +957,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 P_U = 0.0;
                 P_V = 0.0;
                 double Ex = (p.E * (KabsL - 0.0)) / KabsL;
-                const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
+                const double *kpc = md.KpCls + (size_t)((int)P_TC >> 1) * (TMAXD + 1);
                 const double *Kp0 = tb.Kp; // species 0
                 const double mass = P_MASS;
                 int jw;
@@ -974,7 +983,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     const double Thint = __longlong_as_double((long long)__hip_atomic_load(
                         &ty.Tcell[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
                     unsigned long long q;
-                    unsigned long long old = add_eabs(ty, c, Ex, A.invE, q, scnt);
+                    unsigned long long old = add_eabs(ty, c, Ex, Bm.invE, q, scnt);
                     if (jw < 1) {
                         jw = (int)(Thint / 1.0);
                         if (jw < 1) jw = 1;
@@ -988,21 +997,21 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                             q2 = Kp0[jw + 2];
                         }
                     }
-                    if (old != 0ull) T0 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)old * A.unitE) / mass, Thint);
-                    T1 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)(old + q) * A.unitE) / mass, T0);
+                    if (old != 0ull) T0 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)old * Bm.unitE) / mass, Thint);
+                    T1 = dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)(old + q) * Bm.unitE) / mass, T0);
                     atomicMax(&ty.Tcell[c], (unsigned long long)__double_as_longlong(T1));
                     P_TRW = (T1 > Thint) ? T1 : Thint; // the cell's T after this update
                 } else {
                     // staged: the slice (S_k p/P_k + own, + q) of this
                     // packet's view of the cell; the add itself returns nothing
                     // anyone waits for
-                    const unsigned long long q = eabs_q(Ex, A.invE, scnt);
+                    const unsigned long long q = eabs_q(Ex, Bm.invE, scnt);
                     atomicAdd(&ty.Eabs[c], q);
                     const unsigned long long o0 = (unsigned long long)__double_as_longlong(P_OWN), o1 = o0 + q;
                     P_OWN = __longlong_as_double((long long)o1);
                     if (P_VSLOT >= 0.0 && P_VSLOT < (double)(1 << VIS_LOG2)) P_VSLOT += (double)(1 << VIS_LOG2);
                     const double base = P_BASE;
-                    const double Eb = (base + (double)o0) * A.unitE, E1 = (base + (double)o1) * A.unitE;
+                    const double Eb = (base + (double)o0) * Bm.unitE, E1 = (base + (double)o1) * Bm.unitE;
                     // the window of the previous absorption in this visit
                     // still brackets both energies in a long walk inside the
                     // cell: no load
@@ -1085,7 +1094,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 // the wavelength from the cell's mixture spectrum: its class
                 // table, else the species' rows summed per probe (one
                 // species: its own rows)
-                const double *Cb = MIX ? md.CUMmix + (size_t)md.cls[c] * (TMAXD + 1) * nbins : tb.CUM;
+                const double *Cb = MIX ? md.CUMmix + (size_t)((int)P_TC >> 1) * (TMAXD + 1) * nbins : tb.CUM;
                 lk.R0 = Cb + (size_t)iT0 * nbins;
                 lk.R1 = Cb + (size_t)iT1 * nbins;
                 lk.e0 = e0;
@@ -1105,10 +1114,10 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     } else if (P_TRW < 0.0) {
                         // the cell's T as this packet sees it (random-walk test)
                         const double Eb =
-                            (P_BASE + (double)(unsigned long long)__double_as_longlong(P_OWN)) * A.unitE;
+                            (P_BASE + (double)(unsigned long long)__double_as_longlong(P_OWN)) * Bm.unitE;
                         double Tv = 0.0;
                         if (Eb != 0.0) {
-                            const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
+                            const double *kpc = md.KpCls + (size_t)((int)P_TC >> 1) * (TMAXD + 1);
                             const int jw = win_start((c_Tl >= 1.0) ? c_Tl : P_TG);
                             Tv = dev_T_win(kpc, jw, kpc[jw], kpc[jw + 1], kpc[jw + 2], Eb / P_MASS);
                         }
@@ -1379,7 +1388,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
 
         // ---- modified random walk (MCRadiation.f:916-929, RandomWalk.f:47-147)
         if (st == S_RW) {
-            const DevRW &rw = A.rw;
+            const DevRW &rw = Bm.rw;
             const int c = p.i * nt + p.jj;
             // MinDist (RandomWalk.f:15-43)
             double rho = p.x * p.x + p.y * p.y;
@@ -1411,7 +1420,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 CNT(C_NRW);
                 // the window of the cell's table at the walk's start T, fetched
                 // together with the Eabs update
-                const double *kpc = md.KpCell + (size_t)c * (TMAXD + 1);
+                const double *kpc = md.KpCls + (size_t)((int)P_TC >> 1) * (TMAXD + 1);
                 int jw = (int)(Thint / 1.0);
                 if (jw < 1) jw = 1;
                 if (jw > TMAXD - 2) jw = TMAXD - 2;
@@ -1430,18 +1439,18 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 p.nEJv = p.nEJv + ejv; // the module-level nEJv of RandomWalk.f:115
                 if (A.serial) {
                     unsigned long long q;
-                    const unsigned long long old = add_eabs(ty, c, ejv * c_dens, A.invE, q, scnt);
+                    const unsigned long long old = add_eabs(ty, c, ejv * c_dens, Bm.invE, q, scnt);
                     const double T1 =
-                        dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)(old + q) * A.unitE) / P_MASS, Thint);
+                        dev_increaseT_w(kpc, jw, w0, w1, w2, ((double)(old + q) * Bm.unitE) / P_MASS, Thint);
                     atomicMax(&ty.Tcell[c], (unsigned long long)__double_as_longlong(T1));
                     P_TRW = (T1 > Thint) ? T1 : Thint;
                 } else {
-                    const unsigned long long q = eabs_q(ejv * c_dens, A.invE, scnt);
+                    const unsigned long long q = eabs_q(ejv * c_dens, Bm.invE, scnt);
                     atomicAdd(&ty.Eabs[c], q);
                     const unsigned long long o1 = (unsigned long long)__double_as_longlong(P_OWN) + q;
                     P_OWN = __longlong_as_double((long long)o1);
                     if (P_VSLOT >= 0.0 && P_VSLOT < (double)(1 << VIS_LOG2)) P_VSLOT += (double)(1 << VIS_LOG2);
-                    P_TRW = dev_T_win(kpc, jw, w0, w1, w2, ((P_BASE + (double)o1) * A.unitE) / P_MASS);
+                    P_TRW = dev_T_win(kpc, jw, w0, w1, w2, ((P_BASE + (double)o1) * Bm.unitE) / P_MASS);
                 }
                 tautot += v / dmin;
                 // leave in a direction of the forward hemisphere (MCRadiation.f:919-925)
@@ -1464,7 +1473,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
             if (ia > NANGD) ia = NANGD;
             int il = (p.wl1 > 1.0 - p.wl1) ? p.l1 : p.l1 + 1;
             size_t bin = (size_t)ia * nlam + il;
-            unsigned long long q = (unsigned long long)rint(p.E * A.invE);
+            unsigned long long q = (unsigned long long)rint(p.E * Bm.invE);
             atomicAdd(&ty.specE[bin], q);
             atomicAdd(&ty.ispec[bin], 1u);
             if (p.scatt) {
@@ -1523,14 +1532,16 @@ __global__ void k_cell_opacity(int ncell, int nlam, int ng, const double *__rest
     KT[idx] = make_double2(ka, ks);
 }
 
-__global__ void k_cell_kp(int ncell, int ng, const double *__restrict__ Kp, const double *__restrict__ w,
-                          double *KpCell) {
+// computeE (Radiation.f:611-627) as a table per species-mix class (the cells
+// of a class have bitwise equal weights, so their rows are equal)
+__global__ void k_class_kp(int nclass, int ng, const double *__restrict__ Kp, const double *__restrict__ wcls,
+                           double *KpCls) {
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)ncell * (TMAXD + 1)) return;
-    int c = (int)(idx / (TMAXD + 1)), T = (int)(idx % (TMAXD + 1));
+    if (idx >= (size_t)nclass * (TMAXD + 1)) return;
+    int k = (int)(idx / (TMAXD + 1)), T = (int)(idx % (TMAXD + 1));
     double e = 0.0;
-    for (int s = 0; s < ng; s++) e = e + (Kp[(size_t)s * (TMAXD + 1) + T] * w[(size_t)c * ng + s]) * 1.0;
-    KpCell[idx] = e;
+    for (int s = 0; s < ng; s++) e = e + (Kp[(size_t)s * (TMAXD + 1) + T] * wcls[(size_t)k * ng + s]) * 1.0;
+    KpCls[idx] = e;
 }
 
 // the reemission cumulatives of each species-mix class: sum_s w_s CUM_s
@@ -1618,13 +1629,15 @@ __global__ void k_diffcoeff(int nclass, int nlam, int ng, int scattering, const 
 
 // the stage's snapshot: Eabs of the packets before it, and the temperature
 // it gives each cell (a start for the temperature searches of the stage)
-__global__ void k_stage_snap(int ncell, const unsigned long long *__restrict__ Eabs, const double *__restrict__ KpCell,
-                             const double *__restrict__ mass, double unitE, unsigned long long *Esnap, float *Tst) {
+__global__ void k_stage_snap(int ncell, const unsigned long long *__restrict__ Eabs, const double *__restrict__ KpCls,
+                             const int *__restrict__ cls, const double *__restrict__ mass, double unitE,
+                             unsigned long long *Esnap, float *Tst) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     const unsigned long long e = Eabs[c];
     Esnap[c] = e;
-    Tst[c] = e ? (float)dev_determineT(KpCell + (size_t)c * (TMAXD + 1), ((double)e * unitE) / mass[c], 1.0) : 0.f;
+    Tst[c] = e ? (float)dev_determineT(KpCls + (size_t)cls[c] * (TMAXD + 1), ((double)e * unitE) / mass[c], 1.0)
+               : 0.f;
 }
 
 __global__ void k_fill_u4(size_t n, uint4 *p) {
@@ -1640,8 +1653,8 @@ __global__ void k_clock(unsigned long long *p) { *p = wall_clock64(); }
 // after the pass, T(Eabs/m) (the serial loop's last increaseT)
 __global__ void k_finalize(int ncell, const unsigned long long *EJvq, const unsigned long long *EJv2q,
                            const unsigned long long *Eabsq, const double *invU1, const double *invU2, double unitE,
-                           const double *__restrict__ KpCell, const double *__restrict__ mass, double *EJv,
-                           double *EJv2, double *Eabs, double *Tbw) {
+                           const double *__restrict__ KpCls, const int *__restrict__ cls,
+                           const double *__restrict__ mass, double *EJv, double *EJv2, double *Eabs, double *Tbw) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     EJv[c] = (double)EJvq[c] / invU1[c];
@@ -1650,13 +1663,14 @@ __global__ void k_finalize(int ncell, const unsigned long long *EJvq, const unsi
     EJv2[c] = s / invU2[c];
     const unsigned long long e = Eabsq[c];
     Eabs[c] = (double)e * unitE;
-    Tbw[c] = e ? dev_determineT(KpCell + (size_t)c * (TMAXD + 1), ((double)e * unitE) / mass[c], 1.0) : 0.0;
+    Tbw[c] = e ? dev_determineT(KpCls + (size_t)cls[c] * (TMAXD + 1), ((double)e * unitE) / mass[c], 1.0) : 0.0;
 }
 
 // Main.f:498-549 per cell; void cell temperature Main.f:546-548.
 __global__ void k_temperature(int nr, int nt, int ng, const double *__restrict__ EJv, const double *__restrict__ EJv2,
                               const int *__restrict__ Ni, const double *__restrict__ Tbw, const double *__restrict__ V,
-                              const double *__restrict__ KpCell, const double *__restrict__ w,
+                              const double *__restrict__ KpCls, const int *__restrict__ cls,
+                              const double *__restrict__ w,
                               const double *__restrict__ Kp, double Tvoid, double *T, double *dT, double *dEJv) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nr * nt) return;
@@ -1670,7 +1684,7 @@ __global__ void k_temperature(int nr, int nt, int ng, const double *__restrict__
     double Vc = V[c];
     double E = EJv[c] / Vc;
     double E2 = EJv2[c] / (Vc * Vc);
-    double Tc = dev_determineT(KpCell + (size_t)c * (TMAXD + 1), E, Tbw[c]);
+    double Tc = dev_determineT(KpCls + (size_t)cls[c] * (TMAXD + 1), E, Tbw[c]);
     if (Tc < 1.0) Tc = 1.0;
     double dTc, dE;
     if (Ni[c] > 1) {
@@ -1749,6 +1763,7 @@ extern "C" void mcx_destroy(mcx_ctx *ctx) {
     free_list(ctx->run_alloc);
     free_list(ctx->mod_alloc);
     free_list(ctx->tab_alloc);
+    if (ctx->d_bms) (void)hipFree(ctx->d_bms);
     release_blas(ctx);
     release_structure(ctx);
     for (int k = 0; k < 4; k++)
@@ -1929,19 +1944,14 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
     ctx->d_R2 = (double *)md.R2;
     double2 *KT;
     if ((r = dalloc(ctx, ctx->mod_alloc, &KT, (size_t)ncell * nlam))) return r;
-    double *KpCell;
-    if ((r = dalloc(ctx, ctx->mod_alloc, &KpCell, (size_t)ncell * (TMAXD + 1)))) return r;
+
     {
         size_t n = (size_t)ncell * nlam;
         hipLaunchKernelGGL(k_cell_opacity, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, ncell, nlam, ng,
                            ctx->d_Kabs, ctx->d_Ksca, md.w, KT);
-        size_t m = (size_t)ncell * (TMAXD + 1);
-        hipLaunchKernelGGL(k_cell_kp, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream, ncell, ng,
-                           ctx->tb.Kp, md.w, KpCell);
         HIPCHK(ctx, hipGetLastError());
     }
     md.KT = KT;
-    md.KpCell = KpCell;
     // species-mix classes (cells with bitwise equal weights), radial edges;
     // the random walk's diffusion-coefficient table when Kext and g are known
     ctx->rw = DevRW{};
@@ -1971,6 +1981,15 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
         ctx->nclass = nclass;
         ctx->rw.cls = dcls;
         md.cls = dcls;
+        double *KpCls;
+        if ((r = dalloc(ctx, ctx->mod_alloc, &KpCls, (size_t)nclass * (TMAXD + 1)))) return r;
+        {
+            const size_t m = (size_t)nclass * (TMAXD + 1);
+            hipLaunchKernelGGL(k_class_kp, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream, nclass, ng,
+                               ctx->tb.Kp, ctx->d_wcls, KpCls);
+            HIPCHK(ctx, hipGetLastError());
+        }
+        md.KpCls = KpCls;
         // several species: one reemission cumulative per species-mix class,
         // so that an absorption searches one table instead of summing the
         // species' tables at every probe
@@ -2049,6 +2068,7 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_dT, ncell))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_dEJv, ncell))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_next, 1))) return r;
+
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->have_model = true;
     return MCX_OK;
@@ -2063,19 +2083,14 @@ static long long stage_next(long long P, long long N, long long stage0, int rati
     return (nx >= (double)N) ? N : (long long)nx;
 }
 
-extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
-    if (!ctx || !run) return MCX_EINVAL;
+// Per-model set-up of a pass: the thick flags, the packet energy, cleared
+// tallies, the fixed-point units; fills the model's launch descriptor.
+static int prep_model(mcx_ctx *ctx, const mcx_run *run, uint64_t seed, hipStream_t s, BatchModel &hb) {
     if (!ctx->have_model) return fail(ctx, MCX_ESTATE, "mcx_set_model first");
-    if (run->nphot <= 0 || run->nphot >= (1ll << 31)) return fail(ctx, MCX_EINVAL, "nphot must be in 1 .. 2^31-1");
-    if (run->precision != MCX_FP64 && run->precision != MCX_FP32) return fail(ctx, MCX_EINVAL, "unknown precision mode");
-    if (run->stage0 < 0 || run->stage_ratio < -1 || run->stage_ratio == 1)
-        return fail(ctx, MCX_EINVAL, "stage0 >= 0 and stage_ratio -1, 0 or >= 2 required");
-    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     const DevModel &md = ctx->md;
     DevTally &ty = ctx->ty;
     const int ncell = md.ncell, nlam = ctx->nlam;
-    hipStream_t s = ctx->stream;
-    const bool serial = (run->nlanes == 1);
     DevRW rw = ctx->rw;
     rw.on = 0;
     if (run->randomwalk) {
@@ -2095,49 +2110,9 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     const double E0 = ctx->Lstar / ((double)(float)run->nphot * ((1.0 - 0.0) - 0.0));
     ctx->E0 = E0;
     ctx->unitE = ldexp(E0, -EABS_BITS);
-    // the kernel for the model's species (one; several with class tables; several)
-    void (*kern)(TransportArgs);
-    if (run->precision == MCX_FP32)
-        kern = (ctx->ng == 1) ? k_transport<0, true> : md.CUMmix ? k_transport<1, true> : k_transport<2, true>;
-    else
-        kern = (ctx->ng == 1) ? k_transport<0, false> : md.CUMmix ? k_transport<1, false> : k_transport<2, false>;
-    // launch shape: a full device of resident wavefronts, or one lane
-    int threads = serial ? 64 : 256; // one wavefront: lane 0 traces, the others serve its lookups
-    size_t lds = sizeof(double) * ((size_t)(md.nr + 1) + 2 * (size_t)(md.nt + 1) + 2 * 181 + (size_t)(nlam - 1) +
-                                   NPL * (size_t)threads);
-    if (rw.on) lds += sizeof(double) * ((size_t)(md.nr + 1) + (size_t)(md.nt + 1));
-    // rings of Philox draws: 16 per thread when they fit beside the rest
-    // with two workgroups per CU (half the CU's 160 KiB of LDS), else fewer
-    int rb = 16;
-    const size_t lds_two = std::min(ctx->lds_max, (size_t)80 * 1024);
-    while (rb > 4 && lds + sizeof(double) * (size_t)rb * threads > (serial ? ctx->lds_max : lds_two)) rb >>= 1;
-    lds += sizeof(double) * (size_t)rb * threads;
-    if (lds > ctx->lds_max)
-        return fail(ctx, MCX_EINVAL, "grid too large for the kernel's LDS tables (" + std::to_string(lds) + " bytes)");
-    int max_blocks = 1;
-    if (!serial) {
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) != hipSuccess || per_cu < 1)
-            per_cu = 1;
-        max_blocks = per_cu * ctx->sm_count;
-        if (run->nlanes > 1) max_blocks = std::max(1, std::min(max_blocks, (run->nlanes + threads - 1) / threads));
-    }
-    // the lanes' visit tables (cleared per pass: tags are packet numbers)
-    const size_t lanes = (size_t)max_blocks * threads;
-    if (lanes > ctx->vtab_lanes) {
-        HIPCHK(ctx, hipStreamSynchronize(s));
-        free_list(ctx->run_alloc);
-        ty.vtab = nullptr;
-        ctx->vtab_lanes = 0;
-        int r = dalloc(ctx, ctx->run_alloc, &ty.vtab, lanes << VIS_LOG2);
-        if (r) return r;
-        ctx->vtab_lanes = lanes;
-    }
     ty.Ni = ctx->d_Ni;
     ty.Esnap = ctx->d_Esnap;
     ty.Tst = ctx->d_Tst;
-    HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
-    hipLaunchKernelGGL(k_fill_u4, dim3(1024), dim3(256), 0, s, lanes << VIS_LOG2, ty.vtab);
     HIPCHK(ctx, hipMemsetAsync(ty.EJv, 0, sizeof(unsigned long long) * ncell, s));
     HIPCHK(ctx, hipMemsetAsync(ty.EJv2, 0, sizeof(unsigned long long) * 3 * ncell, s));
     HIPCHK(ctx, hipMemsetAsync(ty.Eabs, 0, sizeof(unsigned long long) * ncell, s));
@@ -2150,25 +2125,135 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     hipLaunchKernelGGL(k_cell_units, dim3((ncell + 255) / 256), dim3(256), 0, s, md.nr, md.nt, nlam, md.R2, md.KT, E0,
                        (double *)ty.invU1, (double *)ty.invU2);
     HIPCHK(ctx, hipGetLastError());
+    hb.tb = ctx->tb;
+    hb.md = md;
+    hb.ty = ty;
+    hb.rw = rw;
+    hb.next_pid = ctx->d_next;
+    hb.invE = ldexp(1.0, EABS_BITS) / E0;
+    hb.unitE = ctx->unitE;
+    hb.seed = seed;
+    return MCX_OK;
+}
+
+// The code of a finished pass for one model (and its error text).
+static int pass_status(mcx_ctx *ctx) {
+    unsigned long long cnt[G_NSLOTS];
+    HIPCHK(ctx, hipMemcpy(cnt, ctx->ty.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (cnt[C_NTIMEOUT] > 0) {
+        double d[16];
+        HIPCHK(ctx, hipMemcpy(d, ctx->ty.torep, sizeof(d), hipMemcpyDeviceToHost));
+        char buf[512];
+        snprintf(buf, sizeof(buf),
+                 "; first abandoned: packet %.0f state %.0f after %.0f cell steps in cell (%.0f, %.0f) at (%.9g, %.9g, "
+                 "%.9g) AU, direction (%.9g, %.9g, %.9g), onEdge %.0f edge %.0f, tau %.6g of %.6g, %.0f crossings",
+                 d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11], d[12], d[14], d[13], d[15]);
+        return fail(ctx, MCX_ETIME,
+                    "time limit: " + std::to_string(cnt[C_NTIMEOUT]) + " packets unfinished" + std::string(buf));
+    }
+    if (cnt[C_NGEOM] > 0) return fail(ctx, MCX_EGEOM, std::to_string(cnt[C_NGEOM]) + " packets found nothing to hit");
+    return MCX_OK;
+}
+
+// One pass of nm models in common launches (one per stage), on the first
+// model's stream: bpm blocks per model (0: the device's resident blocks
+// shared out), the staged schedule or one lane.
+static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_req, int64_t seed_stride,
+                    int32_t *status) {
+    mcx_ctx *ctx = ctxs[0];
+    if (run->nphot <= 0 || run->nphot >= (1ll << 31)) return fail(ctx, MCX_EINVAL, "nphot must be in 1 .. 2^31-1");
+    if (run->precision != MCX_FP64 && run->precision != MCX_FP32) return fail(ctx, MCX_EINVAL, "unknown precision mode");
+    if (run->stage0 < 0 || run->stage_ratio < -1 || run->stage_ratio == 1)
+        return fail(ctx, MCX_EINVAL, "stage0 >= 0 and stage_ratio -1, 0 or >= 2 required");
+    const bool serial = (run->nlanes == 1);
+    if (serial && nm > 1) return fail(ctx, MCX_EINVAL, "a one-lane pass runs one model");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    // one kernel for all models: the same species case
+    const int mode = (ctx->ng == 1) ? 0 : ctx->md.CUMmix ? 1 : 2;
+    int nrmax = 0, ntmax = 0, nlmax = 0;
+    bool anyrw = false;
+    for (int m = 0; m < nm; m++) {
+        mcx_ctx *c = ctxs[m];
+        if (!c || !c->have_model) return fail(ctx, MCX_ESTATE, "model " + std::to_string(m) + ": mcx_set_model first");
+        if (c->device != ctx->device) return fail(ctx, MCX_EINVAL, "all models on one device");
+        const int mo = (c->ng == 1) ? 0 : c->md.CUMmix ? 1 : 2;
+        if (mo != mode) return fail(ctx, MCX_EINVAL, "models of one launch need the same species case");
+        nrmax = std::max(nrmax, c->md.nr);
+        ntmax = std::max(ntmax, c->md.nt);
+        nlmax = std::max(nlmax, c->nlam);
+    }
+    anyrw = run->randomwalk != 0;
+    void (*kern)(TransportArgs);
+    if (run->precision == MCX_FP32)
+        kern = (mode == 0) ? k_transport<0, true> : (mode == 1) ? k_transport<1, true> : k_transport<2, true>;
+    else
+        kern = (mode == 0) ? k_transport<0, false> : (mode == 1) ? k_transport<1, false> : k_transport<2, false>;
+    // launch shape: a full device of resident wavefronts, or one lane
+    int threads = serial ? 64 : 256; // one wavefront: lane 0 traces, the others serve its lookups
+    size_t lds = sizeof(double) * ((size_t)(nrmax + 1) + 2 * (size_t)(ntmax + 1) + 2 * 181 + (size_t)(nlmax - 1) +
+                                   NPL * (size_t)threads);
+    if (anyrw) lds += sizeof(double) * ((size_t)(nrmax + 1) + (size_t)(ntmax + 1));
+    // rings of Philox draws: 16 per thread when they fit beside the rest
+    // with two workgroups per CU (half the CU's 160 KiB of LDS), else fewer
+    int rb = 16;
+    const size_t lds_two = std::min(ctx->lds_max, (size_t)80 * 1024);
+    while (rb > 4 && lds + sizeof(double) * (size_t)rb * threads > (serial ? ctx->lds_max : lds_two)) rb >>= 1;
+    lds += sizeof(double) * (size_t)rb * threads;
+    if (lds > ctx->lds_max)
+        return fail(ctx, MCX_EINVAL, "grid too large for the kernel's LDS tables (" + std::to_string(lds) + " bytes)");
+    int resident = 1;
+    if (!serial) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        resident = per_cu * ctx->sm_count;
+        if (run->nlanes > 1) resident = std::max(1, std::min(resident, (run->nlanes + threads - 1) / threads));
+    }
+    const int bpm = serial ? 1 : (bpm_req > 0 ? bpm_req : std::max(1, resident / nm));
+    // the lanes' visit tables (cleared per pass: tags are packet numbers)
+    const size_t lanes = (size_t)bpm * nm * threads;
+    if (lanes > ctx->vtab_lanes) {
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        free_list(ctx->run_alloc);
+        ctx->ty.vtab = nullptr;
+        ctx->vtab_lanes = 0;
+        int r = dalloc(ctx, ctx->run_alloc, &ctx->ty.vtab, lanes << VIS_LOG2);
+        if (r) return r;
+        ctx->vtab_lanes = lanes;
+    }
+    // the launch descriptors
+    if ((size_t)nm > ctx->bm_cap) {
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        if (ctx->d_bms) (void)hipFree(ctx->d_bms);
+        ctx->d_bms = nullptr;
+        ctx->bm_cap = 0;
+        if (hipMalloc(&ctx->d_bms, sizeof(BatchModel) * (size_t)nm) != hipSuccess)
+            return fail(ctx, MCX_ENOMEM, "launch descriptors");
+        ctx->bm_cap = (size_t)nm;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
+    std::vector<BatchModel> hb(nm);
+    for (int m = 0; m < nm; m++) {
+        int r = prep_model(ctxs[m], run, run->seed + (uint64_t)(seed_stride * m), s, hb[m]);
+        if (r) return (m == 0) ? r : fail(ctx, r, "model " + std::to_string(m) + ": " + ctxs[m]->err);
+        hb[m].ty.vtab = ctx->ty.vtab;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_bms, hb.data(), sizeof(BatchModel) * (size_t)nm, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill_u4, dim3(1024), dim3(256), 0, s, lanes << VIS_LOG2, ctx->ty.vtab);
     TransportArgs A;
-    A.tb = ctx->tb;
-    A.md = md;
-    A.ty = ty;
-    A.rw = rw;
-    A.next_pid = ctx->d_next;
+    A.bm = (const BatchModel *)ctx->d_bms;
+    A.bpm = bpm;
     A.nphot = run->nphot;
-    A.seed = run->seed;
     A.maxinteract = run->maxinteract;
     A.deadline = 0ull;
     if (run->max_ms > 0.0 && ctx->wclk_khz > 0) {
         unsigned long long now = 0;
-        hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, ty.counters + G_CLOCK);
-        HIPCHK(ctx, hipMemcpyAsync(&now, ty.counters + G_CLOCK, sizeof(now), hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, ctx->ty.counters + G_CLOCK);
+        HIPCHK(ctx, hipMemcpyAsync(&now, ctx->ty.counters + G_CLOCK, sizeof(now), hipMemcpyDeviceToHost, s));
         HIPCHK(ctx, hipStreamSynchronize(s));
         A.deadline = now + (unsigned long long)(run->max_ms * (double)ctx->wclk_khz);
     }
-    A.invE = ldexp(1.0, EABS_BITS) / E0;
-    A.unitE = ctx->unitE;
     A.serial = serial ? 1 : 0;
     A.coop = (run->coop_lanes < 0) ? COOP_LANES : run->coop_lanes;
     A.rb_mask = rb - 1;
@@ -2182,19 +2267,26 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     static const bool stage_log = getenv("MCX_STAGE_LOG") != nullptr;
     long long Pa = 0, Pb = serial ? N : std::min(N, s0);
     while (Pa < N) {
-        if (!serial) {
-            hipLaunchKernelGGL(k_stage_snap, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell, ty.Eabs, md.KpCell,
-                               md.mass, ctx->unitE, ctx->d_Esnap, ctx->d_Tst);
-            HIPCHK(ctx, hipGetLastError());
+        for (int m = 0; m < nm; m++) {
+            mcx_ctx *c = ctxs[m];
+            const DevModel &md = c->md;
+            if (!serial) {
+                hipLaunchKernelGGL(k_stage_snap, dim3((md.ncell + 255) / 256), dim3(256), 0, s, md.ncell, c->ty.Eabs,
+                                   md.KpCls, md.cls, md.mass, c->unitE, c->d_Esnap, c->d_Tst);
+                HIPCHK(ctx, hipGetLastError());
+            }
+            hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, c->d_next, (unsigned long long)Pa);
+            HIPCHK(ctx, hipMemsetAsync(c->ty.counters + G_DONE, 0, sizeof(unsigned long long), s));
+            HIPCHK(ctx, hipMemsetAsync(c->ty.counters + G_TLINE, 0, sizeof(unsigned long long) * 6, s));
         }
         A.Pk = serial ? 0 : Pa;
         A.pid_base = Pa;
         A.pid_end = Pb;
-        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, ctx->d_next, (unsigned long long)Pa);
-        HIPCHK(ctx, hipMemsetAsync(ty.counters + G_DONE, 0, sizeof(unsigned long long), s));
-        HIPCHK(ctx, hipMemsetAsync(ty.counters + G_TLINE, 0, sizeof(unsigned long long) * 6, s));
-        const int blocks =
-            serial ? 1 : (int)std::max(1ll, std::min((long long)max_blocks, (Pb - Pa + threads - 1) / threads));
+        // a lone model gets no more blocks than its stage has packets for
+        const int blocks = serial ? 1
+                           : (nm == 1) ? (int)std::max(1ll, std::min((long long)bpm, (Pb - Pa + threads - 1) / threads))
+                                       : bpm * nm;
+        if (nm == 1) A.bpm = blocks;
         HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, s, A);
         HIPCHK(ctx, hipGetLastError());
@@ -2203,41 +2295,54 @@ extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
         float ms = 0.f;
         HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
         kms_total += ms;
-        if (stage_log) fprintf(stderr, "[mcx] stage [%lld, %lld): %.1f ms\n", Pa, Pb, ms);
+        if (stage_log) fprintf(stderr, "[mcx] stage [%lld, %lld) of %d model(s): %.1f ms\n", Pa, Pb, nm, ms);
         if (A.deadline) {
-            unsigned long long nto = 0;
-            HIPCHK(ctx, hipMemcpy(&nto, ty.counters + C_NTIMEOUT, sizeof(nto), hipMemcpyDeviceToHost));
+            unsigned long long nto = 0, t;
+            for (int m = 0; m < nm; m++) {
+                HIPCHK(ctx, hipMemcpy(&t, ctxs[m]->ty.counters + C_NTIMEOUT, sizeof(t), hipMemcpyDeviceToHost));
+                nto += t;
+            }
             if (nto) break;
         }
         Pa = Pb;
         Pb = stage_next(Pb, N, s0, ratio);
     }
-    hipLaunchKernelGGL(k_finalize, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell, ty.EJv, ty.EJv2, ty.Eabs,
-                       ty.invU1, ty.invU2, ctx->unitE, md.KpCell, md.mass, ctx->d_EJv, ctx->d_EJv2, ctx->d_Eabs,
-                       ctx->d_Tbw);
-    HIPCHK(ctx, hipGetLastError());
+    for (int m = 0; m < nm; m++) {
+        mcx_ctx *c = ctxs[m];
+        const DevModel &md = c->md;
+        const int ncell = md.ncell;
+        hipLaunchKernelGGL(k_finalize, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell, c->ty.EJv, c->ty.EJv2,
+                           c->ty.Eabs, c->ty.invU1, c->ty.invU2, c->unitE, md.KpCls, md.cls, md.mass, c->d_EJv,
+                           c->d_EJv2, c->d_Eabs, c->d_Tbw);
+        HIPCHK(ctx, hipGetLastError());
+    }
     HIPCHK(ctx, hipEventRecord(ctx->ev[3], s));
     HIPCHK(ctx, hipEventSynchronize(ctx->ev[3]));
     float pms = 0.f;
     HIPCHK(ctx, hipEventElapsedTime(&pms, ctx->ev[0], ctx->ev[3]));
-    ctx->kernel_ms = kms_total;
-    ctx->pass_ms = pms;
-    ctx->have_pass = true;
-    unsigned long long cnt[G_NSLOTS];
-    HIPCHK(ctx, hipMemcpy(cnt, ty.counters, sizeof(cnt), hipMemcpyDeviceToHost));
-    if (cnt[C_NTIMEOUT] > 0) {
-        double d[16];
-        HIPCHK(ctx, hipMemcpy(d, ty.torep, sizeof(d), hipMemcpyDeviceToHost));
-        char buf[512];
-        snprintf(buf, sizeof(buf),
-                 "; first abandoned: packet %.0f state %.0f after %.0f cell steps in cell (%.0f, %.0f) at (%.9g, %.9g, "
-                 "%.9g) AU, direction (%.9g, %.9g, %.9g), onEdge %.0f edge %.0f, tau %.6g of %.6g, %.0f crossings",
-                 d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11], d[12], d[14], d[13], d[15]);
-        return fail(ctx, MCX_ETIME,
-                    "time limit: " + std::to_string(cnt[C_NTIMEOUT]) + " packets unfinished" + std::string(buf));
+    int rc = MCX_OK;
+    for (int m = 0; m < nm; m++) {
+        mcx_ctx *c = ctxs[m];
+        c->kernel_ms = kms_total;
+        c->pass_ms = pms;
+        c->have_pass = true;
+        const int r = pass_status(c);
+        if (status) status[m] = r;
+        if (r && !rc) rc = r;
     }
-    if (cnt[C_NGEOM] > 0) return fail(ctx, MCX_EGEOM, std::to_string(cnt[C_NGEOM]) + " packets found nothing to hit");
-    return MCX_OK;
+    if (rc && nm > 1) ctx->err = "a model's pass failed (see its status word and its context's error)";
+    return rc;
+}
+
+extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
+    if (!ctx || !run) return MCX_EINVAL;
+    return run_pass(&ctx, 1, run, 0, 0, nullptr);
+}
+
+extern "C" int mcx_batch_transport(mcx_ctx *const *ctxs, int32_t nmodels, const mcx_run *run,
+                                   int32_t blocks_per_model, int64_t seed_stride, int32_t *status) {
+    if (!ctxs || !run || nmodels < 1 || !ctxs[0]) return MCX_EINVAL;
+    return run_pass(ctxs, nmodels, run, blocks_per_model, seed_stride, status);
 }
 
 extern "C" int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *o) {
@@ -2316,7 +2421,7 @@ extern "C" int mcx_temperature(mcx_ctx *ctx, mcx_temps *o) {
     const DevModel &md = ctx->md;
     const int ncell = md.ncell;
     hipLaunchKernelGGL(k_temperature, dim3((ncell + 255) / 256), dim3(256), 0, ctx->stream, md.nr, md.nt, ctx->ng,
-                       ctx->d_EJv, ctx->d_EJv2, ctx->d_Ni, ctx->d_Tbw, ctx->d_V, md.KpCell, md.w, ctx->tb.Kp,
+                       ctx->d_EJv, ctx->d_EJv2, ctx->d_Ni, ctx->d_Tbw, ctx->d_V, md.KpCls, md.cls, md.w, ctx->tb.Kp,
                        ctx->Tvoid, ctx->d_T, ctx->d_dT, ctx->d_dEJv);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

@@ -71,7 +71,8 @@ struct DevModel {
     const double *dens, *mass;     // [ncell]
     const double *w;               // [ncell*ng]
     const double2 *KT;             // [ncell*nlam] {KabsTot, KscaTot}
-    const double *KpCell;          // [ncell*(TMAX+1)]
+    const double *KpCls;           // [nclass*(TMAX+1)] sum_s w_s Kp_s(T) of each species-mix
+                                   // class (computeE as a table; cell c: row cls[c])
     const int *cls;                // [ncell] species-mix class of the cell
     const double *CUMmix;          // [nclass][TMAX+1][nlam-1] sum_s w_s CUM_s of each
                                    // class (several species; null: sum per event)
@@ -146,6 +147,8 @@ struct mcx_ctx {
     double *d_EJv = nullptr, *d_EJv2 = nullptr, *d_Eabs = nullptr, *d_Tbw = nullptr;
     double *d_T = nullptr, *d_dT = nullptr, *d_dEJv = nullptr;
     unsigned long long *d_next = nullptr;
+    void *d_bms = nullptr;      // launch descriptors of the models of a pass run from this context
+    size_t bm_cap = 0;
     double unitE = 0.0, E0 = 0.0;
     double kernel_ms = 0.0, pass_ms = 0.0;
     double image_rays_ms = 0.0, image_spread_ms = 0.0;  // last mcx_image: ray part (host clock), k_image (events)

@@ -282,6 +282,28 @@ class Transport:
         return k.value, p.value
 
 
+def batch_transport(trs, nphot, seed=1, seed_stride=1, blocks_per_model=0, randomwalk=None, factRW=None,
+                    maxinteract=0, precision=0, max_ms=0.0, stage0=0, stage_ratio=0):
+    """One pass of nphot packets for every model of the Transport objects
+    trs (one model each) in common launches (mcx_batch_transport); model m
+    uses the key seed + m seed_stride.  Returns the per-model status codes;
+    the tallies stay on each model's context."""
+    lib = trs[0].lib
+    m0 = trs[0].model
+    rw = bool(getattr(m0, "randomwalk", False)) if randomwalk is None else bool(randomwalk)
+    f = float(getattr(m0, "factRW", 10.0)) if factRW is None else float(factRW)
+    r = _lib.Run(nphot=int(nphot), seed=int(seed), maxinteract=int(maxinteract), nlanes=0, stage0=int(stage0),
+                 randomwalk=int(rw), factRW=f, coop_lanes=-1, stage_ratio=int(stage_ratio), precision=int(precision),
+                 max_ms=float(max_ms))
+    arr = (C.c_void_p * len(trs))(*[t.ctx for t in trs])
+    st = np.zeros(len(trs), dtype=np.int32)
+    rc = lib.mcx_batch_transport(arr, len(trs), C.byref(r), int(blocks_per_model), int(seed_stride),
+                                 st.ctypes.data_as(C.POINTER(C.c_int32)))
+    if rc not in (0, _lib.MCX_EGEOM):
+        raise MCXError(rc, lib.mcx_last_error(trs[0].ctx).decode())
+    return st
+
+
 def make_obs(angle_deg=45.0, nphi=45, nj0=2, nphot=1000, nphotstar=1000, nexits=5, usepol=True, tracestar=True,
              traceemis=True, tracescat=True, pfstop=0.25, seed=1):
     """An observation with the reference's defaults (Telescope.f:590-657,

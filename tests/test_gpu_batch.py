@@ -1,0 +1,53 @@
+"""Many models in common launches (mcx_batch_transport): each model's tallies
+are bit-identical to its own single-model pass with the same Philox key,
+whatever the other models of the launch (their grids differ in size, dust
+mass and star)."""
+import numpy as np
+import pytest
+
+from mcmax_amd import transport
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("EJv", "EJv2", "Eabs", "Tbw", "Ni", "spec", "ispec", "scatspec")
+COUNTS = ("nemit", "ninteract", "nabs", "nscat", "nescape", "nstarhit", "nsteps", "nrw", "maxsteps")
+
+
+@pytest.fixture(scope="module")
+def models(workdir):
+    from mcmax_amd import configs
+    specs = [dict(mdust=1e-4), dict(nrad=30, ntheta=14, mdust=1e-3, tstar=6000.0, rstar=1.5),
+             dict(nrad=24, ntheta=8, mdust=3e-4, rin=0.3, randomwalk=True, factrw=2.0),
+             dict(nrad=16, ntheta=12, mdust=1e-5, tstar=9000.0)]
+    return [configs.load_config("tiny", f"{workdir}/batch{k}", **sp) for k, sp in enumerate(specs)]
+
+
+@pytest.mark.parametrize("bpm", [1, 3])
+def test_batch_equals_single(models, bpm):
+    trs = []
+    try:
+        for m in models:
+            t = transport.Transport(0)
+            t.set_model(m)
+            trs.append(t)
+        n, seed, stride = 20000, 77, 11
+        st = transport.batch_transport(trs, n, seed=seed, seed_stride=stride, blocks_per_model=bpm,
+                                       randomwalk=True, factRW=2.0, max_ms=120000.0)
+        assert np.all(st == 0), st
+        batch = [t.tallies() for t in trs]
+        Tb = [t.temperature()[0] for t in trs]
+        for k, (t, m) in enumerate(zip(trs, models)):
+            t.transport(n, seed=seed + stride * k, randomwalk=True, factRW=2.0, max_ms=120000.0)
+            g = t.tallies()
+            g["T"] = t.temperature()[0]
+            assert g["nescape"] == n and abs(g["Eescape"] / m.Lstar - 1) < 1e-9
+            for key in KEYS:
+                assert np.array_equal(batch[k][key], g[key]), (k, key)
+            for key in COUNTS:
+                assert batch[k][key] == g[key], (k, key)
+            assert np.array_equal(Tb[k], g["T"]), k
+        # the models differ: different energies and grids
+        assert len({b["EJv"].shape for b in batch}) > 1
+    finally:
+        for t in trs:
+            t.close()
