@@ -207,9 +207,15 @@ int mcx_transport(mcx_ctx *ctx, const mcx_run *run);
  * its own context with that key.  Read them with mcx_get_tallies /
  * mcx_temperature on the model's context.  status[m] (may be NULL)
  * receives model m's code (MCX_OK, MCX_EGEOM, MCX_ETIME); the call returns
- * the first failure.  One-lane runs (nlanes = 1) take one model. */
+ * the first failure.  One-lane runs (nlanes = 1) take one model.
+ * use_graph != 0: the pass's launches are captured in a HIP graph on the
+ * first call (for these contexts and this run configuration) and replayed
+ * on later calls with the new keys, without host synchronisation between
+ * stages (no time limit in this mode; mcx_last_timing reports the graph's
+ * duration).  The structure of the models may change between calls through
+ * mcx_set_model only if the grid sizes stay the same. */
 int mcx_batch_transport(mcx_ctx *const *ctxs, int32_t nmodels, const mcx_run *run, int32_t blocks_per_model,
-                        int64_t seed_stride, int32_t *status);
+                        int64_t seed_stride, int32_t use_graph, int32_t *status);
 
 /* Copy the last pass's tallies to the host (any pointer may be NULL). */
 int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *out);

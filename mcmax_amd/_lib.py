@@ -140,7 +140,7 @@ def load():
     lib.mcx_set_tables.argtypes = [vp, C.POINTER(Tables)]
     lib.mcx_set_model.argtypes = [vp, C.POINTER(Grid)]
     lib.mcx_transport.argtypes = [vp, C.POINTER(Run)]
-    lib.mcx_batch_transport.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Run), C.c_int32, C.c_int64,
+    lib.mcx_batch_transport.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Run), C.c_int32, C.c_int64, C.c_int32,
                                         C.POINTER(C.c_int32)]
     lib.mcx_get_tallies.argtypes = [vp, C.POINTER(Tallies)]
     lib.mcx_temperature.argtypes = [vp, C.POINTER(Temps)]

@@ -1647,6 +1647,14 @@ __global__ void k_fill_u4(size_t n, uint4 *p) {
 
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
+__global__ void k_fill_u64(size_t n, unsigned long long *p, unsigned long long v) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = v;
+}
+
+__global__ void k_fill_u32(size_t n, unsigned *p, unsigned v) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = v;
+}
+
 __global__ void k_clock(unsigned long long *p) { *p = wall_clock64(); }
 
 // raw tallies from the fixed-point sums; Tbw = the cell's Bjorkman-Wood T
@@ -1764,6 +1772,8 @@ extern "C" void mcx_destroy(mcx_ctx *ctx) {
     free_list(ctx->mod_alloc);
     free_list(ctx->tab_alloc);
     if (ctx->d_bms) (void)hipFree(ctx->d_bms);
+    if (ctx->gexec) (void)hipGraphExecDestroy((hipGraphExec_t)ctx->gexec);
+    if (ctx->graph) (void)hipGraphDestroy((hipGraph_t)ctx->graph);
     release_blas(ctx);
     release_structure(ctx);
     for (int k = 0; k < 4; k++)
@@ -2083,14 +2093,88 @@ static long long stage_next(long long P, long long N, long long stage0, int rati
     return (nx >= (double)N) ? N : (long long)nx;
 }
 
-// Per-model set-up of a pass: the thick flags, the packet energy, cleared
-// tallies, the fixed-point units; fills the model's launch descriptor.
-static int prep_model(mcx_ctx *ctx, const mcx_run *run, uint64_t seed, hipStream_t s, BatchModel &hb) {
+// ---- the pass's per-model kernels, reading model m's buffers from its
+// launch descriptor (so that a captured pass stays valid when a model's
+// buffers are reallocated between calls) ----------------------------------
+__global__ void k_bm_prep(const BatchModel *__restrict__ bm, int m, double factRW) {
+    const BatchModel &B = bm[m];
+    const DevModel &md = B.md;
+    const DevTally &ty = B.ty;
+    const int ncell = md.ncell, nlam = B.tb.nlam;
+    const size_t nspec = (size_t)2 * (NANGD + 1) * nlam;
+    const size_t n = std::max((size_t)3 * ncell, nspec);
+    const double E0 = B.unitE * ldexp(1.0, EABS_BITS);
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        if (k < (size_t)ncell) {
+            const int c = (int)k, i = c / md.nt;
+            ty.EJv[c] = 0ull;
+            ty.Eabs[c] = 0ull;
+            ty.Tcell[c] = 0ull;
+            ty.Ni[c] = 0;
+            if (ty.thick) {
+                int th = 0;
+                if (B.rw.on && i > 0)
+                    for (int iT = 100; iT <= 1600; iT += 500) {
+                        const double lr = 1.0 / ((MCX_AU * B.rw.KD[(size_t)md.cls[c] * (TMAXD + 1) + iT].x) * md.dens[c]);
+                        if ((B.rw.R[i + 1] - B.rw.R[i]) > factRW * lr) th = 1;
+                    }
+                ty.thick[c] = th;
+            }
+            // fixed-point units (see k_cell_units)
+            double kmax = 0.0;
+            for (int l = 0; l < nlam; l++) kmax = fmax(kmax, md.KT[(size_t)c * nlam + l].x);
+            const double chord = 2.0 * sqrt(fmax(md.R2[i + 1] - md.R2[i], 0.0));
+            double Bc = E0 * MCX_AU * chord * kmax * (1.0 + 1.0 / 64.0);
+            if (!(Bc > 0.0)) Bc = 1e-300;
+            ((double *)ty.invU1)[c] = ldexp(1.0, EJV_BITS) / Bc;
+            ((double *)ty.invU2)[c] = ldexp(1.0, EJV2_BITS) / (Bc * Bc);
+        }
+        if (k < (size_t)3 * ncell) ty.EJv2[k] = 0ull;
+        if (k < nspec) {
+            ty.specE[k] = 0ull;
+            ty.ispec[k] = 0u;
+        }
+        if (k < G_NSLOTS) ty.counters[k] = 0ull;
+        if (k == 0) *ty.tautot = 0.0;
+    }
+}
+
+__global__ void k_bm_stage(const BatchModel *__restrict__ bm, int m, unsigned long long Pa) {
+    const BatchModel &B = bm[m];
+    const DevModel &md = B.md;
+    const DevTally &ty = B.ty;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < md.ncell) {
+        const unsigned long long e = ty.Eabs[c];
+        ((unsigned long long *)ty.Esnap)[c] = e;
+        ((float *)ty.Tst)[c] =
+            e ? (float)dev_determineT(md.KpCls + (size_t)md.cls[c] * (TMAXD + 1), ((double)e * B.unitE) / md.mass[c], 1.0)
+              : 0.f;
+    }
+    if (c == 0) *B.next_pid = Pa;
+    if (c < 7) ty.counters[G_DONE + c] = 0ull;
+}
+
+__global__ void k_bm_finalize(const BatchModel *__restrict__ bm, int m) {
+    const BatchModel &B = bm[m];
+    const DevModel &md = B.md;
+    const DevTally &ty = B.ty;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= md.ncell) return;
+    ty.oEJv[c] = (double)ty.EJv[c] / ty.invU1[c];
+    const unsigned long long *L = ty.EJv2 + 3 * (size_t)c;
+    const double s = ((double)L[2] * 0x1p64 + (double)L[1] * 0x1p32) + (double)L[0];
+    ty.oEJv2[c] = s / ty.invU2[c];
+    const unsigned long long e = ty.Eabs[c];
+    ty.oEabs[c] = (double)e * B.unitE;
+    ty.oTbw[c] = e ? dev_determineT(md.KpCls + (size_t)md.cls[c] * (TMAXD + 1), ((double)e * B.unitE) / md.mass[c], 1.0)
+                   : 0.0;
+}
+
+// Per-model set-up of a pass, host part: checks, the packet energy and the
+// model's launch descriptor.
+static int host_prep(mcx_ctx *ctx, const mcx_run *run, uint64_t seed, BatchModel &hb) {
     if (!ctx->have_model) return fail(ctx, MCX_ESTATE, "mcx_set_model first");
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    const DevModel &md = ctx->md;
-    DevTally &ty = ctx->ty;
-    const int ncell = md.ncell, nlam = ctx->nlam;
     DevRW rw = ctx->rw;
     rw.on = 0;
     if (run->randomwalk) {
@@ -2101,38 +2185,55 @@ static int prep_model(mcx_ctx *ctx, const mcx_run *run, uint64_t seed, hipStream
         if (!(run->factRW > 0.0)) return fail(ctx, MCX_EINVAL, "factRW must be positive");
         rw.on = 1;
         rw.factRW = run->factRW;
-        hipLaunchKernelGGL(k_thick_cells, dim3((ncell + 255) / 256), dim3(256), 0, s, md.nr, md.nt, rw.R, md.dens, rw.cls,
-                           rw.KD, run->factRW, ctx->d_thick);
-        HIPCHK(ctx, hipGetLastError());
-    } else if (ctx->d_thick) {
-        HIPCHK(ctx, hipMemsetAsync(ctx->d_thick, 0, sizeof(int) * ncell, s));
     }
     const double E0 = ctx->Lstar / ((double)(float)run->nphot * ((1.0 - 0.0) - 0.0));
     ctx->E0 = E0;
     ctx->unitE = ldexp(E0, -EABS_BITS);
+    DevTally &ty = ctx->ty;
     ty.Ni = ctx->d_Ni;
     ty.Esnap = ctx->d_Esnap;
     ty.Tst = ctx->d_Tst;
-    HIPCHK(ctx, hipMemsetAsync(ty.EJv, 0, sizeof(unsigned long long) * ncell, s));
-    HIPCHK(ctx, hipMemsetAsync(ty.EJv2, 0, sizeof(unsigned long long) * 3 * ncell, s));
-    HIPCHK(ctx, hipMemsetAsync(ty.Eabs, 0, sizeof(unsigned long long) * ncell, s));
-    HIPCHK(ctx, hipMemsetAsync(ty.Tcell, 0, sizeof(unsigned long long) * ncell, s));
-    HIPCHK(ctx, hipMemsetAsync(ty.specE, 0, sizeof(unsigned long long) * 2 * (NANGD + 1) * nlam, s));
-    HIPCHK(ctx, hipMemsetAsync(ty.ispec, 0, sizeof(unsigned) * 2 * (NANGD + 1) * nlam, s));
-    HIPCHK(ctx, hipMemsetAsync(ty.counters, 0, sizeof(unsigned long long) * G_NSLOTS, s));
-    HIPCHK(ctx, hipMemsetAsync(ty.tautot, 0, sizeof(double), s));
-    HIPCHK(ctx, hipMemsetAsync(ctx->d_Ni, 0, sizeof(int) * ncell, s));
-    hipLaunchKernelGGL(k_cell_units, dim3((ncell + 255) / 256), dim3(256), 0, s, md.nr, md.nt, nlam, md.R2, md.KT, E0,
-                       (double *)ty.invU1, (double *)ty.invU2);
-    HIPCHK(ctx, hipGetLastError());
+    ty.oEJv = ctx->d_EJv;
+    ty.oEJv2 = ctx->d_EJv2;
+    ty.oEabs = ctx->d_Eabs;
+    ty.oTbw = ctx->d_Tbw;
+    ty.thick = ctx->d_thick;
     hb.tb = ctx->tb;
-    hb.md = md;
+    hb.md = ctx->md;
     hb.ty = ty;
     hb.rw = rw;
     hb.next_pid = ctx->d_next;
     hb.invE = ldexp(1.0, EABS_BITS) / E0;
     hb.unitE = ctx->unitE;
     hb.seed = seed;
+    return MCX_OK;
+}
+
+// Per-model set-up, device part (kernels only, so that it can be captured in
+// a graph): the thick flags, cleared tallies, the fixed-point units.
+static int enqueue_prep(mcx_ctx *ctx, const BatchModel &hb, double factRW, hipStream_t s) {
+    const DevModel &md = ctx->md;
+    const DevTally &ty = ctx->ty;
+    const int ncell = md.ncell, nlam = ctx->nlam;
+    const dim3 g1((ncell + 255) / 256), b1(256);
+    if (hb.rw.on) {
+        hipLaunchKernelGGL(k_thick_cells, g1, b1, 0, s, md.nr, md.nt, hb.rw.R, md.dens, hb.rw.cls, hb.rw.KD, factRW,
+                           ctx->d_thick);
+    } else if (ctx->d_thick) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(64), b1, 0, s, (size_t)ncell, (unsigned *)ctx->d_thick, 0u);
+    }
+    hipLaunchKernelGGL(k_fill_u64, dim3(64), b1, 0, s, (size_t)ncell, ty.EJv, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(64), b1, 0, s, (size_t)3 * ncell, ty.EJv2, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(64), b1, 0, s, (size_t)ncell, ty.Eabs, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(64), b1, 0, s, (size_t)ncell, ty.Tcell, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(64), b1, 0, s, (size_t)2 * (NANGD + 1) * nlam, ty.specE, 0ull);
+    hipLaunchKernelGGL(k_fill_u32, dim3(64), b1, 0, s, (size_t)2 * (NANGD + 1) * nlam, ty.ispec, 0u);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), b1, 0, s, (size_t)G_NSLOTS, ty.counters, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), b1, 0, s, (size_t)1, (unsigned long long *)ty.tautot, 0ull);
+    hipLaunchKernelGGL(k_fill_u32, dim3(64), b1, 0, s, (size_t)ncell, (unsigned *)ctx->d_Ni, 0u);
+    hipLaunchKernelGGL(k_cell_units, g1, b1, 0, s, md.nr, md.nt, nlam, md.R2, md.KT, ctx->E0, (double *)ty.invU1,
+                       (double *)ty.invU2);
+    HIPCHK(ctx, hipGetLastError());
     return MCX_OK;
 }
 
@@ -2155,11 +2256,123 @@ static int pass_status(mcx_ctx *ctx) {
     return MCX_OK;
 }
 
+// Launch shape of a pass.
+struct LaunchCfg {
+    void (*kern)(TransportArgs);
+    int threads, rb, bpm;
+    size_t lds, lanes;
+};
+
+// The pass's device work after the descriptors are in place: set-up, the
+// stages (snapshot, counters, transport launch) and the tallies' finish, all
+// enqueued on s without host synchronisation (a graph captures it whole).
+static void enqueue_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, const LaunchCfg &L,
+                         const std::vector<BatchModel> &hb, hipStream_t s) {
+    mcx_ctx *ctx = ctxs[0];
+    const BatchModel *dbm = (const BatchModel *)ctx->d_bms;
+    for (int m = 0; m < nm; m++) {
+        const int nc = ctxs[m]->md.ncell;
+        hipLaunchKernelGGL(k_bm_prep, dim3((unsigned)std::min(1024, (3 * nc + 255) / 256 + 1)), dim3(256), 0, s, dbm, m,
+                           run->factRW);
+    }
+    hipLaunchKernelGGL(k_fill_u4, dim3(1024), dim3(256), 0, s, L.lanes << VIS_LOG2, ctx->ty.vtab);
+    TransportArgs A;
+    A.bm = dbm;
+    A.bpm = L.bpm;
+    A.nphot = run->nphot;
+    A.maxinteract = run->maxinteract;
+    A.deadline = 0ull;
+    A.serial = 0;
+    A.coop = (run->coop_lanes < 0) ? COOP_LANES : run->coop_lanes;
+    A.rb_mask = L.rb - 1;
+    const long long N = run->nphot;
+    const long long s0 = run->stage0 > 0 ? run->stage0 : 64;
+    const int ratio = run->stage_ratio == 0 ? 2 : run->stage_ratio;
+    long long Pa = 0, Pb = std::min(N, s0);
+    while (Pa < N) {
+        for (int m = 0; m < nm; m++)
+            hipLaunchKernelGGL(k_bm_stage, dim3((ctxs[m]->md.ncell + 255) / 256), dim3(256), 0, s, dbm, m,
+                               (unsigned long long)Pa);
+        A.Pk = Pa;
+        A.pid_base = Pa;
+        A.pid_end = Pb;
+        const int blocks = (nm == 1) ? (int)std::max(1ll, std::min((long long)L.bpm, (Pb - Pa + L.threads - 1) / L.threads))
+                                     : L.bpm * nm;
+        if (nm == 1) A.bpm = blocks;
+        hipLaunchKernelGGL(L.kern, dim3(blocks), dim3(L.threads), L.lds, s, A);
+        Pa = Pb;
+        Pb = stage_next(Pb, N, s0, ratio);
+    }
+    for (int m = 0; m < nm; m++)
+        hipLaunchKernelGGL(k_bm_finalize, dim3((ctxs[m]->md.ncell + 255) / 256), dim3(256), 0, s, dbm, m);
+    (void)hb;
+}
+
+// A pass replayed from a graph: the whole sequence is captured once for a
+// set of models and a run configuration and replayed for later calls (the
+// descriptors, which hold the keys and energies, are rewritten before each
+// replay).  No time limit and no per-stage timing in this mode.
+static int graph_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, const LaunchCfg &L,
+                      const std::vector<BatchModel> &hb, int32_t *status) {
+    mcx_ctx *ctx = ctxs[0];
+    hipStream_t s = ctx->stream;
+    char kb[256];
+    snprintf(kb, sizeof(kb), "%d %lld %d %d %.17g %d %d %lld %d %d %d %d %zu %zu", nm, (long long)run->nphot, L.bpm,
+             run->randomwalk, run->factRW, run->precision, run->maxinteract, (long long)run->stage0, run->stage_ratio,
+             run->coop_lanes, L.threads, L.rb, L.lds, L.lanes);
+    std::string key(kb);
+    // the descriptors carry every model buffer's address: the graph depends
+    // on the models' sizes, the descriptors' and the visit tables' places
+    key += " " + std::to_string((unsigned long long)(uintptr_t)ctx->d_bms) + " " +
+           std::to_string((unsigned long long)(uintptr_t)ctx->ty.vtab);
+    for (int m = 0; m < nm; m++)
+        key += " " + std::to_string(ctxs[m]->md.nr) + "x" + std::to_string(ctxs[m]->md.nt) + "x" +
+               std::to_string(ctxs[m]->nlam) + (ctxs[m]->d_thick ? "t" : "");
+    if (!ctx->gexec || ctx->graph_key != key) {
+        if (ctx->gexec) (void)hipGraphExecDestroy((hipGraphExec_t)ctx->gexec);
+        if (ctx->graph) (void)hipGraphDestroy((hipGraph_t)ctx->graph);
+        ctx->gexec = ctx->graph = nullptr;
+        ctx->graph_key.clear();
+        HIPCHK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        enqueue_pass(ctxs, nm, run, L, hb, s);
+        hipGraph_t g = nullptr;
+        const hipError_t e1 = hipStreamEndCapture(s, &g);
+        if (e1 != hipSuccess || !g) return fail(ctx, MCX_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e1));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e2 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        if (e2 != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return fail(ctx, MCX_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e2));
+        }
+        ctx->graph = g;
+        ctx->gexec = ex;
+        ctx->graph_key = key;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
+    HIPCHK(ctx, hipGraphLaunch((hipGraphExec_t)ctx->gexec, s));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[3], s));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[3]));
+    float pms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&pms, ctx->ev[0], ctx->ev[3]));
+    int rc = MCX_OK;
+    for (int m = 0; m < nm; m++) {
+        mcx_ctx *c = ctxs[m];
+        c->kernel_ms = pms;
+        c->pass_ms = pms;
+        c->have_pass = true;
+        const int r = pass_status(c);
+        if (status) status[m] = r;
+        if (r && !rc) rc = r;
+    }
+    if (rc && nm > 1) ctx->err = "a model's pass failed (see its status word and its context's error)";
+    return rc;
+}
+
 // One pass of nm models in common launches (one per stage), on the first
 // model's stream: bpm blocks per model (0: the device's resident blocks
 // shared out), the staged schedule or one lane.
 static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_req, int64_t seed_stride,
-                    int32_t *status) {
+                    int use_graph, int32_t *status) {
     mcx_ctx *ctx = ctxs[0];
     if (run->nphot <= 0 || run->nphot >= (1ll << 31)) return fail(ctx, MCX_EINVAL, "nphot must be in 1 .. 2^31-1");
     if (run->precision != MCX_FP64 && run->precision != MCX_FP32) return fail(ctx, MCX_EINVAL, "unknown precision mode");
@@ -2226,20 +2439,34 @@ static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_re
     if ((size_t)nm > ctx->bm_cap) {
         HIPCHK(ctx, hipStreamSynchronize(s));
         if (ctx->d_bms) (void)hipFree(ctx->d_bms);
+    if (ctx->gexec) (void)hipGraphExecDestroy((hipGraphExec_t)ctx->gexec);
+    if (ctx->graph) (void)hipGraphDestroy((hipGraph_t)ctx->graph);
         ctx->d_bms = nullptr;
         ctx->bm_cap = 0;
         if (hipMalloc(&ctx->d_bms, sizeof(BatchModel) * (size_t)nm) != hipSuccess)
             return fail(ctx, MCX_ENOMEM, "launch descriptors");
         ctx->bm_cap = (size_t)nm;
     }
-    HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
     std::vector<BatchModel> hb(nm);
     for (int m = 0; m < nm; m++) {
-        int r = prep_model(ctxs[m], run, run->seed + (uint64_t)(seed_stride * m), s, hb[m]);
+        HIPCHK(ctx, hipStreamSynchronize(ctxs[m]->stream));
+        int r = host_prep(ctxs[m], run, run->seed + (uint64_t)(seed_stride * m), hb[m]);
         if (r) return (m == 0) ? r : fail(ctx, r, "model " + std::to_string(m) + ": " + ctxs[m]->err);
         hb[m].ty.vtab = ctx->ty.vtab;
     }
-    HIPCHK(ctx, hipMemcpyAsync(ctx->d_bms, hb.data(), sizeof(BatchModel) * (size_t)nm, hipMemcpyHostToDevice, s));
+    // the descriptors hold each model's key: a graph replays with the keys
+    // of this call
+    HIPCHK(ctx, hipMemcpy(ctx->d_bms, hb.data(), sizeof(BatchModel) * (size_t)nm, hipMemcpyHostToDevice));
+    if (use_graph) {
+        if (serial) return fail(ctx, MCX_EINVAL, "graph replay runs concurrent passes only");
+        LaunchCfg L{kern, threads, rb, bpm, lds, lanes};
+        return graph_pass(ctxs, nm, run, L, hb, status);
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
+    for (int m = 0; m < nm; m++) {
+        int r = enqueue_prep(ctxs[m], hb[m], run->factRW, s);
+        if (r) return r;
+    }
     hipLaunchKernelGGL(k_fill_u4, dim3(1024), dim3(256), 0, s, lanes << VIS_LOG2, ctx->ty.vtab);
     TransportArgs A;
     A.bm = (const BatchModel *)ctx->d_bms;
@@ -2276,8 +2503,7 @@ static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_re
                 HIPCHK(ctx, hipGetLastError());
             }
             hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, c->d_next, (unsigned long long)Pa);
-            HIPCHK(ctx, hipMemsetAsync(c->ty.counters + G_DONE, 0, sizeof(unsigned long long), s));
-            HIPCHK(ctx, hipMemsetAsync(c->ty.counters + G_TLINE, 0, sizeof(unsigned long long) * 6, s));
+            hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (size_t)7, c->ty.counters + G_DONE, 0ull);
         }
         A.Pk = serial ? 0 : Pa;
         A.pid_base = Pa;
@@ -2336,13 +2562,14 @@ static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_re
 
 extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     if (!ctx || !run) return MCX_EINVAL;
-    return run_pass(&ctx, 1, run, 0, 0, nullptr);
+    return run_pass(&ctx, 1, run, 0, 0, 0, nullptr);
 }
 
 extern "C" int mcx_batch_transport(mcx_ctx *const *ctxs, int32_t nmodels, const mcx_run *run,
-                                   int32_t blocks_per_model, int64_t seed_stride, int32_t *status) {
+                                   int32_t blocks_per_model, int64_t seed_stride, int32_t use_graph,
+                                   int32_t *status) {
     if (!ctxs || !run || nmodels < 1 || !ctxs[0]) return MCX_EINVAL;
-    return run_pass(ctxs, nmodels, run, blocks_per_model, seed_stride, status);
+    return run_pass(ctxs, nmodels, run, blocks_per_model, seed_stride, use_graph, status);
 }
 
 extern "C" int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *o) {

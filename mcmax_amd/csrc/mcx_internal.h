@@ -93,6 +93,9 @@ struct DevTally {
     unsigned long long *counters;
     double *tautot;
     double *torep;  // [16] the first abandoned packet's state (time limit)
+    // the pass's results as doubles (k_finalize)
+    double *oEJv, *oEJv2, *oEabs, *oTbw;
+    int *thick;     // C%thick of the pass (null without random-walk tables)
 };
 
 // modified random walk (RandomWalk.f)
@@ -149,6 +152,8 @@ struct mcx_ctx {
     unsigned long long *d_next = nullptr;
     void *d_bms = nullptr;      // launch descriptors of the models of a pass run from this context
     size_t bm_cap = 0;
+    void *graph = nullptr, *gexec = nullptr; // captured pass (mcx_batch_transport with a graph)
+    std::string graph_key;
     double unitE = 0.0, E0 = 0.0;
     double kernel_ms = 0.0, pass_ms = 0.0;
     double image_rays_ms = 0.0, image_spread_ms = 0.0;  // last mcx_image: ray part (host clock), k_image (events)

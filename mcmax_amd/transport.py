@@ -283,11 +283,12 @@ class Transport:
 
 
 def batch_transport(trs, nphot, seed=1, seed_stride=1, blocks_per_model=0, randomwalk=None, factRW=None,
-                    maxinteract=0, precision=0, max_ms=0.0, stage0=0, stage_ratio=0):
+                    maxinteract=0, precision=0, max_ms=0.0, stage0=0, stage_ratio=0, graph=False):
     """One pass of nphot packets for every model of the Transport objects
     trs (one model each) in common launches (mcx_batch_transport); model m
-    uses the key seed + m seed_stride.  Returns the per-model status codes;
-    the tallies stay on each model's context."""
+    uses the key seed + m seed_stride.  graph=True: captured once, then
+    replayed (mcx_batch_transport's use_graph).  Returns the per-model
+    status codes; the tallies stay on each model's context."""
     lib = trs[0].lib
     m0 = trs[0].model
     rw = bool(getattr(m0, "randomwalk", False)) if randomwalk is None else bool(randomwalk)
@@ -297,7 +298,7 @@ def batch_transport(trs, nphot, seed=1, seed_stride=1, blocks_per_model=0, rando
                  max_ms=float(max_ms))
     arr = (C.c_void_p * len(trs))(*[t.ctx for t in trs])
     st = np.zeros(len(trs), dtype=np.int32)
-    rc = lib.mcx_batch_transport(arr, len(trs), C.byref(r), int(blocks_per_model), int(seed_stride),
+    rc = lib.mcx_batch_transport(arr, len(trs), C.byref(r), int(blocks_per_model), int(seed_stride), int(bool(graph)),
                                  st.ctypes.data_as(C.POINTER(C.c_int32)))
     if rc not in (0, _lib.MCX_EGEOM):
         raise MCXError(rc, lib.mcx_last_error(trs[0].ctx).decode())

@@ -51,3 +51,39 @@ def test_batch_equals_single(models, bpm):
     finally:
         for t in trs:
             t.close()
+
+
+def test_batch_graph_replays(models):
+    """The pass captured in a HIP graph and replayed: three replays with
+    different keys, eager passes and synchronisations between them, and a new
+    density for one model (same grid) before the last; every replay equals
+    the eager batched pass with the same keys, bit for bit."""
+    trs = []
+    try:
+        for m in models[:3]:
+            t = transport.Transport(0)
+            t.set_model(m)
+            trs.append(t)
+        n = 12000
+        kw = dict(seed_stride=5, blocks_per_model=2, randomwalk=True, factRW=2.0)
+        for rep, seed in enumerate((101, 202, 303)):
+            if rep == 2:
+                m2 = models[1]
+                dens = m2.dens.copy()
+                m2.dens = dens * 1.5
+                trs[1].set_model(m2, tables=False)
+                m2.dens = dens
+            st = transport.batch_transport(trs, n, seed=seed, graph=True, **kw)
+            assert np.all(st == 0), st
+            got = [t.tallies() for t in trs]
+            # eager work between replays
+            transport.batch_transport(trs, n, seed=seed, graph=False, **kw)
+            want = [t.tallies() for t in trs]
+            for k in range(len(trs)):
+                for key in KEYS:
+                    assert np.array_equal(got[k][key], want[k][key]), (rep, k, key)
+                for key in COUNTS:
+                    assert got[k][key] == want[k][key], (rep, k, key)
+    finally:
+        for t in trs:
+            t.close()
