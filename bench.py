@@ -339,6 +339,52 @@ def precision_study(a, work):
             "ms_fp64": times[0], "ms_fp32": times[1]}
 
 
+def sweep_bench(a, work):
+    """The many-model sweep: --sweep models drawn from the sweep ranges
+    (uneven grids 60-240 x 40-120, dust masses 1e-6..1e-2 Msun, stars
+    3000-10000 K), the production dust, --nphot packets per model per
+    iteration, 3 structure iterations with every pass batched (graph
+    replay, blocks per model from the autotuner).  Then --sweep-check models
+    run alone, model by model (mcx_transport) with the same keys: their
+    final temperatures must equal the batched sweep's bit for bit, and their
+    mean time per model gives the model-by-model time of the whole sweep."""
+    import numpy as np
+    from mcmax_amd import autotune, sweep
+    log = lambda s_: print(f"[sweep] {s_}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    params = sweep.draw(a.sweep, seed=20261203)
+    S = sweep.Sweep(params, work, a.nphot, randomwalk=not a.plain, log=log)
+    setup_s = time.perf_counter() - t0
+    log(f"{a.sweep} models set up in {setup_s:.1f} s")
+    init = [(m.dens.copy(), m.w.copy()) for m in S.models]
+    bpm, tune_rec = autotune.tune(S.trs, a.nphot, log=log)
+    res = S.run(niter=3, seed=a.seed, graph=True, blocks_per_model=bpm)
+    log(f"batched sweep: {res['total_s']:.1f} s ({res['transport_s']:.1f} s in the passes)")
+    rng = np.random.default_rng(7)
+    check = sorted(rng.choice(a.sweep, size=min(a.sweep_check, a.sweep), replace=False).tolist())
+    alone, identical = [], []
+    for k in check:
+        m = S.models[k]
+        m.dens, m.w = init[k][0].copy(), init[k][1].copy()
+        S.trs[k].set_model(m, tables=False)
+        r = S.run_one(k, niter=3, seed=a.seed)
+        alone.append(r["total_s"])
+        identical.append(bool(np.array_equal(r["T"], res["T"][k])))
+        log(f"model {k} alone: {r['total_s']:.1f} s, identical {identical[-1]}")
+    S.close()
+    seq_est = float(np.mean(alone)) * a.sweep
+    ncell = [int(p["nrad"]) * int(p["ntheta"]) for p in params]
+    return {"metric": f"{a.sweep}-model sweep: wall time to 3 structure iterations", "value": res["total_s"],
+            "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"{a.sweep} disk models drawn from the sweep ranges (seed 20261203), production "
+                                   f"dust, {a.nphot} packets per model per iteration, 3 iterations, modified random "
+                                   f"walk {not a.plain}", "models": a.sweep, "cells_min": min(ncell),
+                       "cells_max": max(ncell)},
+            "transport_s": res["transport_s"], "setup_s": setup_s, "blocks_per_model": bpm, "autotune": tune_rec,
+            "model_by_model_s_per_model": alone, "checked_models": check, "identical_to_alone": identical,
+            "model_by_model_estimate_s": seq_est, "speedup_vs_model_by_model": seq_est / res["total_s"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -362,6 +408,10 @@ def main():
                     help="transport precision mode (fp32: MCX_FP32 direction arithmetic)")
     ap.add_argument("--precision-study", action="store_true",
                     help="FP32 against FP64 passes with the same seeds (per-cell T and MC SED errors) instead")
+    ap.add_argument("--sweep", type=int, default=0,
+                    help="time the batched many-model sweep of this many models instead (--nphot per model)")
+    ap.add_argument("--sweep-check", type=int, default=4,
+                    help="models of --sweep also run alone (bit-identity and the model-by-model time)")
     ap.add_argument("--stage-ratio", type=int, default=0,
                     help="staged schedule: stage ratio (0: the library's default 2; -1: growing ratio)")
     ap.add_argument("--budget", type=float, default=400.0,
@@ -395,6 +445,9 @@ def main():
         return
     if a.image:
         print(json.dumps(image_bench(a, work)))
+        return
+    if a.sweep:
+        print(json.dumps(sweep_bench(a, work)))
         return
     if a.precision_study:
         print(json.dumps(precision_study(a, work)))

@@ -1,0 +1,117 @@
+"""Sweeps of many disk models run together (mcx_batch_transport).
+
+The reference holds one model in its module globals and runs the Monte
+Carlo pass for it (Modules.f:141-150, MCRadiation.f:6); a parameter study is
+many runs one after another.  Here every model has its own context on the
+device and each structure iteration's transport pass runs for all of them
+in common launches: the packet histories that end each stage run side by
+side instead of model after model.  Each model's pass is bit-identical to
+its own single-model pass with the same key, so the results do not depend
+on which models share a launch.
+"""
+import os
+import time
+
+import numpy as np
+
+from . import configs, iterate, transport
+
+# the parameter ranges of the sweep (uniform unless noted)
+RANGES = dict(nrad=(60, 240), ntheta=(40, 120), mdust=(1e-6, 1e-2, "log"), rin=(0.05, 1.0, "log"),
+              rout=(50.0, 300.0), denspow=(0.5, 1.5), shpow=(1.0, 1.3), tstar=(3000.0, 10000.0),
+              rstar=(1.0, 3.0))
+
+
+def draw(n, seed=20261203):
+    """n models' keyword overrides drawn from RANGES with a seeded generator
+    (grids nr x ntheta uneven; dust mass, inner radius log-uniform)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        p = {}
+        for k, r in RANGES.items():
+            if k in ("nrad", "ntheta"):
+                p[k] = int(rng.integers(r[0], r[1] + 1))
+            elif len(r) == 3:
+                p[k] = float(np.exp(rng.uniform(np.log(r[0]), np.log(r[1]))))
+            else:
+                p[k] = float(rng.uniform(r[0], r[1]))
+        out.append(p)
+    return out
+
+
+class Sweep:
+    """Models of the production dust (ten DHS species) with drawn star, disk
+    and grid, each on its own context."""
+
+    def __init__(self, params, workdir, nphot, randomwalk=True, device=0, base="production", log=None):
+        self.nphot = int(nphot)
+        self.log = log
+        self.models, self.trs = [], []
+        dust = transport.Transport(device)
+        try:
+            for k, p in enumerate(params):
+                m = configs.load_config(base, os.path.join(workdir, f"m{k:03d}"), tr=dust, randomwalk=randomwalk, **p)
+                self.models.append(m)
+        finally:
+            dust.close()
+        for m in self.models:
+            t = transport.Transport(device)
+            t.set_model(m)
+            self.trs.append(t)
+
+    def close(self):
+        for t in self.trs:
+            t.close()
+        self.trs = []
+
+    def _after_pass(self, k, states, niter):
+        """the rest of one structure iteration for model k (Main.f:458-791):
+        inversion, Diffuse, TempAverage, DiskStructure, the new structure"""
+        t, m = self.trs[k], self.models[k]
+        T, dT, dEJv = t.temperature()
+        tal = t.tallies()
+        EJv = tal["EJv"] / m.V
+        T, _, EJv, _ = t.diffuse(25, 1.0, EJv=EJv)
+        fw = iterate.average_weight(niter, 0.5, 1000)
+        if niter == 1:
+            t.temp_average(fw, first=True)
+        t.temp_average(fw)
+        states[k], _ = t.disk_structure(states[k])
+        m.dens, m.w = states[k]["dens"], states[k]["w"]
+        t.set_model(m, tables=False)
+        return T
+
+    def run(self, niter=3, seed=1, graph=True, blocks_per_model=0, seed_stride=1):
+        """niter structure iterations of every model, each pass batched.
+        Returns dict(T: last temperatures, transport_s, total_s)."""
+        states = [iterate.density_state(m) for m in self.models]
+        t_tr, t0 = 0.0, time.perf_counter()
+        T = [None] * len(self.models)
+        for it in range(1, niter + 1):
+            t1 = time.perf_counter()
+            st = transport.batch_transport(self.trs, self.nphot, seed=seed + 1000 * it, seed_stride=seed_stride,
+                                           blocks_per_model=blocks_per_model, graph=graph)
+            if np.any(st != 0):
+                raise RuntimeError(f"batched pass: status {st}")
+            t_tr += time.perf_counter() - t1
+            for k in range(len(self.models)):
+                T[k] = self._after_pass(k, states, it)
+            if self.log:
+                self.log(f"iteration {it}: {time.perf_counter() - t0:.1f} s")
+        return dict(T=T, transport_s=t_tr, total_s=time.perf_counter() - t0)
+
+    def run_one(self, k, niter=3, seed=1, seed_stride=1):
+        """model k alone, its passes model by model (mcx_transport) with the
+        keys the batched sweep gives it.  Returns dict(T, transport_s,
+        total_s)."""
+        t, m = self.trs[k], self.models[k]
+        states = {k: iterate.density_state(m)}
+        t_tr, t0 = 0.0, time.perf_counter()
+        T = None
+        for it in range(1, niter + 1):
+            t1 = time.perf_counter()
+            t.transport(self.nphot, seed=seed + 1000 * it + seed_stride * k)
+            t_tr += time.perf_counter() - t1
+            T = self._after_pass(k, states, it)
+        return dict(T=T, transport_s=t_tr, total_s=time.perf_counter() - t0)
