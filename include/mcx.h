@@ -417,6 +417,67 @@ typedef struct {
 int mcx_compute_parts(mcx_ctx *ctx, int32_t nlam, const double *lam, int32_t nspec, const mcx_part_spec *specs,
                       mcx_part_out *out, mcx_part_stats *stats);
 
+/* ---- warm-start network ------------------------------------------------ */
+
+/* No counterpart in the reference, which starts every structure iteration
+ * from the fixed guess Tstar sqrt(R_av(1)/R_av(i))/10 (Init.f:3935) and reads
+ * the first pass's temperatures in the first vertical-structure solve
+ * (DiskStructure.f:19, 36-44, 71; MCRadiation.f:276-282 zeroes T per pass).
+ * A network maps a cell's features to log T so that the first solve can
+ * start near the converged field; the Monte Carlo pass judges it.
+ *
+ * Shape: MCX_MLP_NIN inputs -> MCX_MLP_DEPTH hidden layers of MCX_MLP_WIDTH
+ * (SiLU) -> 1 output.  Parameters are one FP32 vector [MCX_MLP_NPARAMS]:
+ * W1 [nin][width], b1 [width], then per hidden layer l = 2..depth
+ * W_l [width in][width out], b_l [width], then w_out [width], b_out.
+ * Layers 2..depth and the output run in BF16 (weights and activations) on
+ * the matrix cores with FP32 accumulation; layer 1 runs in FP32.  Arrays are
+ * host memory owned by the caller; the handle owns its device buffers. */
+#define MCX_MLP_NIN 8
+#define MCX_MLP_WIDTH 256
+#define MCX_MLP_DEPTH 4
+#define MCX_MLP_NPARAMS 199937
+
+typedef struct mcx_mlp mcx_mlp;
+
+/* Other shapes than the one above return MCX_EINVAL. */
+int mcx_mlp_create(mcx_mlp **net, int device, int32_t nin, int32_t width, int32_t depth);
+void mcx_mlp_destroy(mcx_mlp *net);
+const char *mcx_mlp_last_error(const mcx_mlp *net);
+
+/* Load / read the FP32 master parameters (loading resets the Adam state). */
+int mcx_mlp_set_params(mcx_mlp *net, const float *params);
+int mcx_mlp_get_params(mcx_mlp *net, float *params);
+
+/* y [n] = network(x [n][nin]). */
+int mcx_mlp_forward(mcx_mlp *net, int64_t n, const float *x, float *y);
+
+/* The loss sum_r (y_r - t_r)^2 / n over the n rows of x [n][nin], t [n], and
+ * its gradient grad [MCX_MLP_NPARAMS] (may be NULL) with respect to the
+ * parameters: forward and backward on the device. */
+int mcx_mlp_gradient(mcx_mlp *net, int64_t n, const float *x, const float *t, double *loss, float *grad);
+
+/* The training set (copied to the device): x [n][nin], t [n]. */
+int mcx_mlp_set_data(mcx_mlp *net, int64_t n, const float *x, const float *t);
+
+typedef struct {
+    int32_t batch;                   /* rows per step, drawn with replacement
+                                        from the training set               */
+    int32_t nsteps;
+    float lr, beta1, beta2, eps;     /* Adam                                */
+    uint64_t seed;                   /* key of the batch draws              */
+} mcx_mlp_train_opts;
+
+/* nsteps Adam steps on the mean squared error of minibatches of the
+ * training set; loss [nsteps] (may be NULL) receives each step's loss.
+ * Deterministic for a given seed and starting state. */
+int mcx_mlp_train(mcx_mlp *net, const mcx_mlp_train_opts *opts, float *loss);
+
+/* Kernel time of the last mcx_mlp_forward (HIP events, summed over its
+ * chunks) and its row count, and the mean time of a training step of the
+ * last mcx_mlp_train. */
+int mcx_mlp_last_timing(const mcx_mlp *net, double *forward_ms, int64_t *forward_rows, double *step_ms);
+
 #ifdef __cplusplus
 }
 #endif

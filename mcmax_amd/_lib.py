@@ -116,7 +116,16 @@ EXPORTS = (
     "mcx_transport", "mcx_get_tallies", "mcx_temperature", "mcx_last_timing", "mcx_sed",
     "mcx_set_temperature", "mcx_diffuse", "mcx_temp_average", "mcx_disk_structure", "mcx_compare_cells",
     "mcx_compute_parts", "mcx_image", "mcx_last_image_timing", "mcx_batch_transport",
+    "mcx_mlp_create", "mcx_mlp_destroy", "mcx_mlp_last_error", "mcx_mlp_set_params", "mcx_mlp_get_params",
+    "mcx_mlp_forward", "mcx_mlp_gradient", "mcx_mlp_set_data", "mcx_mlp_train", "mcx_mlp_last_timing",
 )
+
+MLP_NIN, MLP_WIDTH, MLP_DEPTH, MLP_NPARAMS = 8, 256, 4, 199937
+
+
+class MlpTrainOpts(C.Structure):
+    _fields_ = [("batch", C.c_int32), ("nsteps", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64)]
 
 _lib = None
 
@@ -157,6 +166,21 @@ def load():
     lib.mcx_last_image_timing.argtypes = [vp, _dp, _dp]
     lib.mcx_image.argtypes = [vp, C.POINTER(Obs), _dp, _dp, C.c_double, C.POINTER(ImageOpts), _dp, _dp, _dp, _dp,
                               C.POINTER(C.c_int32), C.c_int32, _dp, _dp]
+    lib.mcx_mlp_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int32, C.c_int32, C.c_int32]
+    lib.mcx_mlp_destroy.argtypes = [vp]
+    lib.mcx_mlp_destroy.restype = None
+    lib.mcx_mlp_last_error.argtypes = [vp]
+    lib.mcx_mlp_last_error.restype = C.c_char_p
+    lib.mcx_mlp_set_params.argtypes = [vp, _fp]
+    lib.mcx_mlp_get_params.argtypes = [vp, _fp]
+    lib.mcx_mlp_forward.argtypes = [vp, C.c_int64, _fp, _fp]
+    lib.mcx_mlp_gradient.argtypes = [vp, C.c_int64, _fp, _fp, _dp, _fp]
+    lib.mcx_mlp_set_data.argtypes = [vp, C.c_int64, _fp, _fp]
+    lib.mcx_mlp_train.argtypes = [vp, C.POINTER(MlpTrainOpts), _fp]
+    lib.mcx_mlp_last_timing.argtypes = [vp, _dp, C.POINTER(C.c_int64), _dp]
+    for n in ("mcx_mlp_create", "mcx_mlp_set_params", "mcx_mlp_get_params", "mcx_mlp_forward", "mcx_mlp_gradient",
+              "mcx_mlp_set_data", "mcx_mlp_train", "mcx_mlp_last_timing"):
+        getattr(lib, n).restype = C.c_int
     for n in ("mcx_set_tables", "mcx_set_model", "mcx_transport", "mcx_get_tallies",
               "mcx_temperature", "mcx_last_timing", "mcx_sed", "mcx_set_temperature", "mcx_diffuse",
               "mcx_temp_average", "mcx_disk_structure", "mcx_compare_cells", "mcx_compute_parts", "mcx_image",
@@ -174,4 +198,6 @@ def ptr(a):
         return a.ctypes.data_as(_dp)
     if a.dtype == np.int32:
         return a.ctypes.data_as(_ip)
+    if a.dtype == np.float32:
+        return a.ctypes.data_as(_fp)
     raise TypeError(a.dtype)

@@ -15,6 +15,7 @@ Arithmetic keeps the reference's association and its single-precision
 places (REAL(i)/REAL(n) in the theta grid, default-REAL constants AU, Rsun,
 Msun), so the tables agree with an independent restatement to rounding.
 """
+import hashlib
 import os
 
 import numpy as np
@@ -300,6 +301,32 @@ def write_analytic_particle(path, lam, kappa0=1e3, albedo=0.5, g=0.5):
 
 
 # ---------------------------------------------------------------------------
+# computepart results by (wavelength grid, species inputs): the models of a
+# sweep share their dust, which is then computed once per process
+_PART_CACHE = {}
+_PART_CACHE_MAX = 4
+
+
+def _parts_key(lam, specs):
+    h = hashlib.sha256(np.ascontiguousarray(lam, dtype=np.float64).tobytes())
+    for sp in specs:
+        for k in sorted(sp):
+            v = sp[k]
+            h.update(k.encode())
+            h.update(np.ascontiguousarray(v).tobytes() if isinstance(v, (np.ndarray, list, tuple)) else repr(v).encode())
+    return h.hexdigest()
+
+
+def compute_parts_cached(tr, lam, specs):
+    """tr.compute_parts(lam, specs), remembered for the same inputs"""
+    key = _parts_key(lam, specs)
+    if key not in _PART_CACHE:
+        if len(_PART_CACHE) >= _PART_CACHE_MAX:
+            _PART_CACHE.pop(next(iter(_PART_CACHE)))
+        _PART_CACHE[key] = tr.compute_parts(lam, specs)
+    return _PART_CACHE[key]
+
+
 class Model:
     """All setup arrays of one disk model (numpy, float64)."""
 
@@ -383,12 +410,12 @@ def build_model(kw, basedir=".", tr=None, device=0):
     comp = [i for i, (kind, _) in enumerate(species) if kind == "compute"]
     m.dust_stats = None
     if comp:
-        own = tr is None
+        own = tr is None and _parts_key(m.lam, [species[i][1] for i in comp]) not in _PART_CACHE
         if own:
             from .transport import Transport
             tr = Transport(device)
         try:
-            res, m.dust_stats = tr.compute_parts(m.lam, [species[i][1] for i in comp])
+            res, m.dust_stats = compute_parts_cached(tr, m.lam, [species[i][1] for i in comp])
         finally:
             if own:
                 tr.close()
