@@ -473,6 +473,16 @@ typedef struct {
  * Deterministic for a given seed and starting state. */
 int mcx_mlp_train(mcx_mlp *net, const mcx_mlp_train_opts *opts, float *loss);
 
+/* No counterpart in the reference: mcx_mlp_forward for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops).  The same computation, bit for bit, with x and y staged through a
+ * fixed device array of the library (MCX_MLP_NOALLOC_MAXROWS rows, 2.25 MiB
+ * per GPU, held until the process ends); no allocation of any kind.  Larger
+ * n returns MCX_EINVAL.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (from several threads or handles) race. */
+#define MCX_MLP_NOALLOC_MAXROWS 65536
+int mcx_mlp_forward_noalloc(mcx_mlp *net, int64_t n, const float *x, float *y);
+
 /* Kernel time of the last mcx_mlp_forward (HIP events, summed over its
  * chunks) and its row count, and the mean time of a training step of the
  * last mcx_mlp_train. */

@@ -118,9 +118,11 @@ EXPORTS = (
     "mcx_compute_parts", "mcx_image", "mcx_last_image_timing", "mcx_batch_transport",
     "mcx_mlp_create", "mcx_mlp_destroy", "mcx_mlp_last_error", "mcx_mlp_set_params", "mcx_mlp_get_params",
     "mcx_mlp_forward", "mcx_mlp_gradient", "mcx_mlp_set_data", "mcx_mlp_train", "mcx_mlp_last_timing",
+    "mcx_mlp_forward_noalloc",
 )
 
 MLP_NIN, MLP_WIDTH, MLP_DEPTH, MLP_NPARAMS = 8, 256, 4, 199937
+MLP_NOALLOC_MAXROWS = 65536
 
 
 class MlpTrainOpts(C.Structure):
@@ -174,11 +176,13 @@ def load():
     lib.mcx_mlp_set_params.argtypes = [vp, _fp]
     lib.mcx_mlp_get_params.argtypes = [vp, _fp]
     lib.mcx_mlp_forward.argtypes = [vp, C.c_int64, _fp, _fp]
+    lib.mcx_mlp_forward_noalloc.argtypes = [vp, C.c_int64, _fp, _fp]
     lib.mcx_mlp_gradient.argtypes = [vp, C.c_int64, _fp, _fp, _dp, _fp]
     lib.mcx_mlp_set_data.argtypes = [vp, C.c_int64, _fp, _fp]
     lib.mcx_mlp_train.argtypes = [vp, C.POINTER(MlpTrainOpts), _fp]
     lib.mcx_mlp_last_timing.argtypes = [vp, _dp, C.POINTER(C.c_int64), _dp]
-    for n in ("mcx_mlp_create", "mcx_mlp_set_params", "mcx_mlp_get_params", "mcx_mlp_forward", "mcx_mlp_gradient",
+    for n in ("mcx_mlp_create", "mcx_mlp_set_params", "mcx_mlp_get_params", "mcx_mlp_forward",
+              "mcx_mlp_forward_noalloc", "mcx_mlp_gradient",
               "mcx_mlp_set_data", "mcx_mlp_train", "mcx_mlp_last_timing"):
         getattr(lib, n).restype = C.c_int
     for n in ("mcx_set_tables", "mcx_set_model", "mcx_transport", "mcx_get_tallies",

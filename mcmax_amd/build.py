@@ -5,7 +5,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRCS = [os.path.join(HERE, "csrc", f) for f in ("mcx.hip", "sed.hip", "diffuse.hip", "structure.hip", "dust.hip", "mlp.hip")]
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("mcx.hip", "sed.hip", "diffuse.hip", "structure.hip", "dust.hip", "mlp.hip", "mlp_noalloc.hip")]
 LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 OUT = os.path.join(HERE, "libmcx.so")
 
