@@ -385,6 +385,81 @@ def sweep_bench(a, work):
             "model_by_model_estimate_s": seq_est, "speedup_vs_model_by_model": seq_est / res["total_s"]}
 
 
+def warmstart_bench(a, work):
+    """The learned warm start on a fresh sweep: --sweep models (default 64)
+    drawn from the sweep ranges with the evaluation stream (the network was
+    trained on another stream), --nphot packets per model per pass, K =
+    --passes passes each, batched: two independent cold starts (A: the
+    reference field, its last pass; B: the cold accuracy per pass) and the
+    warm start W (the network's field first, judged by the first pass with
+    fallback above 5%).  Reports per pass the median over models of the
+    median |dT|/T against A's final field, the passes cold and warm need to
+    come within 10% of the cold run's own final accuracy, and the network's
+    forward throughput on the matrix cores."""
+    import numpy as np
+    from mcmax_amd import mlp, warmstart
+    log = lambda s_: print(f"[warmstart] {s_}", file=sys.stderr, flush=True)
+    path = a.network or os.path.join(ROOT, "mcmax_amd", "data", "warmstart_net.npz")
+    net = mlp.Network.load(path)
+    nm, K = (a.sweep or 64), a.passes
+    params = warmstart.draw_models(nm, seed=0)
+    runs = {}
+    for name, seed, warm in (("A", a.seed, False), ("B", a.seed + 7, False), ("W", a.seed + 13, True)):
+        B = warmstart.Batch(params, os.path.join(work, name), a.nphot, randomwalk=not a.plain, log=None)
+        try:
+            T0 = None
+            t_net = 0.0
+            if warm:
+                t1 = time.perf_counter()
+                T0 = warmstart.predict_fields(net, B.models)
+                t_net = time.perf_counter() - t1
+            t1 = time.perf_counter()
+            r = B.iterate(K, seed=seed, T_init=T0)
+            r["wall_s"] = time.perf_counter() - t1
+            r["net_s"] = t_net
+            r["T0"] = T0
+            runs[name] = r
+            log(f"run {name}: {r['wall_s']:.1f} s ({r['transport_s']:.1f} s in the passes)")
+        finally:
+            B.close()
+    ref = [runs["A"]["T"][k][-1] for k in range(nm)]
+
+    def curve(r):
+        out = []
+        for p in range(K):
+            v = [warmstart.median_rel(r["T"][k][p], ref[k]) for k in range(nm) if r["T"][k][p] is not None]
+            out.append(float(np.median(v)) if v else None)
+        return out
+    cold, warm = curve(runs["B"]), curve(runs["W"])
+    net0 = float(np.median([warmstart.median_rel(runs["W"]["T0"][k], ref[k]) for k in range(nm)]))
+    floor = cold[-1]
+    target = 1.1 * floor
+    reach = lambda c: next((p + 1 for p, v in enumerate(c) if v is not None and v <= target), None)
+    pc, pw = reach(cold), reach(warm)
+    acc = [x for x in runs["W"]["accepted"] if x is not None]
+    # network throughput on a large batch of cells
+    X = np.random.default_rng(1).normal(size=(1 << 22, 8)).astype(np.float32)
+    net.forward_raw(X)
+    t = net.timing()
+    flop = 2.0 * (8 * 256 + 3 * 256 * 256 + 256) * t["forward_rows"]
+    tfs = flop / (t["forward_ms"] * 1e-3) / 1e12
+    return {"metric": "learned warm start: passes to the cold start's final accuracy", "value": pw, "unit": "passes",
+            "higher_is_better": False, "n_gpus": 1, "dtype": "bf16 network, f64 transport", "data": "synthetic",
+            "config": {"workload": f"{nm} disk models from the sweep ranges (evaluation stream), production dust, "
+                                   f"{a.nphot} packets per model per pass, {K} passes", "models": nm},
+            "cold_passes": pc, "warm_passes": pw,
+            "passes_saved": (pc - pw) if (pc and pw) else None,
+            "packets_saved": ((pc - pw) * a.nphot * nm) if (pc and pw) else None,
+            "target_median_dT_over_T": target, "cold_final_accuracy": floor,
+            "median_dT_over_T_per_pass": {"cold": cold, "warm": warm, "network_alone": net0},
+            "accepted": int(sum(acc)), "judged": len(acc),
+            "judge_median": float(np.median([x for x in runs["W"]["judged"] if x is not None])),
+            "wall_s": {k: runs[k]["wall_s"] for k in runs}, "network_s": runs["W"]["net_s"],
+            "roofline": {"bound": "mfma", "kernel": "k_mlp_fwd", "achieved": tfs, "peak": 2500.0, "unit": "TFLOP/s",
+                         "frac": tfs / 2500.0, "rows": t["forward_rows"], "ms": t["forward_ms"],
+                         "flop_per_row": 2.0 * (8 * 256 + 3 * 256 * 256 + 256)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -412,6 +487,12 @@ def main():
                     help="time the batched many-model sweep of this many models instead (--nphot per model)")
     ap.add_argument("--sweep-check", type=int, default=4,
                     help="models of --sweep also run alone (bit-identity and the model-by-model time)")
+    ap.add_argument("--warmstart", action="store_true",
+                    help="evaluate the learned warm start on a fresh sweep instead (--sweep models, --passes)")
+    ap.add_argument("--passes", type=int, default=4, help="passes per model of --warmstart")
+    ap.add_argument("--network", default="", help="network file of --warmstart (default: the committed one)")
+    ap.add_argument("--epoch-deposits", type=int, default=0,
+                    help="staged schedule: deposits per epoch step (0: the library's default; -1: none)")
     ap.add_argument("--stage-ratio", type=int, default=0,
                     help="staged schedule: stage ratio (0: the library's default 2; -1: growing ratio)")
     ap.add_argument("--budget", type=float, default=400.0,
@@ -446,6 +527,9 @@ def main():
     if a.image:
         print(json.dumps(image_bench(a, work)))
         return
+    if a.warmstart:
+        print(json.dumps(warmstart_bench(a, work)))
+        return
     if a.sweep:
         print(json.dumps(sweep_bench(a, work)))
         return
@@ -468,7 +552,7 @@ def main():
             break
         t1 = time.perf_counter()
         tr.transport(a.nphot, seed=a.seed + 1000 + w, nlanes=a.lanes, stage_ratio=a.stage_ratio, precision=prec,
-                     max_ms=1e3 * a.budget)
+                     max_ms=1e3 * a.budget, epoch_deposits=a.epoch_deposits)
         tr.temperature()
         longest = max(longest, time.perf_counter() - t1)
         warm += 1
@@ -481,7 +565,7 @@ def main():
             break
         t1 = time.perf_counter()
         rc = tr.transport(a.nphot, seed=a.seed + s, nlanes=a.lanes, stage_ratio=a.stage_ratio, precision=prec,
-                          max_ms=1e3 * a.budget)
+                          max_ms=1e3 * a.budget, epoch_deposits=a.epoch_deposits)
         k, _ = tr.timing()
         kms.append(k)
         tr.temperature()

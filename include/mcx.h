@@ -110,8 +110,10 @@ typedef struct {
                                         own earlier deposits in c, S_k(c) the
                                         Eabs of the packets before the stage
                                         (the serial loop's running total in
-                                        expectation).  A pass is then
-                                        bit-reproducible for a given seed   */
+                                        expectation; see epoch_deposits).
+                                        All stages run in one launch; a
+                                        pass is bit-reproducible for a given
+                                        seed                                */
     int32_t randomwalk;              /* keyword randomwalk (RNDW): modified
                                         random walk in thick cells          */
     double factRW;                   /* keyword factRW (Init.f:205: 10)     */
@@ -135,6 +137,19 @@ typedef struct {
                                         323-342, as a hard stop): past it
                                         every lane abandons its packet and
                                         the call returns MCX_ETIME          */
+    int64_t epoch_deposits;          /* staged schedule, B: a packet of
+                                        stage j files its d-th Eabs deposit
+                                        under epoch j + m, m the smallest
+                                        m >= 1 with d < B (2^m - 1), and S_k
+                                        sums epochs 1..k; stage k starts once
+                                        every earlier packet has finished or
+                                        gone past the deposits S_k counts,
+                                        so a long history holds back only
+                                        the next stages' starts by B, 3B,
+                                        7B, ... of its deposits.  0: B =
+                                        16000; < 0: S_k holds every deposit
+                                        of the packets before P_k (each
+                                        stage waits for all of them)        */
 } mcx_run;
 
 #define MCX_FP64 0

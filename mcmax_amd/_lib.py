@@ -43,7 +43,7 @@ class Run(C.Structure):
         ("nphot", C.c_int64), ("seed", C.c_uint64), ("maxinteract", C.c_int32),
         ("nlanes", C.c_int32), ("stage0", C.c_int64), ("randomwalk", C.c_int32),
         ("factRW", C.c_double), ("coop_lanes", C.c_int32), ("stage_ratio", C.c_int32),
-        ("precision", C.c_int32), ("max_ms", C.c_double),
+        ("precision", C.c_int32), ("max_ms", C.c_double), ("epoch_deposits", C.c_int64),
     ]
 
 

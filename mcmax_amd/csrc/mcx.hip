@@ -52,8 +52,13 @@ struct TransportArgs {
     int bpm;              // blocks per model
     long long pid_end;
     long long pid_base;
-    long long Pk;  // packets before the stage (the snapshot's count; 0: none)
     long long nphot;
+    // the stages of a concurrent pass: stage k holds packets [P[k], P[k+1]),
+    // P[nstage] = nphot; depB: the deposits a packet makes per epoch unit
+    // (0: every deposit of a stage-j packet is in epoch j+1)
+    int nstage;
+    unsigned depB;
+    const long long *P;  // [nstage + 1] in device memory
     int maxinteract;
     unsigned long long deadline; // wall_clock64() past which lanes abandon their packets (0: none)
     int serial;    // only lane 0 of the grid takes packets (in packet order)
@@ -61,7 +66,7 @@ struct TransportArgs {
     int rb_mask;   // draws per lane ring minus one (a power of two minus one)
 };
 
-enum { S_NEED = 0, S_EMIT, S_STEP, S_INTERACT, S_LOOKUP, S_AFTER, S_ESCAPE, S_RW, S_DONE };
+enum { S_NEED = 0, S_EMIT, S_STEP, S_INTERACT, S_LOOKUP, S_AFTER, S_ESCAPE, S_RW, S_WAIT, S_DONE };
 
 // x**0.25 of the T^4 interpolation as two correctly rounded square roots:
 // within 2 ulp of pow(x, 0.25) and an order of magnitude shorter on the GPU.
@@ -209,6 +214,7 @@ struct Packet {
     double x, y, z, vx, vy, vz, E, wl1;
     double tau0, tau, nEJv;
     int i, jj, l1, edgeNr, ntrace, nstep;
+    unsigned ndep; // Eabs deposits so far (their epochs, concurrent passes)
     bool onEdge, scatt, pol;
 };
 
@@ -232,6 +238,64 @@ __device__ __forceinline__ unsigned long long add_eabs(const DevTally &ty, int c
     }
     q = (unsigned long long)qd;
     return atomicAdd(&ty.Eabs[c], q);
+}
+
+// ---- the gated stages of a concurrent pass -------------------------------
+// A packet of stage j files its d-th Eabs deposit (0-based) under epoch j + m,
+// m the smallest m >= 1 with d < B (2^m - 1) (B = 0: m = 1), at most cap.
+// Stage k's snapshot S_k sums epochs 1..k: a packet's deposits reach the
+// snapshots of later stages in steps of B, 2B, 4B, ... deposits, so a long
+// history does not hold the next stages back, and every snapshot is still a
+// fixed function of the packets' own histories.
+__device__ __forceinline__ int dep_epoch(int j, unsigned d, unsigned B, int cap) {
+    const int m = B ? 32 - __clz(d / B + 1u) : 1;
+    const int e = j + m;
+    return e < cap ? e : cap;
+}
+
+// loads and stores that other XCDs see within the launch (the snapshots)
+__device__ __forceinline__ unsigned long long ld_coh(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float ld_coh(const float *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(float *p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the stage of packet pid in the stage table sP[0..nstage] (LDS)
+__device__ __forceinline__ int stage_of(const long long *sP, int nstage, long long pid) {
+    int lo = 0, hi = nstage - 1;  // sP[lo] <= pid < sP[hi + 1]
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pid >= sP[mid]) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// A gated pass's deposit of q Eabs units in cell c by a packet of stage pk
+// that has made ndep deposits so far: filed under its epoch; when the packet
+// passes B (2^m - 1) deposits it is ready for stage pk + m (its deposits up
+// to there performed before the count, release order)
+__device__ __forceinline__ void gated_deposit(const DevTally &ty, const TransportArgs &A, int ncell, int c,
+                                              unsigned long long q, int pk, unsigned &ndep) {
+    atomicAdd(&ty.Eabs[(size_t)dep_epoch(pk, ndep, A.depB, A.nstage) * ncell + c], q);
+    ndep++;
+    if (A.depB && ndep % A.depB == 0u) {
+        const unsigned r = ndep / A.depB + 1u;
+        if ((r & (r - 1u)) == 0u) {
+            const int k = pk + (31 - __clz(r));
+            if (k < A.nstage) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                atomicAdd(&ty.gate[G_PASS + k], 1ull);
+            }
+        }
+    }
 }
 
 // hunt (Subroutines.f:2086-2130) in the decreasing survival table, from jlo = 1;
@@ -486,7 +550,7 @@ struct RingRng {
     }
 };
 
-#define NPL 23          // per-thread LDS doubles, 22 used (odd stride spreads the banks)
+#define NPL 23          // per-thread LDS doubles, all used (odd stride spreads the banks)
 #define COOP_LANES 6 // default: a wave serves lookups together when at most this many lanes need one
 
 // MODE 0: one dust species; 1: several, with per-class reemission tables;
@@ -552,6 +616,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
 #define P_W0 pl[19]   // ... and its three values
 #define P_W1 pl[20]
 #define P_W2 pl[21]
+#define P_PK pl[22]   // the stage of the lane's packet
     for (int k = threadIdx.x; k <= nr; k += blockDim.x) sR2[k] = md.R2[k];
     for (int k = threadIdx.x; k <= nt; k += blockDim.x) {
         sT2[k] = md.T2[k];
@@ -593,6 +658,14 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     __shared__ int rq_own[256];
     long long pid = -1;
     int st = S_NEED;
+    // the highest stage whose snapshot each wave has seen complete
+    __shared__ int s_gopen[16];
+    if ((threadIdx.x & 63) == 0) s_gopen[threadIdx.x >> 6] = 0;
+    // the stage table
+    __shared__ long long sP[KSTAGE_MAX + 1];
+    for (int q = threadIdx.x; q <= A.nstage; q += blockDim.x) sP[q] = A.P[q];
+    __syncthreads();
+    P_PK = 0.0;
     if (A.serial && (blockIdx.x * blockDim.x + threadIdx.x) != 0) st = S_DONE;
     const long long raar_limit = (long long)nr * (long long)(nt + 1) * 2;
     const double AU = MCX_AU;
@@ -638,6 +711,14 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 atomicMax(&ty.counters[G_MAXSTEPS], (unsigned long long)p.nstep);
                 atomicAdd(&scnt[C_NSTEPS], (unsigned long long)p.nstep);
             }
+            if (!A.serial) {
+                // a finished packet is ready for every stage from the epoch
+                // its next deposit would have had (release: its deposits first)
+                const int kf = dep_epoch((int)P_PK, p.ndep, A.depB, A.nstage);
+                const bool cred = fin && kf < A.nstage;
+                if (__ballot(cred)) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                if (cred) atomicAdd(&ty.gate[G_FIN + kf], 1ull);
+            }
             if (fmask && lane == __ffsll((long long)fmask) - 1) {
                 const unsigned long long nf = (unsigned long long)__popcll(fmask);
                 const unsigned long long d0 = atomicAdd(&ty.counters[G_DONE], nf);
@@ -671,10 +752,75 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                         cc = -1;
                         P_IC = -1.0;
                         P_VSLOT = -1.0;
-                        P_FS = (A.Pk > 0) ? (double)pid / (double)A.Pk : 0.0;
-                        st = S_EMIT;
+                        p.ndep = 0u;
+                        const int pk = A.serial ? 0 : stage_of(sP, A.nstage, mine);
+                        P_PK = (double)pk;
+                        P_FS = (pk > 0) ? (double)pid / (double)sP[pk] : 0.0;
+                        // a packet of a stage whose snapshot is not complete waits
+                        st = (pk <= s_gopen[threadIdx.x >> 6]) ? S_EMIT : S_WAIT;
                     }
                 }
+            }
+        }
+        // ---- gated stages: lanes whose packet waits for its stage's
+        //      snapshot; their wave checks the next stage's gate (every packet
+        //      before the stage finished, or past the deposits the stage
+        //      counts) and, once open, builds a chunk of 64 cells of its
+        //      snapshot; the wave that completes the last chunk opens it
+        if (!A.serial) {
+            const unsigned long long wmask = __ballot(st == S_WAIT);
+            if (wmask) {
+                const int ld = __ffsll((long long)wmask) - 1;
+                unsigned op = 0;
+                if (lane == ld) op = (unsigned)ld_coh(&ty.gate[G_OPEN]);
+                const int open = (int)__shfl(op, ld);
+                int g_open = s_gopen[threadIdx.x >> 6];
+                if (open > g_open) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    g_open = open;
+                    if (lane == 0) s_gopen[threadIdx.x >> 6] = open;
+                }
+                if (st == S_WAIT && (int)P_PK <= g_open) st = S_EMIT;
+                const int gk = g_open + 1;
+                if (__ballot(st == S_WAIT) && gk < A.nstage) {
+                    unsigned long long v = 0ull;
+                    if (lane >= 1 && lane <= gk) v = ld_coh(&ty.gate[G_FIN + lane]);
+                    if (lane == 0) v = ld_coh(&ty.gate[G_PASS + gk]);
+                    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+                    const int nchunk = (md.ncell + 63) >> 6;
+                    if (v == (unsigned long long)sP[gk]) {
+                        unsigned ch = 0;
+                        if (lane == 0) ch = (unsigned)atomicAdd(&ty.gate[G_CHUNK + gk], 1ull);
+                        ch = __shfl(ch, 0);
+                        if (ch < (unsigned)nchunk) {
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                            const int c = (int)ch * 64 + lane;
+                            if (c < md.ncell) {
+                                const size_t nc = (size_t)md.ncell;
+                                const unsigned long long S =
+                                    ((gk > 1) ? ld_coh(&ty.Sk[(size_t)(gk - 1) * nc + c]) : 0ull) +
+                                    ld_coh(&ty.Eabs[(size_t)gk * nc + c]);
+                                st_coh(&ty.Sk[(size_t)gk * nc + c], S);
+                                const float Tg =
+                                    S ? (float)dev_determineT(md.KpCls + (size_t)md.cls[c] * (TMAXD + 1),
+                                                              ((double)S * Bm.unitE) / md.mass[c], 1.0)
+                                      : 0.f;
+                                st_coh(&ty.Tk[(size_t)gk * nc + c], Tg);
+                            }
+                            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                            if (lane == 0) {
+                                const unsigned long long d = atomicAdd(&ty.gate[G_CHDONE + gk], 1ull);
+                                if (d + 1ull == (unsigned long long)nchunk) {
+                                    ty.gate[G_TOPEN + gk] = wall_clock64();
+                                    __hip_atomic_store(&ty.gate[G_OPEN], (unsigned long long)gk, __ATOMIC_RELEASE,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+                                }
+                            }
+                        }
+                    }
+                }
+                // a wave with nothing to do but wait polls less often
+                if (__all(st == S_WAIT || st == S_DONE)) __builtin_amdgcn_s_sleep(16);
             }
         }
         if (__all(st == S_DONE)) break;
@@ -802,9 +948,10 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 // c's slot of the visit table, and the stage's snapshot
                 vh = ((unsigned)c * 0x9E3779B1u) >> (32 - VIS_LOG2);
                 ve = vt[vh];
-                if (!A.serial) {
-                    nS = ty.Esnap[c];
-                    nTg = ty.Tst[c];
+                const int pk = (int)P_PK;
+                if (!A.serial && pk > 0) {
+                    nS = ld_coh(&ty.Sk[(size_t)pk * md.ncell + c]);
+                    nTg = ld_coh(&ty.Tk[(size_t)pk * md.ncell + c]);
                 }
             }
             if (newk) {
@@ -1006,7 +1153,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     // packet's view of the cell; the add itself returns nothing
                     // anyone waits for
                     const unsigned long long q = eabs_q(Ex, Bm.invE, scnt);
-                    atomicAdd(&ty.Eabs[c], q);
+                    gated_deposit(ty, A, md.ncell, c, q, (int)P_PK, p.ndep);
                     const unsigned long long o0 = (unsigned long long)__double_as_longlong(P_OWN), o1 = o0 + q;
                     P_OWN = __longlong_as_double((long long)o1);
                     if (P_VSLOT >= 0.0 && P_VSLOT < (double)(1 << VIS_LOG2)) P_VSLOT += (double)(1 << VIS_LOG2);
@@ -1446,7 +1593,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                     P_TRW = (T1 > Thint) ? T1 : Thint;
                 } else {
                     const unsigned long long q = eabs_q(ejv * c_dens, Bm.invE, scnt);
-                    atomicAdd(&ty.Eabs[c], q);
+                    gated_deposit(ty, A, md.ncell, c, q, (int)P_PK, p.ndep);
                     const unsigned long long o1 = (unsigned long long)__double_as_longlong(P_OWN) + q;
                     P_OWN = __longlong_as_double((long long)o1);
                     if (P_VSLOT >= 0.0 && P_VSLOT < (double)(1 << VIS_LOG2)) P_VSLOT += (double)(1 << VIS_LOG2);
@@ -1508,6 +1655,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
 #undef P_KDIT
 #undef P_FS
 #undef P_BASE
+#undef P_PK
 #undef P_OWN
 #undef P_VSLOT
 #undef P_TG
@@ -1627,19 +1775,6 @@ __global__ void k_diffcoeff(int nclass, int nlam, int ng, int scattering, const 
     KD[idx] = make_double2(i2 / i1, i3 / i2);
 }
 
-// the stage's snapshot: Eabs of the packets before it, and the temperature
-// it gives each cell (a start for the temperature searches of the stage)
-__global__ void k_stage_snap(int ncell, const unsigned long long *__restrict__ Eabs, const double *__restrict__ KpCls,
-                             const int *__restrict__ cls, const double *__restrict__ mass, double unitE,
-                             unsigned long long *Esnap, float *Tst) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    const unsigned long long e = Eabs[c];
-    Esnap[c] = e;
-    Tst[c] = e ? (float)dev_determineT(KpCls + (size_t)cls[c] * (TMAXD + 1), ((double)e * unitE) / mass[c], 1.0)
-               : 0.f;
-}
-
 __global__ void k_fill_u4(size_t n, uint4 *p) {
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
         p[k] = make_uint4(0u, 0u, 0u, 0u);
@@ -1659,7 +1794,7 @@ __global__ void k_clock(unsigned long long *p) { *p = wall_clock64(); }
 
 // raw tallies from the fixed-point sums; Tbw = the cell's Bjorkman-Wood T
 // after the pass, T(Eabs/m) (the serial loop's last increaseT)
-__global__ void k_finalize(int ncell, const unsigned long long *EJvq, const unsigned long long *EJv2q,
+__global__ void k_finalize(int ncell, int nb, const unsigned long long *EJvq, const unsigned long long *EJv2q,
                            const unsigned long long *Eabsq, const double *invU1, const double *invU2, double unitE,
                            const double *__restrict__ KpCls, const int *__restrict__ cls,
                            const double *__restrict__ mass, double *EJv, double *EJv2, double *Eabs, double *Tbw) {
@@ -1669,7 +1804,9 @@ __global__ void k_finalize(int ncell, const unsigned long long *EJvq, const unsi
     const unsigned long long *L = EJv2q + 3 * (size_t)c;
     double s = ((double)L[2] * 0x1p64 + (double)L[1] * 0x1p32) + (double)L[0];
     EJv2[c] = s / invU2[c];
-    const unsigned long long e = Eabsq[c];
+    // the pass's absorbed energy: every epoch's deposits
+    unsigned long long e = 0ull;
+    for (int b = 0; b < nb; b++) e += Eabsq[(size_t)b * ncell + c];
     Eabs[c] = (double)e * unitE;
     Tbw[c] = e ? dev_determineT(KpCls + (size_t)cls[c] * (TMAXD + 1), ((double)e * unitE) / mass[c], 1.0) : 0.0;
 }
@@ -2056,7 +2193,7 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
     double *invU1, *invU2;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ty.EJv, ncell))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ty.EJv2, (size_t)3 * ncell))) return r;
-    if ((r = dalloc(ctx, ctx->mod_alloc, &ty.Eabs, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ty.Eabs, (size_t)(KSTAGE_MAX + 2) * ncell))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &invU1, ncell))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &invU2, ncell))) return r;
     ty.invU1 = invU1;
@@ -2068,8 +2205,10 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
     if ((r = dalloc(ctx, ctx->mod_alloc, &ty.tautot, 1))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ty.torep, 16))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Ni, ncell))) return r;
-    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Esnap, ncell))) return r;
-    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Tst, ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Sk, (size_t)(KSTAGE_MAX + 1) * ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Tk, (size_t)(KSTAGE_MAX + 1) * ncell))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_gate, G_GATE_SLOTS))) return r;
+    if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_stages, KSTAGE_MAX + 1))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_EJv, ncell))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_EJv2, ncell))) return r;
     if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_Eabs, ncell))) return r;
@@ -2091,6 +2230,45 @@ static long long stage_next(long long P, long long N, long long stage0, int rati
     if (r > 16.0) r = 16.0;
     const double nx = ceil((double)P * r);
     return (nx >= (double)N) ? N : (long long)nx;
+}
+
+// The stage table of a concurrent pass of N packets: P[k] the first packet
+// of stage k, P[nstage] = N; one stage for a one-lane pass.  Sets nstage and
+// depB in A; false when the schedule has more than KSTAGE_MAX stages.
+struct StageTab {
+    long long P[KSTAGE_MAX + 1];
+};
+
+static bool stage_table(const mcx_run *run, bool serial, TransportArgs &A, StageTab &T) {
+    const long long N = run->nphot;
+    const long long s0 = run->stage0 > 0 ? run->stage0 : 64;
+    const int ratio = run->stage_ratio == 0 ? 2 : run->stage_ratio;
+    for (int k = 0; k <= KSTAGE_MAX; k++) T.P[k] = N;
+    T.P[0] = 0;
+    int ns = 0;
+    if (serial) {
+        ns = 1;
+    } else {
+        long long a = 0, b = std::min(N, s0);
+        while (a < N) {
+            if (ns >= KSTAGE_MAX) return false;
+            T.P[ns++] = a;
+            a = b;
+            b = stage_next(b, N, s0, ratio);
+        }
+    }
+    T.P[ns] = N;
+    A.nstage = ns;
+    // deposits per epoch step (0: the default; negative: every deposit of a
+    // stage's packets counts for the next stage)
+    A.depB = (run->epoch_deposits < 0) ? 0u : (run->epoch_deposits == 0) ? 16000u : (unsigned)run->epoch_deposits;
+    return true;
+}
+
+// the stage table into device memory (a kernel, so that a graph captures it)
+__global__ void k_set_stages(StageTab t, long long *out) {
+    const int k = threadIdx.x;
+    if (k <= KSTAGE_MAX) out[k] = t.P[k];
 }
 
 // ---- the pass's per-model kernels, reading model m's buffers from its
@@ -2135,27 +2313,19 @@ __global__ void k_bm_prep(const BatchModel *__restrict__ bm, int m, double factR
             ty.ispec[k] = 0u;
         }
         if (k < G_NSLOTS) ty.counters[k] = 0ull;
-        if (k == 0) *ty.tautot = 0.0;
+        if (k < G_GATE_SLOTS) ty.gate[k] = 0ull;
+        if (k == 0) {
+            *ty.tautot = 0.0;
+            *B.next_pid = 0ull;
+        }
     }
+    // the other epochs' absorbed energy (epoch 0 above)
+    for (size_t k = (size_t)ncell + (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)(KSTAGE_MAX + 2) * ncell;
+         k += (size_t)gridDim.x * blockDim.x)
+        ty.Eabs[k] = 0ull;
 }
 
-__global__ void k_bm_stage(const BatchModel *__restrict__ bm, int m, unsigned long long Pa) {
-    const BatchModel &B = bm[m];
-    const DevModel &md = B.md;
-    const DevTally &ty = B.ty;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < md.ncell) {
-        const unsigned long long e = ty.Eabs[c];
-        ((unsigned long long *)ty.Esnap)[c] = e;
-        ((float *)ty.Tst)[c] =
-            e ? (float)dev_determineT(md.KpCls + (size_t)md.cls[c] * (TMAXD + 1), ((double)e * B.unitE) / md.mass[c], 1.0)
-              : 0.f;
-    }
-    if (c == 0) *B.next_pid = Pa;
-    if (c < 7) ty.counters[G_DONE + c] = 0ull;
-}
-
-__global__ void k_bm_finalize(const BatchModel *__restrict__ bm, int m) {
+__global__ void k_bm_finalize(const BatchModel *__restrict__ bm, int m, int nb) {
     const BatchModel &B = bm[m];
     const DevModel &md = B.md;
     const DevTally &ty = B.ty;
@@ -2165,7 +2335,8 @@ __global__ void k_bm_finalize(const BatchModel *__restrict__ bm, int m) {
     const unsigned long long *L = ty.EJv2 + 3 * (size_t)c;
     const double s = ((double)L[2] * 0x1p64 + (double)L[1] * 0x1p32) + (double)L[0];
     ty.oEJv2[c] = s / ty.invU2[c];
-    const unsigned long long e = ty.Eabs[c];
+    unsigned long long e = 0ull;
+    for (int b = 0; b < nb; b++) e += ty.Eabs[(size_t)b * md.ncell + c];
     ty.oEabs[c] = (double)e * B.unitE;
     ty.oTbw[c] = e ? dev_determineT(md.KpCls + (size_t)md.cls[c] * (TMAXD + 1), ((double)e * B.unitE) / md.mass[c], 1.0)
                    : 0.0;
@@ -2191,8 +2362,9 @@ static int host_prep(mcx_ctx *ctx, const mcx_run *run, uint64_t seed, BatchModel
     ctx->unitE = ldexp(E0, -EABS_BITS);
     DevTally &ty = ctx->ty;
     ty.Ni = ctx->d_Ni;
-    ty.Esnap = ctx->d_Esnap;
-    ty.Tst = ctx->d_Tst;
+    ty.Sk = ctx->d_Sk;
+    ty.Tk = ctx->d_Tk;
+    ty.gate = ctx->d_gate;
     ty.oEJv = ctx->d_EJv;
     ty.oEJv2 = ctx->d_EJv2;
     ty.oEabs = ctx->d_Eabs;
@@ -2224,7 +2396,8 @@ static int enqueue_prep(mcx_ctx *ctx, const BatchModel &hb, double factRW, hipSt
     }
     hipLaunchKernelGGL(k_fill_u64, dim3(64), b1, 0, s, (size_t)ncell, ty.EJv, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(64), b1, 0, s, (size_t)3 * ncell, ty.EJv2, 0ull);
-    hipLaunchKernelGGL(k_fill_u64, dim3(64), b1, 0, s, (size_t)ncell, ty.Eabs, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(256), b1, 0, s, (size_t)(KSTAGE_MAX + 2) * ncell, ty.Eabs, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), b1, 0, s, (size_t)G_GATE_SLOTS, ctx->d_gate, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(64), b1, 0, s, (size_t)ncell, ty.Tcell, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(64), b1, 0, s, (size_t)2 * (NANGD + 1) * nlam, ty.specE, 0ull);
     hipLaunchKernelGGL(k_fill_u32, dim3(64), b1, 0, s, (size_t)2 * (NANGD + 1) * nlam, ty.ispec, 0u);
@@ -2285,26 +2458,19 @@ static void enqueue_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, const
     A.serial = 0;
     A.coop = (run->coop_lanes < 0) ? COOP_LANES : run->coop_lanes;
     A.rb_mask = L.rb - 1;
-    const long long N = run->nphot;
-    const long long s0 = run->stage0 > 0 ? run->stage0 : 64;
-    const int ratio = run->stage_ratio == 0 ? 2 : run->stage_ratio;
-    long long Pa = 0, Pb = std::min(N, s0);
-    while (Pa < N) {
-        for (int m = 0; m < nm; m++)
-            hipLaunchKernelGGL(k_bm_stage, dim3((ctxs[m]->md.ncell + 255) / 256), dim3(256), 0, s, dbm, m,
-                               (unsigned long long)Pa);
-        A.Pk = Pa;
-        A.pid_base = Pa;
-        A.pid_end = Pb;
-        const int blocks = (nm == 1) ? (int)std::max(1ll, std::min((long long)L.bpm, (Pb - Pa + L.threads - 1) / L.threads))
-                                     : L.bpm * nm;
-        if (nm == 1) A.bpm = blocks;
-        hipLaunchKernelGGL(L.kern, dim3(blocks), dim3(L.threads), L.lds, s, A);
-        Pa = Pb;
-        Pb = stage_next(Pb, N, s0, ratio);
-    }
+    // every stage in one launch (the stages open inside it)
+    StageTab T;
+    (void)stage_table(run, false, A, T);
+    hipLaunchKernelGGL(k_set_stages, dim3(1), dim3(64), 0, s, T, ctx->d_stages);
+    A.P = ctx->d_stages;
+    A.pid_base = 0;
+    A.pid_end = run->nphot;
+    const int blocks = (nm == 1) ? (int)std::max(1ll, std::min((long long)L.bpm, ((long long)run->nphot + L.threads - 1) / L.threads))
+                                 : L.bpm * nm;
+    if (nm == 1) A.bpm = blocks;
+    hipLaunchKernelGGL(L.kern, dim3(blocks), dim3(L.threads), L.lds, s, A);
     for (int m = 0; m < nm; m++)
-        hipLaunchKernelGGL(k_bm_finalize, dim3((ctxs[m]->md.ncell + 255) / 256), dim3(256), 0, s, dbm, m);
+        hipLaunchKernelGGL(k_bm_finalize, dim3((ctxs[m]->md.ncell + 255) / 256), dim3(256), 0, s, dbm, m, A.nstage + 1);
     (void)hb;
 }
 
@@ -2380,6 +2546,13 @@ static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_re
         return fail(ctx, MCX_EINVAL, "stage0 >= 0 and stage_ratio -1, 0 or >= 2 required");
     const bool serial = (run->nlanes == 1);
     if (serial && nm > 1) return fail(ctx, MCX_EINVAL, "a one-lane pass runs one model");
+    {
+        TransportArgs T;
+        StageTab P;
+        if (!stage_table(run, serial, T, P))
+            return fail(ctx, MCX_EINVAL, "the stage schedule has more than " + std::to_string(KSTAGE_MAX) +
+                                             " stages (raise stage0 or stage_ratio)");
+    }
     HIPCHK(ctx, hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     // one kernel for all models: the same species case
@@ -2484,60 +2657,52 @@ static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_re
     A.serial = serial ? 1 : 0;
     A.coop = (run->coop_lanes < 0) ? COOP_LANES : run->coop_lanes;
     A.rb_mask = rb - 1;
-    // stages: [0, stage0), then each ends at ratio x its start; one launch
-    // for the whole pass when one lane runs it
-    const long long N = run->nphot;
-    const long long s0 = run->stage0 > 0 ? run->stage0 : 64;
-    const int ratio = run->stage_ratio == 0 ? 2 : run->stage_ratio;
+    // one launch for the whole pass: the stages open inside it (one stage
+    // when one lane runs it)
+    StageTab T;
+    (void)stage_table(run, serial, A, T);
+    hipLaunchKernelGGL(k_set_stages, dim3(1), dim3(64), 0, s, T, ctx->d_stages);
+    A.P = ctx->d_stages;
+    for (int m = 0; m < nm; m++) {
+        mcx_ctx *c = ctxs[m];
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, c->d_next, 0ull);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (size_t)7, c->ty.counters + G_DONE, 0ull);
+    }
+    A.pid_base = 0;
+    A.pid_end = run->nphot;
+    const int blocks = serial ? 1
+                       : (nm == 1) ? (int)std::max(1ll, std::min((long long)bpm, ((long long)run->nphot + threads - 1) / threads))
+                                   : bpm * nm;
+    if (nm == 1) A.bpm = blocks;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, s, A);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->ev[2], s));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[2]));
     float kms_total = 0.f;
-    // MCX_STAGE_LOG set: one line per stage on stderr (its packets and time)
+    HIPCHK(ctx, hipEventElapsedTime(&kms_total, ctx->ev[1], ctx->ev[2]));
+    // MCX_STAGE_LOG set: when each stage of each model opened (ms after the
+    // launch's start)
     static const bool stage_log = getenv("MCX_STAGE_LOG") != nullptr;
-    long long Pa = 0, Pb = serial ? N : std::min(N, s0);
-    while (Pa < N) {
+    if (stage_log && !serial && ctx->wclk_khz > 0) {
         for (int m = 0; m < nm; m++) {
-            mcx_ctx *c = ctxs[m];
-            const DevModel &md = c->md;
-            if (!serial) {
-                hipLaunchKernelGGL(k_stage_snap, dim3((md.ncell + 255) / 256), dim3(256), 0, s, md.ncell, c->ty.Eabs,
-                                   md.KpCls, md.cls, md.mass, c->unitE, c->d_Esnap, c->d_Tst);
-                HIPCHK(ctx, hipGetLastError());
-            }
-            hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, c->d_next, (unsigned long long)Pa);
-            hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (size_t)7, c->ty.counters + G_DONE, 0ull);
+            std::vector<unsigned long long> g(G_GATE_SLOTS), cn(G_NSLOTS);
+            HIPCHK(ctx, hipMemcpy(g.data(), ctxs[m]->d_gate, sizeof(unsigned long long) * G_GATE_SLOTS,
+                                  hipMemcpyDeviceToHost));
+            HIPCHK(ctx, hipMemcpy(cn.data(), ctxs[m]->ty.counters, sizeof(unsigned long long) * G_NSLOTS,
+                                  hipMemcpyDeviceToHost));
+            fprintf(stderr, "[mcx] model %d: %d stages, B %u; stage opened at (ms):", m, A.nstage, A.depB);
+            for (int k = 1; k < A.nstage; k++)
+                fprintf(stderr, " %.1f", (double)(g[G_TOPEN + k] - cn[G_TLINE]) / (double)ctx->wclk_khz);
+            fprintf(stderr, "\n");
         }
-        A.Pk = serial ? 0 : Pa;
-        A.pid_base = Pa;
-        A.pid_end = Pb;
-        // a lone model gets no more blocks than its stage has packets for
-        const int blocks = serial ? 1
-                           : (nm == 1) ? (int)std::max(1ll, std::min((long long)bpm, (Pb - Pa + threads - 1) / threads))
-                                       : bpm * nm;
-        if (nm == 1) A.bpm = blocks;
-        HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, s, A);
-        HIPCHK(ctx, hipGetLastError());
-        HIPCHK(ctx, hipEventRecord(ctx->ev[2], s));
-        HIPCHK(ctx, hipEventSynchronize(ctx->ev[2]));
-        float ms = 0.f;
-        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
-        kms_total += ms;
-        if (stage_log) fprintf(stderr, "[mcx] stage [%lld, %lld) of %d model(s): %.1f ms\n", Pa, Pb, nm, ms);
-        if (A.deadline) {
-            unsigned long long nto = 0, t;
-            for (int m = 0; m < nm; m++) {
-                HIPCHK(ctx, hipMemcpy(&t, ctxs[m]->ty.counters + C_NTIMEOUT, sizeof(t), hipMemcpyDeviceToHost));
-                nto += t;
-            }
-            if (nto) break;
-        }
-        Pa = Pb;
-        Pb = stage_next(Pb, N, s0, ratio);
     }
     for (int m = 0; m < nm; m++) {
         mcx_ctx *c = ctxs[m];
         const DevModel &md = c->md;
         const int ncell = md.ncell;
-        hipLaunchKernelGGL(k_finalize, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell, c->ty.EJv, c->ty.EJv2,
+        hipLaunchKernelGGL(k_finalize, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell, A.nstage + 1, c->ty.EJv,
+                           c->ty.EJv2,
                            c->ty.Eabs, c->ty.invU1, c->ty.invU2, c->unitE, md.KpCls, md.cls, md.mass, c->d_EJv,
                            c->d_EJv2, c->d_Eabs, c->d_Tbw);
         HIPCHK(ctx, hipGetLastError());

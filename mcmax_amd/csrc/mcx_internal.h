@@ -45,6 +45,21 @@ enum { C_NEMIT = 0, C_NINTERACT, C_NABS, C_NSCAT, C_NESCAPE, C_NSTARHIT, C_NSTEP
 #define G_TOREP 23 // set by the first packet abandoned at the time limit
 #define G_NSLOTS 24
 
+// Stages of a concurrent pass (at most KSTAGE_MAX) and the readiness counters
+// of their gates: gate[G_FIN + k] packets that finished with their last
+// deposit in epoch k, gate[G_PASS + k] packets of earlier stages that went
+// past the deposits stage k waits for, gate[G_CHUNK + k] / gate[G_CHDONE + k]
+// chunks of stage k's snapshot claimed / built, gate[G_OPEN] the highest
+// stage whose snapshot is complete.
+#define KSTAGE_MAX 40
+#define G_FIN 0
+#define G_PASS (KSTAGE_MAX + 2)
+#define G_CHUNK (2 * (KSTAGE_MAX + 2))
+#define G_CHDONE (3 * (KSTAGE_MAX + 2))
+#define G_OPEN (4 * (KSTAGE_MAX + 2))
+#define G_TOPEN (4 * (KSTAGE_MAX + 2) + 1) // wall clock when stage k opened
+#define G_GATE_SLOTS (5 * (KSTAGE_MAX + 2) + 1)
+
 #define NY_RW 1000 // survival-table length (RandomWalk.f:6-7)
 
 struct DevTables {
@@ -81,13 +96,15 @@ struct DevModel {
 struct DevTally {
     unsigned long long *EJv;    // [ncell]
     unsigned long long *EJv2;   // [ncell*3] limbs of weight 2^0, 2^32, 2^64
-    unsigned long long *Eabs;   // [ncell]
+    unsigned long long *Eabs;   // [KSTAGE_MAX+2][ncell] absorbed energy by deposit epoch (one-lane
+                                // passes: epoch 0 only); the pass's Eabs is the sum
     const double *invU1, *invU2;
     unsigned long long *Tcell;  // [ncell] bits of the latest BW T (monotone; one-lane passes)
     int *Ni;                    // [ncell] distinct packets per cell
     uint4 *vtab;                // [lanes << VIS_LOG2] visited cells of each lane's packet
-    const unsigned long long *Esnap; // [ncell] Eabs of the packets before the stage
-    const float *Tst;           // [ncell] T(Esnap/m) at the stage's start
+    unsigned long long *Sk;     // [KSTAGE_MAX+1][ncell] snapshot S_k of stage k (epochs 1..k)
+    float *Tk;                  // [KSTAGE_MAX+1][ncell] T(S_k/m): start of stage k's T searches
+    unsigned long long *gate;   // [G_GATE_SLOTS] the stages' readiness counters
     unsigned long long *specE;  // [2][(NANG+1)*nlam]
     unsigned *ispec;            // [2][(NANG+1)*nlam]
     unsigned long long *counters;
@@ -144,8 +161,9 @@ struct mcx_ctx {
     // run
     DevTally ty{};
     size_t vtab_lanes = 0;      // lanes the visit table holds
-    unsigned long long *d_Esnap = nullptr;
-    float *d_Tst = nullptr;
+    unsigned long long *d_Sk = nullptr, *d_gate = nullptr;
+    long long *d_stages = nullptr;  // the stage table of the last pass run from this context
+    float *d_Tk = nullptr;
     int *d_Ni = nullptr;
     double *d_EJv = nullptr, *d_EJv2 = nullptr, *d_Eabs = nullptr, *d_Tbw = nullptr;
     double *d_T = nullptr, *d_dT = nullptr, *d_dEJv = nullptr;

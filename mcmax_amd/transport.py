@@ -73,11 +73,11 @@ class Transport:
         self.model = m
 
     def transport(self, nphot, seed=1, nlanes=0, maxinteract=0, randomwalk=None, factRW=None, coop_lanes=-1,
-                  stage0=0, stage_ratio=0, precision=0, max_ms=0.0):
+                  stage0=0, stage_ratio=0, precision=0, max_ms=0.0, epoch_deposits=0):
         """One MC pass (mcx_transport); tallies stay on the device.  The
         random-walk switches default to the model's keywords.  nlanes=1 runs
         the serial loop on one lane; otherwise the staged schedule (stage0,
-        stage_ratio; 0 picks 64 and 2).  precision 1: FP32 direction
+        stage_ratio, epoch_deposits; 0 picks 64, 2 and 16000).  precision 1: FP32 direction
         arithmetic (MCX_FP32).  max_ms > 0: the pass's time limit (MCX_ETIME
         raises)."""
         m = self.model
@@ -85,7 +85,8 @@ class Transport:
         f = float(getattr(m, "factRW", 10.0)) if factRW is None else float(factRW)
         r = _lib.Run(nphot=int(nphot), seed=int(seed), maxinteract=int(maxinteract), nlanes=int(nlanes),
                      stage0=int(stage0), randomwalk=int(rw), factRW=f, coop_lanes=int(coop_lanes),
-                     stage_ratio=int(stage_ratio), precision=int(precision), max_ms=float(max_ms))
+                     stage_ratio=int(stage_ratio), precision=int(precision), max_ms=float(max_ms),
+                     epoch_deposits=int(epoch_deposits))
         rc = self.lib.mcx_transport(self.ctx, C.byref(r))
         if rc not in (0, _lib.MCX_EGEOM):
             self._check(rc)
@@ -283,7 +284,7 @@ class Transport:
 
 
 def batch_transport(trs, nphot, seed=1, seed_stride=1, blocks_per_model=0, randomwalk=None, factRW=None,
-                    maxinteract=0, precision=0, max_ms=0.0, stage0=0, stage_ratio=0, graph=False):
+                    maxinteract=0, precision=0, max_ms=0.0, stage0=0, stage_ratio=0, graph=False, epoch_deposits=0):
     """One pass of nphot packets for every model of the Transport objects
     trs (one model each) in common launches (mcx_batch_transport); model m
     uses the key seed + m seed_stride.  graph=True: captured once, then
@@ -295,7 +296,7 @@ def batch_transport(trs, nphot, seed=1, seed_stride=1, blocks_per_model=0, rando
     f = float(getattr(m0, "factRW", 10.0)) if factRW is None else float(factRW)
     r = _lib.Run(nphot=int(nphot), seed=int(seed), maxinteract=int(maxinteract), nlanes=0, stage0=int(stage0),
                  randomwalk=int(rw), factRW=f, coop_lanes=-1, stage_ratio=int(stage_ratio), precision=int(precision),
-                 max_ms=float(max_ms))
+                 max_ms=float(max_ms), epoch_deposits=int(epoch_deposits))
     arr = (C.c_void_p * len(trs))(*[t.ctx for t in trs])
     st = np.zeros(len(trs), dtype=np.int32)
     rc = lib.mcx_batch_transport(arr, len(trs), C.byref(r), int(blocks_per_model), int(seed_stride), int(bool(graph)),
@@ -316,11 +317,12 @@ def make_obs(angle_deg=45.0, nphi=45, nj0=2, nphot=1000, nphotstar=1000, nexits=
 
 
 def mc_radiation(tr, nphot, seed=1, nlanes=0, maxinteract=0, coop_lanes=-1, stage0=0, stage_ratio=0, precision=0,
-                 max_ms=0.0):
+                 max_ms=0.0, epoch_deposits=0):
     """MCRadiation + temperature inversion; returns a dict with the tallies
     and T, dT, dEJv."""
     rc = tr.transport(nphot, seed=seed, nlanes=nlanes, maxinteract=maxinteract, coop_lanes=coop_lanes,
-                      stage0=stage0, stage_ratio=stage_ratio, precision=precision, max_ms=max_ms)
+                      stage0=stage0, stage_ratio=stage_ratio, precision=precision, max_ms=max_ms,
+                      epoch_deposits=epoch_deposits)
     out = tr.tallies()
     out["T"], out["dT"], out["dEJv"] = tr.temperature()
     out["rc"] = rc

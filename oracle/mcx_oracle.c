@@ -384,6 +384,11 @@ typedef struct {
     int64_t *own_pkt;
     double fS, unitE, invE;
     int64_t overflow;
+    /* deposits by epoch: Eb[e][c]; the current packet's stage and deposit
+       count; B (0: every deposit in epoch stage+1) and the stage count */
+    uint64_t *Eb;
+    int jstage, nstage;
+    uint32_t ndep, depB;
 } ctx_t;
 
 /* Radiation.f:1007-1024 */
@@ -958,6 +963,19 @@ static double staged_T(const ctx_t *C, int c, int64_t pkt)
 
 /* deposit E in cell c: the device's fixed-point add (2^24 units per E0),
  * then T before and after the deposit as packet pkt sees them */
+/* the epoch of deposit d (0-based) of a packet of stage j: j + m, m the
+ * smallest m >= 1 with d < B (2^m - 1) (B = 0: m = 1), at most cap */
+static int dep_epoch(int j, uint32_t d, uint32_t B, int cap)
+{
+    int m = 1;
+    if (B) {
+        uint32_t q = d / B + 1u;
+        m = 0;
+        while (q) { m++; q >>= 1; }
+    }
+    return (j + m < cap) ? j + m : cap;
+}
+
 static void staged_deposit(ctx_t *C, int c, int64_t pkt, double E, double *T0, double *T1)
 {
     double qd = rint(E * C->invE);
@@ -967,6 +985,9 @@ static void staged_deposit(ctx_t *C, int c, int64_t pkt, double E, double *T0, d
     }
     uint64_t q = (uint64_t)qd;
     uint64_t o = own_of(C, c, pkt);
+    const int ncell = C->nr * C->nt;
+    C->Eb[(size_t)dep_epoch(C->jstage, C->ndep, C->depB, C->nstage) * ncell + c] += q;
+    C->ndep++;
     C->Eq[c] += q;
     C->own_pkt[c] = pkt;
     C->own[c] = o + q;
@@ -1442,23 +1463,40 @@ int orc_mcradiation(const orc_model *m, const orc_run *run, orc_ran1_state *rs, 
         C.unitE = ldexp(E0, -24);
         const int64_t s0 = run->stage0 > 0 ? run->stage0 : 64;
         const int ratio = run->stage_ratio == 0 ? 2 : run->stage_ratio; /* -1: the growing ratio */
+        /* the number of stages, then the deposits' epochs (B per epoch step:
+           0 picks 1000, negative: every deposit in the next stage's epoch) */
+        int nstage = 0;
+        for (int64_t a = 0, b = s0 < Nphot ? s0 : Nphot; a < Nphot; a = b, b = orc_stage_next(b, Nphot, s0, ratio))
+            nstage++;
+        C.nstage = nstage;
+        C.depB = (run->epoch_deposits < 0) ? 0u : (run->epoch_deposits == 0) ? 16000u : (uint32_t)run->epoch_deposits;
+        C.Eb = (uint64_t *)calloc((size_t)(nstage + 1) * ncell, sizeof(uint64_t));
         int64_t Pa = 0, Pb = s0 < Nphot ? s0 : Nphot;
+        int js = 0;
         while (Pa < Nphot && !err) {
-            memcpy(C.Sq, C.Eq, sizeof(uint64_t) * (size_t)ncell);
+            /* S_js: the deposits of epochs 1..js */
+            for (int c = 0; c < ncell; c++) {
+                uint64_t S = 0;
+                for (int e = 1; e <= js; e++) S += C.Eb[(size_t)e * ncell + c];
+                C.Sq[c] = S;
+            }
+            C.jstage = js;
             for (int64_t n = 0; n < Pb - Pa && !err; n++) {
                 /* staged == 2: the stage's packets in reverse order */
                 const int64_t ip = (run->staged == 2) ? Pb - n : Pa + 1 + n;
                 C.fS = (Pa > 0) ? (double)(ip - 1) / (double)Pa : 0.0;
+                C.ndep = 0;
                 err = run_packet(&C, run, &p, ip, wangle, wlam);
             }
             Pa = Pb;
             Pb = orc_stage_next(Pb, Nphot, s0, ratio);
+            js++;
         }
         for (int c = 0; c < ncell; c++) {
             t->Eabs[c] = (double)C.Eq[c] * C.unitE;
             t->T[c] = (C.Eq[c] != 0) ? T_of(&C, c, t->Eabs[c] / C.mass[c]) : 0.0;
         }
-        free(C.Eq); free(C.Sq); free(C.own); free(C.own_pkt);
+        free(C.Eq); free(C.Sq); free(C.own); free(C.own_pkt); free(C.Eb);
     } else {
         for (int64_t ip = 1; ip <= Nphot && !err; ip++) err = run_packet(&C, run, &p, ip, wangle, wlam);
     }

@@ -91,6 +91,9 @@ typedef struct {
                                each stage's packets in reverse order     */
     int64_t stage0;         /* packets of the first stage                 */
     int stage_ratio;        /* each later stage ends at ratio x its start */
+    int64_t epoch_deposits; /* staged: deposits per epoch step B (0: 16000;
+                               negative: none, every deposit of a stage's
+                               packets counts for the next stage)         */
 } orc_run;
 
 typedef struct {
@@ -153,7 +156,13 @@ void orc_phi_tables(double *cos2phi, double *sin2phi);
  * realisation: packet p of a stage starting at P_k sees, in cell c,
  *     E_b = (S_k(c) p / P_k + own_p(c)) unitE
  * where S_k(c) is the fixed-point Eabs of packets 0..P_k-1 (0 in the first
- * stage) and own_p(c) packet p's own earlier deposits in c.  Bjorkman-Wood
+ * stage) and own_p(c) packet p's own earlier deposits in c.  Precisely,
+ * S_k sums the deposits of epochs 1..k: a packet of stage j files its d-th
+ * deposit (0-based) under epoch j + m, m the smallest m >= 1 with
+ * d < B (2^m - 1) (run->epoch_deposits = B; without epochs m = 1, and S_k
+ * is every deposit of packets 0..P_k-1).  On the device the stages then
+ * overlap: stage k starts once every earlier packet has finished or gone
+ * past the deposits S_k counts.  Bjorkman-Wood
  * reemission then uses T0 = T(E_b/m), T1 = T((E_b + e)/m), the random-walk
  * test T(E_b/m).  Eabs is kept in the device's fixed point (2^24 units per
  * nominal packet energy E0 = L/N).  Each history then depends only on its

@@ -31,7 +31,8 @@ class OrcModel(C.Structure):
 
 class OrcRun(C.Structure):
     _fields_ = [("nphot", C.c_int64), ("rng", C.c_int), ("seed", C.c_uint64), ("maxinteract", C.c_int),
-                ("inflight", C.c_int64), ("staged", C.c_int), ("stage0", C.c_int64), ("stage_ratio", C.c_int)]
+                ("inflight", C.c_int64), ("staged", C.c_int), ("stage0", C.c_int64), ("stage_ratio", C.c_int),
+                ("epoch_deposits", C.c_int64)]
 
 
 class OrcTally(C.Structure):
@@ -372,7 +373,7 @@ class Oracle:
         return self.lib.orc_min_dist(C.byref(self.om), x, y, z, int(i), int(jj))
 
     def mcradiation(self, nphot, rng="philox", seed=1, maxinteract=0, per_packet=False, inflight=0, staged=False,
-                    stage0=64, stage_ratio=2):
+                    stage0=64, stage_ratio=2, epoch_deposits=0):
         """One pass: the serial loop, or with staged=True the staged
         Bjorkman-Wood schedule of the device's concurrent pass (orc_run in
         mcx_oracle.h)."""
@@ -390,7 +391,7 @@ class Oracle:
             t.pkt_steps = out["pkt_steps"].ctypes.data_as(C.POINTER(C.c_int64))
         run = OrcRun(nphot=int(nphot), rng=0 if rng == "ran1" else 1, seed=int(seed), maxinteract=int(maxinteract),
                      inflight=int(inflight), staged=int(staged), stage0=int(stage0),
-                     stage_ratio=int(stage_ratio))
+                     stage_ratio=int(stage_ratio), epoch_deposits=int(epoch_deposits))
         rc = self.lib.orc_mcradiation(C.byref(self.om), C.byref(run), C.byref(self.ran1), C.byref(t))
         if rc != 0:
             raise RuntimeError(f"oracle transport failed: {rc}")

@@ -305,6 +305,15 @@ def test_staged_parity_stage_schedule(gpu, tiny_thick_model, oracle_lib):
     _serial_compare(gpu, tiny_thick_model, oracle_lib, 2000, seed=43, staged=True, stage0=100, stage_ratio=3)
 
 
+@pytest.mark.parametrize("epoch_deposits", [3, 40, -1])
+def test_staged_parity_epochs(gpu, tiny_thick_model, oracle_lib, epoch_deposits):
+    """The deposit epochs of the staged schedule (a packet's deposits reach
+    the next stages' snapshots after B, 3B, 7B, ... of them; -1: none, each
+    stage waits for every earlier packet): small B makes every long history
+    cross epochs, and the device still follows the oracle packet by packet."""
+    _serial_compare(gpu, tiny_thick_model, oracle_lib, 3000, seed=53, staged=True, epoch_deposits=epoch_deposits)
+
+
 def test_staged_reproducible(gpu, small_rw_model):
     """Two concurrent passes with one seed give bit-identical tallies, also
     with a different number of packets in flight."""
