@@ -18,6 +18,32 @@ HIPCC_FLAGS = [
 ]
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "mcx_driver")
+
+
+def build_fortran(verbose=False):
+    """The Fortran caller (mcx_binding.f90 + mcx_driver.f90) with amdflang,
+    linked against the in-tree libmcx.so; None when amdflang is absent."""
+    fc = os.environ.get("FC", "/opt/rocm/llvm/bin/amdflang")
+    if not os.path.exists(fc):
+        return None
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        obj = os.path.join(tmp, "mcx_binding.o")
+        cmds = [[fc, "-O2", "-c", os.path.join(FORTRAN, "mcx_binding.f90"), "-o", obj, "-module-dir", tmp],
+                [fc, "-O2", os.path.join(FORTRAN, "mcx_driver.f90"), obj, "-I", tmp, "-o", DRIVER + ".tmp",
+                 "-L" + HERE, "-lmcx", "-Wl,-rpath,$ORIGIN/.."]]
+        for cmd in cmds:
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError("amdflang failed:\n" + r.stdout + r.stderr)
+            if verbose and (r.stdout + r.stderr).strip():
+                print(r.stdout + r.stderr, file=sys.stderr)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 def build_lib(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     cmd = [hipcc] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", OUT + ".tmp"] + SRCS + LIBS
@@ -34,3 +60,4 @@ def build_lib(verbose=False):
 
 if __name__ == "__main__":
     print(build_lib(verbose=True))
+    print(build_fortran(verbose=True))

@@ -27,3 +27,19 @@ def test_library_exports_every_declared_symbol():
     for n in declared_functions():
         assert hasattr(lib, n), n
     assert set(declared_functions()) == set(_lib.EXPORTS)
+
+
+def test_fortran_binding_layouts():
+    """The bind(C) types of mcmax_amd/fortran/mcx_binding.f90 have the sizes
+    of the C structures (the driver prints them without a GPU call)."""
+    import ctypes as C
+    import subprocess
+    import pytest
+    from mcmax_amd import _lib, build
+    if not os.path.exists(build.DRIVER):
+        pytest.skip("the Fortran caller is not built (no amdflang)")
+    out = subprocess.run([build.DRIVER, "--sizes"], capture_output=True, text=True, timeout=60).stdout.split()
+    assert out[0] == "sizes"
+    sizes = [int(x) for x in out[1:6]]
+    assert sizes == [C.sizeof(_lib.Tables), C.sizeof(_lib.Grid), C.sizeof(_lib.Run), C.sizeof(_lib.Tallies),
+                     C.sizeof(_lib.Temps)]
