@@ -40,11 +40,10 @@ def algorithmic_bytes(t, ng, nlam):
 
 
 def measured_traffic(config, nphot, ng, schedule):
-    """HBM bytes per k_transport launch (one stage) for this workload, from
-    the rocprofv3 FETCH_SIZE/WRITE_SIZE runs summarised in profiles/ (the
-    pass's dispatches summed, scaled by a plain copy of known size, divided
-    by the pass's launches); None when no measurement of this workload and
-    stage schedule exists."""
+    """HBM bytes of the pass's k_transport launch for this workload, from
+    the rocprofv3 FETCH_SIZE/WRITE_SIZE runs summarised in profiles/ (scaled
+    by what the counters read for a plain copy of known size); None when no
+    measurement of this workload and stage schedule exists."""
     path = os.path.join(ROOT, "profiles", "k_transport_traffic.json")
     if not os.path.exists(path):
         return None
@@ -578,10 +577,13 @@ def main():
     value = a.nphot * steps / wall
     kern_s = sum(kms) / len(kms) * 1e-3
     nbytes, (Bx, Ba, Bs, Brw) = algorithmic_bytes(tal, m.ng, m.nlam)
-    # one launch per stage: bytes per launch / average launch duration
-    launches = len(stage_schedule(a.nphot, a.stage_ratio))
-    achieved = (nbytes / launches) / (kern_s / launches)
-    schedule = f"stage0 64, ratio {a.stage_ratio if a.stage_ratio else 2}"
+    # the whole pass is one k_transport launch (its stages open inside it)
+    launches = 1
+    achieved = nbytes / kern_s
+    nstage = len(stage_schedule(a.nphot, a.stage_ratio))
+    B = a.epoch_deposits if a.epoch_deposits else 16000
+    schedule = (f"one launch, {nstage} stages (stage0 64, ratio {a.stage_ratio if a.stage_ratio else 2}), "
+                f"epoch deposits {B if B > 0 else 'none'}")
     out = {
         "metric": METRIC, "value": value, "unit": "packets/s", "n_gpus": 1, "steps": steps,
         "warmup": warm, "steps_requested": a.steps, "warmup_requested": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",

@@ -49,16 +49,21 @@ class Sweep:
         self.log = log
         self.models, self.trs = [], []
         dust = transport.Transport(device)
+        t0 = time.perf_counter()
         try:
             for k, p in enumerate(params):
                 m = configs.load_config(base, os.path.join(workdir, f"m{k:03d}"), tr=dust, randomwalk=randomwalk, **p)
                 self.models.append(m)
+                if log and (k + 1) % 16 == 0:
+                    log(f"{k + 1} of {len(params)} models built ({time.perf_counter() - t0:.1f} s)")
         finally:
             dust.close()
-        for m in self.models:
+        for k, m in enumerate(self.models):
             t = transport.Transport(device)
             t.set_model(m)
             self.trs.append(t)
+            if log and (k + 1) % 16 == 0:
+                log(f"{k + 1} of {len(self.models)} models on the device ({time.perf_counter() - t0:.1f} s)")
 
     def close(self):
         for t in self.trs:

@@ -84,6 +84,8 @@ class Batch(sweep.Sweep):
     """Models on their own contexts whose passes run batched, with per-model
     histories and an optional warm start."""
 
+    maxinteract = 0  # > 0: the reference's interaction roulette (keyword maxinteract)
+
     def iterate(self, niter, seed=1, T_init=None, judge=0.05, graph=False, blocks_per_model=0):
         """niter Monte Carlo passes per model, each followed by the rest of
         the structure iteration (inversion, Diffuse, TempAverage,
@@ -115,10 +117,12 @@ class Batch(sweep.Sweep):
         for p in range(1, niter + 1):
             t1 = time.perf_counter()
             st = transport.batch_transport(self.trs, self.nphot, seed=seed + 1000 * p, blocks_per_model=blocks_per_model,
-                                           graph=graph)
+                                           graph=graph, maxinteract=self.maxinteract)
             if np.any(st != 0):
                 raise RuntimeError(f"batched pass {p}: status {st}")
             t_tr += time.perf_counter() - t1
+            if self.log:
+                self.log(f"pass {p}: transport of {nm} models {time.perf_counter() - t1:.1f} s")
             for k in range(nm):
                 t, m = self.trs[k], self.models[k]
                 T, _, _ = t.temperature()
@@ -150,16 +154,21 @@ class Batch(sweep.Sweep):
 
 
 def dataset(params, workdir, nphot, niter=3, seed=1, cells_per_model=None, rng_seed=0, log=None, chunk=64,
-            randomwalk=True):
+            randomwalk=True, maxinteract=0, save=None):
     """Features and log10 T targets of the plain method's temperature after
     niter structure iterations, for the models of params (run in batches of
     `chunk`); cells_per_model > 0 keeps a random subset of each model's
-    cells.  Returns (X [rows, 8], y [rows], model index [rows])."""
+    cells; maxinteract > 0 plays the reference's interaction roulette
+    (MCRadiation.f:568-571), which bounds the longest histories of the
+    thickest models.  save(X, y, k), when given, is called after every
+    chunk with the rows so far.  Returns (X [rows, 8], y [rows], model index
+    [rows])."""
     rng = np.random.default_rng(rng_seed)
     Xs, ys, ks = [], [], []
     for a in range(0, len(params), chunk):
         sub = params[a:a + chunk]
-        B = Batch(sub, f"{workdir}/d{a:05d}", nphot, randomwalk=randomwalk, log=None)
+        B = Batch(sub, f"{workdir}/d{a:05d}", nphot, randomwalk=randomwalk, log=log)
+        B.maxinteract = int(maxinteract)
         try:
             setup = [features(m) for m in B.models]
             res = B.iterate(niter, seed=seed + 7919 * a)
@@ -175,6 +184,8 @@ def dataset(params, workdir, nphot, niter=3, seed=1, cells_per_model=None, rng_s
             ks.append(np.full(len(idx), a + k, dtype=np.int32))
         if log:
             log(f"dataset: {a + len(sub)} of {len(params)} models, pass time {res['transport_s']:.1f} s")
+        if save:
+            save(np.concatenate(Xs), np.concatenate(ys), np.concatenate(ks))
     return np.concatenate(Xs), np.concatenate(ys), np.concatenate(ks)
 
 
@@ -229,7 +240,12 @@ def _main(argv=None):
     a1.add_argument("--niter", type=int, default=3)
     a1.add_argument("--cells", type=int, default=0, help="cells kept per model (0: all)")
     a1.add_argument("--chunk", type=int, default=64)
+    a1.add_argument("--maxinteract", type=int, default=0, help="the reference's interaction roulette (0: off)")
     a1.add_argument("--out", required=True)
+    a3 = sub.add_parser("forward", help="time the network's forward pass on random rows (matrix-core throughput)")
+    a3.add_argument("--rows", type=int, default=1 << 22)
+    a3.add_argument("--repeat", type=int, default=3)
+    a3.add_argument("--network", default="")
     a2 = sub.add_parser("train", help="train the network on data files")
     a2.add_argument("data", nargs="+")
     a2.add_argument("--steps", type=int, default=20000)
@@ -239,12 +255,29 @@ def _main(argv=None):
     a2.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     log = lambda s: print(f"[warmstart] {s}", file=sys.stderr, flush=True)
+    if a.cmd == "forward":
+        import json
+        net = mlp.Network.load(a.network) if a.network else mlp.Network(0, seed=1)
+        X = np.random.default_rng(1).normal(size=(a.rows, 8)).astype(np.float32)
+        best = None
+        for _ in range(a.repeat):
+            net.forward_raw(X)
+            t = net.timing()
+            best = t["forward_ms"] if best is None else min(best, t["forward_ms"])
+        flop = 2.0 * (8 * 256 + 3 * 256 * 256 + 256) * a.rows
+        print(json.dumps({"kernel": "k_mlp_fwd", "rows": a.rows, "ms": best, "tflops": flop / (best * 1e-3) / 1e12,
+                          "frac_of_bf16_dense_peak": flop / (best * 1e-3) / 2.5e15}))
+        return
     if a.cmd == "data":
         params = draw_models(a.first + a.models, a.stream)[a.first:]
         t0 = time.perf_counter()
+        def save(X, y, k):
+            np.savez(a.out + ".tmp.npz", X=X, y=y, model=k + a.first, nphot=a.nphot, niter=a.niter, stream=a.stream,
+                     maxinteract=a.maxinteract)
+            os.replace(a.out + ".tmp.npz", a.out)
         X, y, k = dataset(params, tempfile.mkdtemp(prefix="mcx_ws_"), a.nphot, niter=a.niter,
-                          cells_per_model=a.cells, rng_seed=a.first, log=log, chunk=a.chunk)
-        np.savez(a.out, X=X, y=y, model=k + a.first, nphot=a.nphot, niter=a.niter, stream=a.stream)
+                          cells_per_model=a.cells, rng_seed=a.first, log=log, chunk=a.chunk, maxinteract=a.maxinteract,
+                          save=save)
         log(f"{len(y)} rows from {a.models} models in {time.perf_counter() - t0:.1f} s -> {a.out}")
     else:
         Xs, ys = [], []
