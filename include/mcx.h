@@ -182,6 +182,8 @@ typedef struct {
                                         could not hold (Ni may count such a
                                         packet twice; 0 in every test)      */
     int64_t ntimeout;                /* packets abandoned at the time limit */
+    int64_t ntallyatomic;            /* EJv/EJv2 atomics issued (one per
+                                        non-zero limb at each cell exit)    */
 } mcx_tallies;
 
 /* Temperature products of Main.f:498-549 and the void cell (Main.f:546). */

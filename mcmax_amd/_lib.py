@@ -57,6 +57,7 @@ class Tallies(C.Structure):
         ("ngeomfail", C.c_int64), ("tautot", C.c_double), ("Eescape", C.c_double),
         ("nrw", C.c_int64), ("noverflow", C.c_int64), ("thick", _ip),
         ("maxsteps", C.c_int64), ("t_done", C.c_double * 5), ("nvisit_overflow", C.c_int64), ("ntimeout", C.c_int64),
+        ("ntallyatomic", C.c_int64),
     ]
 
 

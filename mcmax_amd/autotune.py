@@ -21,15 +21,15 @@ def _cache_path():
     return os.path.join(d, "autotune.json")
 
 
-def _key(trs, nphot):
+def _key(trs, nphot, maxinteract=0):
     h = hashlib.sha256()
     for t in trs:
         m = t.model
         h.update(f"{m.nr}x{m.nt}x{m.nlam}x{m.ng};".encode())
-    return f"{len(trs)}:{nphot}:{h.hexdigest()[:16]}"
+    return f"{len(trs)}:{nphot}:{maxinteract}:{h.hexdigest()[:16]}"
 
 
-def tune(trs, nphot, candidates=None, probe_nphot=None, randomwalk=None, log=None, use_cache=True):
+def tune(trs, nphot, candidates=None, probe_nphot=None, randomwalk=None, log=None, use_cache=True, maxinteract=0):
     """blocks per model for a batched pass of nphot packets over the models
     of trs: a probe pass of probe_nphot packets (default nphot/16, at least
     4096) per candidate, the fastest kept."""
@@ -38,7 +38,7 @@ def tune(trs, nphot, candidates=None, probe_nphot=None, randomwalk=None, log=Non
     if candidates is None:
         base = max(1, resident // nm)
         candidates = sorted({max(1, base // 2), base, base * 2})
-    key = _key(trs, nphot)
+    key = _key(trs, nphot, maxinteract)
     path = _cache_path()
     cache = {}
     if use_cache and os.path.exists(path):
@@ -53,7 +53,8 @@ def tune(trs, nphot, candidates=None, probe_nphot=None, randomwalk=None, log=Non
     times = {}
     for bpm in candidates:
         t0 = time.perf_counter()
-        transport.batch_transport(trs, probe, seed=987654, blocks_per_model=bpm, randomwalk=randomwalk)
+        transport.batch_transport(trs, probe, seed=987654, blocks_per_model=bpm, randomwalk=randomwalk,
+                                  maxinteract=maxinteract)
         times[bpm] = time.perf_counter() - t0
         if log:
             log(f"autotune: {bpm} blocks per model: {times[bpm]:.2f} s")

@@ -351,13 +351,16 @@ struct TallyAcc {
     unsigned long long e1, l0, l1, l2;
 };
 
-__device__ __forceinline__ void acc_flush(const DevTally &ty, TallyAcc &t) {
+__device__ __forceinline__ void acc_flush(const DevTally &ty, TallyAcc &t, unsigned long long *scnt) {
     if (t.c >= 0) {
         if (t.e1) atomicAdd(&ty.EJv[t.c], t.e1);
         unsigned long long *b = ty.EJv2 + 3 * (size_t)t.c;
         if (t.l0) atomicAdd(b, t.l0);
         if (t.l1) atomicAdd(b + 1, t.l1);
         if (t.l2) atomicAdd(b + 2, t.l2);
+        // the tally atomics issued (the pass's atomic rate)
+        const unsigned n = (t.e1 != 0ull) + (t.l0 != 0ull) + (t.l1 != 0ull) + (t.l2 != 0ull);
+        if (n) atomicAdd(&scnt[C_NTALLYATOM], (unsigned long long)n);
     }
     t.e1 = t.l0 = t.l1 = t.l2 = 0ull;
 }
@@ -659,8 +662,12 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     long long pid = -1;
     int st = S_NEED;
     // the highest stage whose snapshot each wave has seen complete
-    __shared__ int s_gopen[16];
-    if ((threadIdx.x & 63) == 0) s_gopen[threadIdx.x >> 6] = 0;
+    __shared__ int s_gopen[16], s_nap[16], s_polls[16];
+    if ((threadIdx.x & 63) == 0) {
+        s_gopen[threadIdx.x >> 6] = 0;
+        s_nap[threadIdx.x >> 6] = 0;
+        s_polls[threadIdx.x >> 6] = 0;
+    }
     // the stage table
     __shared__ long long sP[KSTAGE_MAX + 1];
     for (int q = threadIdx.x; q <= A.nstage; q += blockDim.x) sP[q] = A.P[q];
@@ -782,7 +789,11 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 }
                 if (st == S_WAIT && (int)P_PK <= g_open) st = S_EMIT;
                 const int gk = g_open + 1;
-                if (__ballot(st == S_WAIT) && gk < A.nstage) {
+                // the gate's counters are summed on every fourth poll of a wave
+                int polls = 0;
+                if (lane == 0) polls = ++s_polls[threadIdx.x >> 6];
+                polls = __shfl(polls, 0);
+                if (__ballot(st == S_WAIT) && gk < A.nstage && (polls & 3) == 1) {
                     unsigned long long v = 0ull;
                     if (lane >= 1 && lane <= gk) v = ld_coh(&ty.gate[G_FIN + lane]);
                     if (lane == 0) v = ld_coh(&ty.gate[G_PASS + gk]);
@@ -819,8 +830,15 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                         }
                     }
                 }
-                // a wave with nothing to do but wait polls less often
-                if (__all(st == S_WAIT || st == S_DONE)) __builtin_amdgcn_s_sleep(16);
+                // a wave with nothing to do but wait polls less and less
+                // often (up to ~8k cycles between polls)
+                if (__all(st == S_WAIT || st == S_DONE)) {
+                    const int nap = s_nap[threadIdx.x >> 6];
+                    for (int z = 0; z < nap; z++) __builtin_amdgcn_s_sleep(127);
+                    if (lane == 0) s_nap[threadIdx.x >> 6] = nap < 2 ? nap + 1 : 2;
+                } else if (lane == 0) {
+                    s_nap[threadIdx.x >> 6] = 0;
+                }
             }
         }
         if (__all(st == S_DONE)) break;
@@ -839,7 +857,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 }
                 st = S_DONE;
             }
-            acc_flush(ty, acc);
+            acc_flush(ty, acc, scnt);
             break;
         }
 
@@ -948,10 +966,13 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 // c's slot of the visit table, and the stage's snapshot
                 vh = ((unsigned)c * 0x9E3779B1u) >> (32 - VIS_LOG2);
                 ve = vt[vh];
+                // plain loads: this wave passed an agent-scope acquire after
+                // it saw the stage open (the builders' release), so stage
+                // pk's snapshot is visible and may be cached in L2
                 const int pk = (int)P_PK;
                 if (!A.serial && pk > 0) {
-                    nS = ld_coh(&ty.Sk[(size_t)pk * md.ncell + c]);
-                    nTg = ld_coh(&ty.Tk[(size_t)pk * md.ncell + c]);
+                    nS = ty.Sk[(size_t)pk * md.ncell + c];
+                    nTg = ty.Tk[(size_t)pk * md.ncell + c];
                 }
             }
             if (newk) {
@@ -972,7 +993,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                                                               (unsigned)(o >> 32));
                     }
                 }
-                acc_flush(ty, acc);
+                acc_flush(ty, acc, scnt);
                 cc = c;
                 acc.c = c;
                 c_Tl = 0.0;
@@ -1631,7 +1652,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
             st = S_NEED;
         }
     }
-    acc_flush(ty, acc);
+    acc_flush(ty, acc, scnt);
 
     // ---- counters: one global atomic per block -------------------------
     if (tautot != 0.0) atomicAdd(&stau, tautot);
@@ -2799,6 +2820,7 @@ extern "C" int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *o) {
     o->maxsteps = (int64_t)cnt[G_MAXSTEPS];
     o->nvisit_overflow = (int64_t)cnt[C_NVISOVF];
     o->ntimeout = (int64_t)cnt[C_NTIMEOUT];
+    o->ntallyatomic = (int64_t)cnt[C_NTALLYATOM];
     for (int k = 0; k < 5; k++) {
         const long long t0 = (long long)cnt[G_TLINE], t1 = (long long)cnt[G_TLINE + 1 + k];
         o->t_done[k] = (t1 != 0 && ctx->wclk_khz > 0) ? (double)(t1 - t0) / (double)ctx->wclk_khz : -1.0;

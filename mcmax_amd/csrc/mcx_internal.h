@@ -26,7 +26,7 @@
 #define EABS_BITS 24  // one nominal packet energy maps to 2^24
 
 enum { C_NEMIT = 0, C_NINTERACT, C_NABS, C_NSCAT, C_NESCAPE, C_NSTARHIT, C_NSTEPS, C_NRAAR, C_NMAXINT,
-       C_NGEOM, C_NOVERFLOW, C_NRW, C_NVISOVF, C_NTIMEOUT, C_NCOUNT };
+       C_NGEOM, C_NOVERFLOW, C_NRW, C_NVISOVF, C_NTIMEOUT, C_NTALLYATOM, C_NCOUNT };
 
 // per-lane table of the cells the lane's current packet has entered (Ni's
 // distinct-packet count and the packet's own absorbed energy per cell):
@@ -38,12 +38,12 @@ enum { C_NEMIT = 0, C_NINTERACT, C_NABS, C_NSCAT, C_NESCAPE, C_NSTARHIT, C_NSTEP
 // slots of DevTally::counters beyond the event counters: the longest packet
 // history (cell steps), packets finished in the current launch, and wall-clock
 // stamps of the launch's start and of 50/90/99/99.9/100% finished
-#define G_MAXSTEPS 14
-#define G_DONE 15
-#define G_TLINE 16
-#define G_CLOCK 22
-#define G_TOREP 23 // set by the first packet abandoned at the time limit
-#define G_NSLOTS 24
+#define G_MAXSTEPS 15
+#define G_DONE 16
+#define G_TLINE 17
+#define G_CLOCK 23
+#define G_TOREP 24 // set by the first packet abandoned at the time limit
+#define G_NSLOTS 25
 
 // Stages of a concurrent pass (at most KSTAGE_MAX) and the readiness counters
 // of their gates: gate[G_FIN + k] packets that finished with their last

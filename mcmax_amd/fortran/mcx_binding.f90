@@ -49,7 +49,7 @@ module mcx_binding
      type(c_ptr) :: thick
      integer(c_int64_t) :: maxsteps
      real(c_double) :: t_done(5)
-     integer(c_int64_t) :: nvisit_overflow, ntimeout
+     integer(c_int64_t) :: nvisit_overflow, ntimeout, ntallyatomic
   end type
 
   type, bind(C) :: mcx_temps

@@ -87,6 +87,8 @@ class Sweep:
         t.set_model(m, tables=False)
         return T
 
+    maxinteract = 0  # > 0: the reference's interaction roulette (keyword maxinteract)
+
     def run(self, niter=3, seed=1, graph=True, blocks_per_model=0, seed_stride=1):
         """niter structure iterations of every model, each pass batched.
         Returns dict(T: last temperatures, transport_s, total_s)."""
@@ -96,12 +98,17 @@ class Sweep:
         for it in range(1, niter + 1):
             t1 = time.perf_counter()
             st = transport.batch_transport(self.trs, self.nphot, seed=seed + 1000 * it, seed_stride=seed_stride,
-                                           blocks_per_model=blocks_per_model, graph=graph)
+                                           blocks_per_model=blocks_per_model, graph=graph,
+                                           maxinteract=self.maxinteract)
             if np.any(st != 0):
                 raise RuntimeError(f"batched pass: status {st}")
             t_tr += time.perf_counter() - t1
+            if self.log:
+                self.log(f"iteration {it}: batched pass of {len(self.models)} models {time.perf_counter() - t1:.1f} s")
             for k in range(len(self.models)):
                 T[k] = self._after_pass(k, states, it)
+                if self.log and (k + 1) % 64 == 0:
+                    self.log(f"iteration {it}: {k + 1} models' structure solved")
             if self.log:
                 self.log(f"iteration {it}: {time.perf_counter() - t0:.1f} s")
         return dict(T=T, transport_s=t_tr, total_s=time.perf_counter() - t0)
@@ -116,7 +123,7 @@ class Sweep:
         T = None
         for it in range(1, niter + 1):
             t1 = time.perf_counter()
-            t.transport(self.nphot, seed=seed + 1000 * it + seed_stride * k)
+            t.transport(self.nphot, seed=seed + 1000 * it + seed_stride * k, maxinteract=self.maxinteract)
             t_tr += time.perf_counter() - t1
             T = self._after_pass(k, states, it)
         return dict(T=T, transport_s=t_tr, total_s=time.perf_counter() - t0)

@@ -112,7 +112,7 @@ class Transport:
             out[k] = out[k].reshape(NANGLE + 1, m.nlam)
         for k in ("nemit", "ninteract", "nabs", "nscat", "nescape", "nstarhit", "nsteps", "nraar",
                   "nmaxinteract", "ngeomfail", "tautot", "Eescape", "nrw", "noverflow", "maxsteps",
-                  "nvisit_overflow", "ntimeout"):
+                  "nvisit_overflow", "ntimeout", "ntallyatomic"):
             out[k] = getattr(t, k)
         out["t_done_ms"] = list(t.t_done)
         return out

@@ -252,6 +252,8 @@ def _main(argv=None):
     a2.add_argument("--batch", type=int, default=8192)
     a2.add_argument("--lr", type=float, default=1e-3)
     a2.add_argument("--seed", type=int, default=1)
+    a2.add_argument("--holdout", type=float, default=0.1,
+                    help="fraction of the models kept out of training to judge the network")
     a2.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     log = lambda s: print(f"[warmstart] {s}", file=sys.stderr, flush=True)
@@ -280,18 +282,29 @@ def _main(argv=None):
                           save=save)
         log(f"{len(y)} rows from {a.models} models in {time.perf_counter() - t0:.1f} s -> {a.out}")
     else:
-        Xs, ys = [], []
+        Xs, ys, ks = [], [], []
         for p in a.data:
             d = np.load(p, allow_pickle=False)
             Xs.append(d["X"])
             ys.append(d["y"])
-        X, y = np.concatenate(Xs), np.concatenate(ys)
+            ks.append(d["model"])
+        X, y, k = np.concatenate(Xs), np.concatenate(ys), np.concatenate(ks)
+        models = np.unique(k)
+        held = models[len(models) - int(round(a.holdout * len(models))):] if a.holdout > 0 else models[:0]
+        test = np.isin(k, held)
         t0 = time.perf_counter()
-        net, loss = train(X, y, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
-        net.save(a.out)
-        log(f"trained on {len(y)} rows, {a.steps} steps in {time.perf_counter() - t0:.1f} s "
-            f"({net.timing()['step_ms']:.3f} ms per step); final loss {loss[-500:].mean():.3e} -> {a.out}")
+        net, loss = train(X[~test], y[~test], steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+        net.save(a.out)
+        log(f"trained on {int((~test).sum())} rows of {len(models) - len(held)} models, {a.steps} steps in "
+            f"{time.perf_counter() - t0:.1f} s ({net.timing()['step_ms']:.3f} ms per step); final loss "
+            f"{loss[-500:].mean():.3e} (normalised) -> {a.out}")
+        if test.any():
+            rel = np.abs(10.0 ** (net.predict(X[test]) - y[test]) - 1.0)
+            per_model = [float(np.median(rel[k[test] == m])) for m in held]
+            log(f"held-out models ({len(held)}): |dT|/T median {np.median(rel):.4f}, 90th percentile "
+                f"{np.percentile(rel, 90):.4f}; per-model medians: median {np.median(per_model):.4f}, "
+                f"worst {max(per_model):.4f}")
 
 
 if __name__ == "__main__":
