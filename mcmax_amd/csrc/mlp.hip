@@ -64,18 +64,21 @@ __device__ __forceinline__ f32x16 mfma32(bf16x8 a, bf16x8 b, f32x16 c) {
 // lane % 32): four groups of four consecutive rows.
 __device__ __forceinline__ int acc_row(int lane, int r) { return (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); }
 
-// Forward pass over rows [blockIdx.x*ROWS, +ROWS) of x [n][NIN]: y [n].
-// TRAIN also stores the pre-activations Z [4][n][WID] for the backward pass.
-template <bool TRAIN>
-__global__ __launch_bounds__(256) void k_mlp_fwd(long n, const float *__restrict__ x, const float *__restrict__ P,
+// Forward pass over rows [blockIdx.x*RT, +RT) of x [n][NIN]: y [n].  TRAIN
+// also stores the pre-activations Z [4][n][WID] for the backward pass.  RT
+// rows per workgroup: each 256 x 256 layer's BF16 weights (128 kB, from L2)
+// serve RT rows, and each wave's B fragment feeds RT/32 MFMA tiles.
+template <bool TRAIN, int RT>
+__global__ __launch_bounds__(256, 2) void k_mlp_fwd(long n, const float *__restrict__ x, const float *__restrict__ P,
                                                  const bf16 *__restrict__ Wt, const bf16 *__restrict__ w5,
                                                  float *__restrict__ y, float *__restrict__ Z) {
-    __shared__ __attribute__((aligned(16))) bf16 H[ROWS * HS];
-    __shared__ float xs[ROWS * NIN];
+    constexpr int NT = RT / 32;  // row tiles of 32
+    __shared__ __attribute__((aligned(16))) bf16 H[RT * HS];
+    __shared__ float xs[RT * NIN];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int i32 = lane & 31, kh = (lane >> 5) * 8;
-    const long r0 = (long)blockIdx.x * ROWS;
-    for (int k = t; k < ROWS * NIN; k += 256) xs[k] = (r0 + k / NIN < n) ? x[r0 * NIN + k] : 0.0f;
+    const long r0 = (long)blockIdx.x * RT;
+    for (int k = t; k < RT * NIN; k += 256) xs[k] = (r0 + k / NIN < n) ? x[r0 * NIN + k] : 0.0f;
     __syncthreads();
     // layer 1: thread t owns column t
     {
@@ -83,7 +86,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(long n, const float *__restrict
 #pragma unroll
         for (int k = 0; k < NIN; k++) w[k] = P[off_w(1) + k * WID + t];
         const float b = P[off_b(1) + t];
-        for (int r = 0; r < ROWS; r++) {
+        for (int r = 0; r < RT; r++) {
             float z = b;
 #pragma unroll
             for (int k = 0; k < NIN; k++) z = fmaf(xs[r * NIN + k], w[k], z);
@@ -91,25 +94,25 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(long n, const float *__restrict
             H[r * HS + t] = (bf16)silu(z);
         }
     }
-    // layers 2..4: wave wv computes columns [64 wv, 64 wv + 64) of all 64 rows
+    // layers 2..4: wave wv computes columns [64 wv, 64 wv + 64) of all RT rows
     for (int l = 2; l <= 4; l++) {
         __syncthreads();
         const bf16 *Wl = Wt + (long)(l - 2) * LW;  // [out][in]
-        f32x16 acc[2][2];
+        f32x16 acc[NT][2];
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int a = 0; a < NT; a++)
 #pragma unroll
             for (int b = 0; b < 2; b++) acc[a][b] = f32x16{};
-#pragma unroll 4
+#pragma unroll 2
         for (int kk = 0; kk < WID; kk += 16) {
-            const bf16x8 a0 = *(const bf16x8 *)&H[i32 * HS + kk + kh];
-            const bf16x8 a1 = *(const bf16x8 *)&H[(32 + i32) * HS + kk + kh];
             const bf16x8 b0 = *(const bf16x8 *)&Wl[(long)(64 * wv + i32) * WID + kk + kh];
             const bf16x8 b1 = *(const bf16x8 *)&Wl[(long)(64 * wv + 32 + i32) * WID + kk + kh];
-            acc[0][0] = mfma32(a0, b0, acc[0][0]);
-            acc[0][1] = mfma32(a0, b1, acc[0][1]);
-            acc[1][0] = mfma32(a1, b0, acc[1][0]);
-            acc[1][1] = mfma32(a1, b1, acc[1][1]);
+#pragma unroll
+            for (int ti = 0; ti < NT; ti++) {
+                const bf16x8 a = *(const bf16x8 *)&H[(32 * ti + i32) * HS + kk + kh];
+                acc[ti][0] = mfma32(a, b0, acc[ti][0]);
+                acc[ti][1] = mfma32(a, b1, acc[ti][1]);
+            }
         }
         __syncthreads();  // every wave has read H
 #pragma unroll
@@ -117,7 +120,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(long n, const float *__restrict
             const int col = 64 * wv + 32 * tj + i32;
             const float b = P[off_b(l) + col];
 #pragma unroll
-            for (int ti = 0; ti < 2; ti++)
+            for (int ti = 0; ti < NT; ti++)
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     const int row = 32 * ti + acc_row(lane, r);
@@ -128,13 +131,14 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(long n, const float *__restrict
         }
     }
     __syncthreads();
-    // output layer: four threads per row, 64 columns each
+    // output layer: 256/RT threads per row
     {
-        const int row = t >> 2, q = t & 3;
+        constexpr int TPR = 256 / RT, CPT = WID / TPR;
+        const int row = t / TPR, q = t % TPR;
         float s = 0.0f;
-        for (int k = q * 64; k < q * 64 + 64; k++) s = fmaf((float)H[row * HS + k], (float)w5[k], s);
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
+        for (int k = q * CPT; k < q * CPT + CPT; k++) s = fmaf((float)H[row * HS + k], (float)w5[k], s);
+#pragma unroll
+        for (int o = 1; o < TPR; o <<= 1) s += __shfl_xor(s, o);
         if (q == 0 && r0 + row < n) y[r0 + row] = s + P[off_b(5)];
     }
 }
@@ -391,8 +395,8 @@ int ensure_rows(mcx_mlp *net, long n) {
 // forward + backward + reduction over the n rows in net->x / net->tg
 int launch_grad(mcx_mlp *net, long n) {
     const int nblk = (int)((n + ROWS - 1) / ROWS);
-    hipLaunchKernelGGL(k_mlp_fwd<true>, dim3(nblk), dim3(256), 0, net->s, n, net->x, net->P, net->Wt, net->w5,
-                       net->y, net->Z);
+    hipLaunchKernelGGL((k_mlp_fwd<true, ROWS>), dim3(nblk), dim3(256), 0, net->s, n, net->x, net->P, net->Wt,
+                       net->w5, net->y, net->Z);
     hipLaunchKernelGGL(k_mlp_bwd, dim3(nblk), dim3(256), 0, net->s, n, (float)n, net->x, net->tg, net->y, net->Z,
                        net->P, net->Wio, net->w5, net->part, net->lossp);
     hipLaunchKernelGGL(k_mlp_reduce, dim3((unsigned)((NPAR + 255) / 256)), dim3(256), 0, net->s, nblk, net->part,
@@ -412,8 +416,9 @@ int launch_cast(mcx_mlp *net, int update, float lr, float b1, float b2, float ep
 
 int mlp_forward_launch(mcx_mlp *net, long n, const float *x, float *y) {
     if (n <= 0) return MCX_OK;
-    hipLaunchKernelGGL(k_mlp_fwd<false>, dim3((unsigned)((n + ROWS - 1) / ROWS)), dim3(256), 0, net->s, n, x, net->P,
-                       net->Wt, net->w5, y, (float *)nullptr);
+    constexpr int RI = 128;  // rows per workgroup of the inference pass
+    hipLaunchKernelGGL((k_mlp_fwd<false, RI>), dim3((unsigned)((n + RI - 1) / RI)), dim3(256), 0, net->s, n, x,
+                       net->P, net->Wt, net->w5, y, (float *)nullptr);
     MCHK(net, hipGetLastError());
     return MCX_OK;
 }
