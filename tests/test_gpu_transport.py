@@ -355,8 +355,8 @@ def _check_stats(res):
 
 def test_staged_statistics_dhs10_thick(gpu, workdir):
     """The production dust (ten DHS species, MRN weights) at the production
-    dust mass on the 20 x 10 grid: 16 concurrent passes of 20000 packets
-    against 16 serial-loop runs of the oracle (tests/golden, written by
+    dust mass on the 20 x 10 grid: 20 concurrent passes of 20000 packets
+    against 48 serial-loop runs of the oracle (tests/golden, written by
     make_staged_fixtures.py): median |dT|/T of the mean fields <= 1%, no
     shift, >= 97.5% of the (run, cell) differences within 3 sigma of one
     run's seed-to-seed noise; the MC spectrum: within 4 sigma of the mean's
@@ -370,7 +370,9 @@ def test_staged_statistics_dhs10_thick(gpu, workdir):
     fx = staged_stats.load_fixture("dhs10_thick")
     m = configs.load_config("tiny", workdir + "/gpu_dhs10_thick", tr=gpu, **CASES["dhs10_thick"])
     assert model_hash(m) == str(fx["model_hash"]), "the device dust differs from the fixture's"
-    runs = _multi_seed(gpu, m, int(fx["nphot"]), range(5001, 5001 + len(fx["seeds"])))
+    # 20 device passes against the fixture's 48 serial runs (the criteria
+    # take each side's own seed-to-seed noise)
+    runs = _multi_seed(gpu, m, int(fx["nphot"]), range(5001, 5021))
     _check_stats(staged_stats.compare(runs, fx))
 
 
