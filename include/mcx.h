@@ -202,8 +202,15 @@ const char *mcx_last_error(const mcx_ctx *ctx);
 int mcx_set_tables(mcx_ctx *ctx, const mcx_tables *tab);
 
 /* Replaces the reads of D%R, D%Theta, C(i,j)%dens/V/w (MCRadiation.f:901,
- * 1150).  Builds KabsTot/KscaTot (MCRadiation.f:210-229) and the per-cell
- * Planck-mean table sum_s w_s Kp_s(T) (computeE, Radiation.f:611-627). */
+ * 1150).  Builds KabsTot/KscaTot (MCRadiation.f:210-229) and, per distinct
+ * species mix, the Planck-mean table sum_s w_s Kp_s(T) (computeE,
+ * Radiation.f:611-627) and with several species the reemission cumulative
+ * sum_s w_s CUM_s (16 MB per mix for 200 wavelengths).  The cumulatives are
+ * built while they fit in 4 GiB, or in the bytes the environment variable
+ * MCX_CLASS_TABLE_BYTES gives (read at each call; many contexts on one GPU
+ * set it low); otherwise the pass sums the species at every probe, which
+ * changes results by rounding only.  Models of one batched launch need the
+ * same choice. */
 int mcx_set_model(mcx_ctx *ctx, const mcx_grid *grid);
 
 /* One transport pass: MCRadiation.f:269-668 on the device.  Tallies stay in

@@ -25,7 +25,7 @@ struct DiffArgs {
     const double *dT;
     const int *Ni;
     int *thick;
-    const double2 *KD;
+    KDSrc KD;
     const int *cls;
     const double *dens, *w, *R, *Kp;
     double *Dc; // [N+1]
@@ -42,7 +42,7 @@ __global__ void k_diff_coef(DiffArgs D) {
     int iT = (int)(D.T[c] / 1.0);
     if (iT <= 1) iT = 1;
     if (iT >= TMAXD - 1) iT = TMAXD - 1;
-    const double Kext = D.KD[(size_t)D.cls[c] * (TMAXD + 1) + iT].x - 0.0;
+    const double Kext = kd_at(D.KD, D.cls[c], iT).x - 0.0;
     double dens = 0.0;
     for (int s = 0; s < D.ng; s++) dens = dens + D.w[(size_t)c * D.ng + s] * D.dens[c];
     const double Dc = 1.0 / (Kext * dens);
@@ -142,7 +142,7 @@ extern "C" int mcx_diffuse(mcx_ctx *ctx, int32_t nphotmin, double dTmax, double 
                            const double *theta_av, double *T, int32_t *thick_out, double *EJv, int32_t *ncells) {
     if (!ctx || !R_av || !theta_av) return MCX_EINVAL;
     if (!ctx->have_temps) return fail(ctx, MCX_ESTATE, "mcx_temperature first");
-    if (!ctx->rw.KD) return fail(ctx, MCX_EINVAL, "Diffuse needs Kext and gsym (mcx_set_tables), thet (mcx_set_model)");
+    if (!ctx->rw.KD.on) return fail(ctx, MCX_EINVAL, "Diffuse needs Kext and gsym (mcx_set_tables), thet (mcx_set_model)");
     if (!(factRW > 0.0)) return fail(ctx, MCX_EINVAL, "factRW must be positive");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;

@@ -74,7 +74,7 @@ __device__ __forceinline__ double qroot(double x) { return sqrt(sqrt(x)); }
 
 // ---------------------------------------------------------------------------
 // Temperature from absorbed energy per gram (Radiation.f:537-562, 629-702).
-__device__ double dev_determineT(const double *__restrict__ kpc, double E1, double Tcell) {
+template <class K> __device__ double dev_determineT(const K &kpc, double E1, double Tcell) {
     int iTmin = 1, iTmax = TMAXD;
     int iT = (int)(Tcell / 1.0);
     if (iT < 1) iT = 1;
@@ -108,7 +108,7 @@ __device__ double dev_determineT(const double *__restrict__ kpc, double E1, doub
 
 // increaseT: walk up from int(Tstart); when the start is already above E1
 // the reference's walk can only fail and fall back to determineT, so do that.
-__device__ double dev_increaseT(const double *__restrict__ kpc, double E1, double Tstart) {
+template <class K> __device__ double dev_increaseT(const K &kpc, double E1, double Tstart) {
     int j = (int)(Tstart / 1.0);
     if (j > TMAXD) j = TMAXD;
     if (j < 1) j = 1;
@@ -129,16 +129,16 @@ __device__ double dev_increaseT(const double *__restrict__ kpc, double E1, doubl
 
 // increaseT with entries jw..jw+2 of the cell's table already in registers
 // (w0, w1, w2): the same walk and result, without the loads it can skip.
-__device__ __forceinline__ double win_at(const double *__restrict__ kpc, int jw, double w0, double w1, double w2,
-                                         int k) {
+template <class K>
+__device__ __forceinline__ double win_at(const K &kpc, int jw, double w0, double w1, double w2, int k) {
     if (k == jw) return w0;
     if (k == jw + 1) return w1;
     if (k == jw + 2) return w2;
     return kpc[k];
 }
 
-__device__ double dev_increaseT_w(const double *__restrict__ kpc, int jw, double w0, double w1, double w2, double E1,
-                                  double Tstart) {
+template <class K>
+__device__ double dev_increaseT_w(const K &kpc, int jw, double w0, double w1, double w2, double E1, double Tstart) {
     int j = (int)(Tstart / 1.0);
     if (j > TMAXD) j = TMAXD;
     if (j < 1) j = 1;
@@ -166,7 +166,7 @@ __device__ __forceinline__ double t_interp(int i, double kp0, double kp1, double
     return qroot((double)a4 + ((double)d4 * (E1 - kp0)) / (kp1 - kp0));
 }
 
-__device__ double dev_T_win(const double *__restrict__ kpc, int jw, double w0, double w1, double w2, double E1) {
+template <class K> __device__ double dev_T_win(const K &kpc, int jw, double w0, double w1, double w2, double E1) {
     if (w0 <= E1 && w1 >= E1) return t_interp(jw, w0, w1, E1);
     if (w1 <= E1 && w2 >= E1) return t_interp(jw + 1, w1, w2, E1);
     return dev_determineT(kpc, E1, (double)((E1 > w2) ? jw + 2 : jw));
@@ -813,8 +813,8 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                                     ld_coh(&ty.Eabs[(size_t)gk * nc + c]);
                                 st_coh(&ty.Sk[(size_t)gk * nc + c], S);
                                 const float Tg =
-                                    S ? (float)dev_determineT(md.KpCls + (size_t)md.cls[c] * (TMAXD + 1),
-                                                              ((double)S * Bm.unitE) / md.mass[c], 1.0)
+                                    S ? (float)dev_determineT(kp_row(md, md.cls[c]), ((double)S * Bm.unitE) / md.mass[c],
+                                                              1.0)
                                       : 0.f;
                                 st_coh(&ty.Tk[(size_t)gk * nc + c], Tg);
                             }
@@ -1125,7 +1125,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 P_U = 0.0;
                 P_V = 0.0;
                 double Ex = (p.E * (KabsL - 0.0)) / KabsL;
-                const double *kpc = md.KpCls + (size_t)((int)P_TC >> 1) * (TMAXD + 1);
+                const KpRow kpc = kp_row(md, (int)P_TC >> 1);
                 const double *Kp0 = tb.Kp; // species 0
                 const double mass = P_MASS;
                 int jw;
@@ -1285,7 +1285,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                             (P_BASE + (double)(unsigned long long)__double_as_longlong(P_OWN)) * Bm.unitE;
                         double Tv = 0.0;
                         if (Eb != 0.0) {
-                            const double *kpc = md.KpCls + (size_t)((int)P_TC >> 1) * (TMAXD + 1);
+                            const KpRow kpc = kp_row(md, (int)P_TC >> 1);
                             const int jw = win_start((c_Tl >= 1.0) ? c_Tl : P_TG);
                             Tv = dev_T_win(kpc, jw, kpc[jw], kpc[jw + 1], kpc[jw + 2], Eb / P_MASS);
                         }
@@ -1575,7 +1575,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
             if (iT == 0) iT = 1;
             if (iT > TMAXD - 1) iT = TMAXD - 1;
             if ((double)iT != P_KDIT) {
-                const double2 kq = rw.KD[(size_t)((int)P_TC >> 1) * (TMAXD + 1) + iT];
+                const double2 kq = kd_at(rw.KD, (int)P_TC >> 1, iT);
                 P_KDX = kq.x;
                 P_KDY = kq.y;
                 P_KDIT = (double)iT;
@@ -1588,7 +1588,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
                 CNT(C_NRW);
                 // the window of the cell's table at the walk's start T, fetched
                 // together with the Eabs update
-                const double *kpc = md.KpCls + (size_t)((int)P_TC >> 1) * (TMAXD + 1);
+                const KpRow kpc = kp_row(md, (int)P_TC >> 1);
                 int jw = (int)(Thint / 1.0);
                 if (jw < 1) jw = 1;
                 if (jw > TMAXD - 2) jw = TMAXD - 2;
@@ -1708,9 +1708,7 @@ __global__ void k_class_kp(int nclass, int ng, const double *__restrict__ Kp, co
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)nclass * (TMAXD + 1)) return;
     int k = (int)(idx / (TMAXD + 1)), T = (int)(idx % (TMAXD + 1));
-    double e = 0.0;
-    for (int s = 0; s < ng; s++) e = e + (Kp[(size_t)s * (TMAXD + 1) + T] * wcls[(size_t)k * ng + s]) * 1.0;
-    KpCls[idx] = e;
+    KpCls[idx] = KpRow{nullptr, Kp, wcls + (size_t)k * ng, ng}[T];
 }
 
 // the reemission cumulatives of each species-mix class: sum_s w_s CUM_s
@@ -1744,56 +1742,11 @@ __global__ void k_cell_units(int nr, int nt, int nlam, const double *__restrict_
     invU2[c] = ldexp(1.0, EJV2_BITS) / (B * B);
 }
 
-// DiffCoeffCell (Dtable.f:97-182) for every species-mix class and integer T:
-// KDext = int dB/dT dnu / int dB/dT / (Kext - g^2 Ksca) dnu,
-// KDabs = int dB/dT Kabs dnu / int dB/dT dnu, trapezoids over nu.
-// The species sums are those the MC pass sees: KabsTot/KscaTot, except at the
-// four temperatures InitRandomWalk evaluates first (Kext - Ksca from the
-// species tables, kept by the one-species cache of Dtable.f:107-112, 175-179).
-__global__ void k_diffcoeff(int nclass, int nlam, int ng, int scattering, const double *__restrict__ Kabs,
-                            const double *__restrict__ Ksca, const double *__restrict__ Kext,
-                            const double *__restrict__ gsym, const double *__restrict__ wcls,
-                            const double *__restrict__ BB, const double *__restrict__ nu, double2 *KD) {
+// DiffCoeffCell (Dtable.f:97-182) for every species-mix class and integer T
+__global__ void k_diffcoeff(int nclass, const KDSrc S, double2 *KD) {
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)nclass * (TMAXD + 1)) return;
-    const int k = (int)(idx / (TMAXD + 1));
-    int iT = (int)(idx % (TMAXD + 1));
-    if (iT < 1) iT = 1;
-    if (iT > TMAXD - 1) iT = TMAXD - 1;
-    const bool kabstot = !(iT == 100 || iT == 600 || iT == 1100 || iT == 1600);
-    const double *w = wcls + (size_t)k * ng;
-    const double *B0 = BB + (size_t)iT * nlam, *B1 = BB + (size_t)(iT + 1) * nlam;
-    double i1 = 0.0, i2 = 0.0, i3 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
-    for (int l = 0; l < nlam; l++) {
-        double ka = 0.0, ks = 0.0, ke = 0.0, gg = 0.0;
-        if (kabstot) {
-            for (int s = 0; s < ng; s++) {
-                ka = ka + (Kabs[(size_t)s * nlam + l] * w[s]) * 1.0;
-                ks = ks + (Ksca[(size_t)s * nlam + l] * w[s]) * 1.0;
-            }
-            ke = ka + ks;
-        } else {
-            for (int s = 0; s < ng; s++) {
-                ke = ke + (Kext[(size_t)s * nlam + l] * w[s]) * 1.0;
-                ks = ks + (Ksca[(size_t)s * nlam + l] * w[s]) * 1.0;
-            }
-            ka = ke - ks;
-        }
-        for (int s = 0; s < ng; s++) gg = gg + ((gsym[(size_t)s * nlam + l] * Ksca[(size_t)s * nlam + l]) * w[s]) * 1.0;
-        const double gl = scattering ? gg / ks : 0.0;
-        const double dB = B1[l] - B0[l];
-        const double s1 = dB / (ke - (gl * gl) * ks), s2 = dB, s3 = dB * (ka - 0.0);
-        if (l > 0) {
-            const double h = fabs(nu[l - 1] - nu[l]) * 0.5;
-            i1 = i1 + h * (p1 + s1);
-            i2 = i2 + h * (p2 + s2);
-            i3 = i3 + h * (p3 + s3);
-        }
-        p1 = s1;
-        p2 = s2;
-        p3 = s3;
-    }
-    KD[idx] = make_double2(i2 / i1, i3 / i2);
+    KD[idx] = dev_diffcoeff(S, (int)(idx / (TMAXD + 1)), (int)(idx % (TMAXD + 1)));
 }
 
 __global__ void k_fill_u4(size_t n, uint4 *p) {
@@ -1817,8 +1770,7 @@ __global__ void k_clock(unsigned long long *p) { *p = wall_clock64(); }
 // after the pass, T(Eabs/m) (the serial loop's last increaseT)
 __global__ void k_finalize(int ncell, int nb, const unsigned long long *EJvq, const unsigned long long *EJv2q,
                            const unsigned long long *Eabsq, const double *invU1, const double *invU2, double unitE,
-                           const double *__restrict__ KpCls, const int *__restrict__ cls,
-                           const double *__restrict__ mass, double *EJv, double *EJv2, double *Eabs, double *Tbw) {
+                           const DevModel md, double *EJv, double *EJv2, double *Eabs, double *Tbw) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     EJv[c] = (double)EJvq[c] / invU1[c];
@@ -1829,15 +1781,14 @@ __global__ void k_finalize(int ncell, int nb, const unsigned long long *EJvq, co
     unsigned long long e = 0ull;
     for (int b = 0; b < nb; b++) e += Eabsq[(size_t)b * ncell + c];
     Eabs[c] = (double)e * unitE;
-    Tbw[c] = e ? dev_determineT(KpCls + (size_t)cls[c] * (TMAXD + 1), ((double)e * unitE) / mass[c], 1.0) : 0.0;
+    Tbw[c] = e ? dev_determineT(kp_row(md, md.cls[c]), ((double)e * unitE) / md.mass[c], 1.0) : 0.0;
 }
 
 // Main.f:498-549 per cell; void cell temperature Main.f:546-548.
 __global__ void k_temperature(int nr, int nt, int ng, const double *__restrict__ EJv, const double *__restrict__ EJv2,
                               const int *__restrict__ Ni, const double *__restrict__ Tbw, const double *__restrict__ V,
-                              const double *__restrict__ KpCls, const int *__restrict__ cls,
-                              const double *__restrict__ w,
-                              const double *__restrict__ Kp, double Tvoid, double *T, double *dT, double *dEJv) {
+                              const DevModel md, const double *__restrict__ w, const double *__restrict__ Kp,
+                              double Tvoid, double *T, double *dT, double *dEJv) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nr * nt) return;
     int i = c / nt;
@@ -1850,7 +1801,7 @@ __global__ void k_temperature(int nr, int nt, int ng, const double *__restrict__
     double Vc = V[c];
     double E = EJv[c] / Vc;
     double E2 = EJv2[c] / (Vc * Vc);
-    double Tc = dev_determineT(KpCls + (size_t)cls[c] * (TMAXD + 1), E, Tbw[c]);
+    double Tc = dev_determineT(kp_row(md, md.cls[c]), E, Tbw[c]);
     if (Tc < 1.0) Tc = 1.0;
     double dTc, dE;
     if (Ni[c] > 1) {
@@ -2149,22 +2100,32 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
         ctx->nclass = nclass;
         ctx->rw.cls = dcls;
         md.cls = dcls;
-        double *KpCls;
-        if ((r = dalloc(ctx, ctx->mod_alloc, &KpCls, (size_t)nclass * (TMAXD + 1)))) return r;
-        {
-            const size_t m = (size_t)nclass * (TMAXD + 1);
-            hipLaunchKernelGGL(k_class_kp, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream, nclass, ng,
+        md.wcls = ctx->d_wcls;
+        md.Kp = ctx->tb.Kp;
+        md.ng = ng;
+        // Tables per class and integer T: the Planck means (8 B per entry),
+        // the diffusion coefficients (16 B) and, with several species, the
+        // reemission cumulatives (8 (nlam-1) B).  Each is built while it
+        // fits in 4 GiB, or in the bytes of the environment's
+        // MCX_CLASS_TABLE_BYTES (many contexts on one GPU); otherwise the
+        // Planck means and diffusion coefficients are evaluated at each read
+        // (the same bits) and the reemission sums the species per probe.
+        size_t cap = (size_t)4 << 30;
+        if (const char *e = getenv("MCX_CLASS_TABLE_BYTES")) cap = (size_t)strtoull(e, nullptr, 10);
+        const size_t nT = (size_t)nclass * (TMAXD + 1);
+        md.KpCls = nullptr;
+        if (nT * sizeof(double) <= cap) {
+            double *KpCls;
+            if ((r = dalloc(ctx, ctx->mod_alloc, &KpCls, nT))) return r;
+            hipLaunchKernelGGL(k_class_kp, dim3((unsigned)((nT + 255) / 256)), dim3(256), 0, ctx->stream, nclass, ng,
                                ctx->tb.Kp, ctx->d_wcls, KpCls);
             HIPCHK(ctx, hipGetLastError());
+            md.KpCls = KpCls;
         }
-        md.KpCls = KpCls;
-        // several species: one reemission cumulative per species-mix class,
-        // so that an absorption searches one table instead of summing the
-        // species' tables at every probe
         md.CUMmix = nullptr;
         const int nb = nlam - 1;
-        const size_t nmix = (size_t)nclass * (TMAXD + 1) * nb;
-        if (ng > 1 && nmix * sizeof(double) <= ((size_t)4 << 30)) {
+        const size_t nmix = nT * nb;
+        if (ng > 1 && nmix * sizeof(double) <= cap) {
             double *mix;
             if ((r = dalloc(ctx, ctx->mod_alloc, &mix, nmix))) return r;
             hipLaunchKernelGGL(k_cum_mix, dim3((unsigned)((nmix + 255) / 256)), dim3(256), 0, ctx->stream, nclass, ng,
@@ -2173,21 +2134,33 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
             md.CUMmix = mix;
         }
         ctx->rw.R = dR;
-        if (ctx->d_Kext && ctx->d_gsym && gr->thet &&
-            (size_t)nclass * (TMAXD + 1) * sizeof(double2) <= ((size_t)4 << 30)) {
+        if (ctx->d_Kext && ctx->d_gsym && gr->thet) {
             std::vector<double> sinT(nt + 1);
             for (int j = 0; j <= nt; j++) sinT[j] = sin(gr->thet[j]);
             const double *dsin;
-            double2 *KD;
             if ((r = dupload(ctx, ctx->mod_alloc, (double **)&dsin, sinT.data(), (size_t)nt + 1))) return r;
-            if ((r = dalloc(ctx, ctx->mod_alloc, &KD, (size_t)nclass * (TMAXD + 1)))) return r;
             if ((r = dalloc(ctx, ctx->mod_alloc, &ctx->d_thick, (size_t)ncell))) return r;
-            size_t n = (size_t)nclass * (TMAXD + 1);
-            hipLaunchKernelGGL(k_diffcoeff, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, nclass, nlam,
-                               ng, ctx->tb.scattering, ctx->d_Kabs, ctx->d_Ksca, ctx->d_Kext, ctx->d_gsym,
-                               ctx->d_wcls, ctx->d_BB, ctx->tb.nu, KD);
-            HIPCHK(ctx, hipGetLastError());
-            ctx->rw.KD = KD;
+            KDSrc S{};
+            S.on = 1;
+            S.Kabs = ctx->d_Kabs;
+            S.Ksca = ctx->d_Ksca;
+            S.Kext = ctx->d_Kext;
+            S.gsym = ctx->d_gsym;
+            S.wcls = ctx->d_wcls;
+            S.BB = ctx->d_BB;
+            S.nu = ctx->tb.nu;
+            S.nlam = nlam;
+            S.ng = ng;
+            S.scattering = ctx->tb.scattering;
+            if (nT * sizeof(double2) <= cap) {
+                double2 *KD;
+                if ((r = dalloc(ctx, ctx->mod_alloc, &KD, nT))) return r;
+                hipLaunchKernelGGL(k_diffcoeff, dim3((unsigned)((nT + 255) / 256)), dim3(256), 0, ctx->stream, nclass,
+                                   S, KD);
+                HIPCHK(ctx, hipGetLastError());
+                S.t = KD;
+            }
+            ctx->rw.KD = S;
             ctx->rw.sinT = dsin;
             ctx->rw.y = ctx->d_rwy;
             ctx->rw.phi = ctx->d_rwphi;
@@ -2314,7 +2287,7 @@ __global__ void k_bm_prep(const BatchModel *__restrict__ bm, int m, double factR
                 int th = 0;
                 if (B.rw.on && i > 0)
                     for (int iT = 100; iT <= 1600; iT += 500) {
-                        const double lr = 1.0 / ((MCX_AU * B.rw.KD[(size_t)md.cls[c] * (TMAXD + 1) + iT].x) * md.dens[c]);
+                        const double lr = 1.0 / ((MCX_AU * kd_at(B.rw.KD, md.cls[c], iT).x) * md.dens[c]);
                         if ((B.rw.R[i + 1] - B.rw.R[i]) > factRW * lr) th = 1;
                     }
                 ty.thick[c] = th;
@@ -2359,8 +2332,7 @@ __global__ void k_bm_finalize(const BatchModel *__restrict__ bm, int m, int nb) 
     unsigned long long e = 0ull;
     for (int b = 0; b < nb; b++) e += ty.Eabs[(size_t)b * md.ncell + c];
     ty.oEabs[c] = (double)e * B.unitE;
-    ty.oTbw[c] = e ? dev_determineT(md.KpCls + (size_t)md.cls[c] * (TMAXD + 1), ((double)e * B.unitE) / md.mass[c], 1.0)
-                   : 0.0;
+    ty.oTbw[c] = e ? dev_determineT(kp_row(md, md.cls[c]), ((double)e * B.unitE) / md.mass[c], 1.0) : 0.0;
 }
 
 // Per-model set-up of a pass, host part: checks, the packet energy and the
@@ -2370,10 +2342,9 @@ static int host_prep(mcx_ctx *ctx, const mcx_run *run, uint64_t seed, BatchModel
     DevRW rw = ctx->rw;
     rw.on = 0;
     if (run->randomwalk) {
-        if (!rw.KD)
+        if (!rw.KD.on)
             return fail(ctx, MCX_EINVAL,
-                        "random walk needs Kext and gsym (mcx_set_tables), thet (mcx_set_model) and at most "
-                        "4 GiB of diffusion-coefficient tables");
+                        "random walk needs Kext and gsym (mcx_set_tables), thet (mcx_set_model)");
         if (!(run->factRW > 0.0)) return fail(ctx, MCX_EINVAL, "factRW must be positive");
         rw.on = 1;
         rw.factRW = run->factRW;
@@ -2724,7 +2695,7 @@ static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_re
         const int ncell = md.ncell;
         hipLaunchKernelGGL(k_finalize, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell, A.nstage + 1, c->ty.EJv,
                            c->ty.EJv2,
-                           c->ty.Eabs, c->ty.invU1, c->ty.invU2, c->unitE, md.KpCls, md.cls, md.mass, c->d_EJv,
+                           c->ty.Eabs, c->ty.invU1, c->ty.invU2, c->unitE, md, c->d_EJv,
                            c->d_EJv2, c->d_Eabs, c->d_Tbw);
         HIPCHK(ctx, hipGetLastError());
     }
@@ -2835,7 +2806,7 @@ extern "C" int mcx_temperature(mcx_ctx *ctx, mcx_temps *o) {
     const DevModel &md = ctx->md;
     const int ncell = md.ncell;
     hipLaunchKernelGGL(k_temperature, dim3((ncell + 255) / 256), dim3(256), 0, ctx->stream, md.nr, md.nt, ctx->ng,
-                       ctx->d_EJv, ctx->d_EJv2, ctx->d_Ni, ctx->d_Tbw, ctx->d_V, md.KpCls, md.cls, md.w, ctx->tb.Kp,
+                       ctx->d_EJv, ctx->d_EJv2, ctx->d_Ni, ctx->d_Tbw, ctx->d_V, md, md.w, ctx->tb.Kp,
                        ctx->Tvoid, ctx->d_T, ctx->d_dT, ctx->d_dEJv);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

@@ -87,11 +87,98 @@ struct DevModel {
     const double *w;               // [ncell*ng]
     const double2 *KT;             // [ncell*nlam] {KabsTot, KscaTot}
     const double *KpCls;           // [nclass*(TMAX+1)] sum_s w_s Kp_s(T) of each species-mix
-                                   // class (computeE as a table; cell c: row cls[c])
+                                   // class (computeE as a table; cell c: row cls[c]);
+                                   // null: summed at each read (KpRow)
     const int *cls;                // [ncell] species-mix class of the cell
     const double *CUMmix;          // [nclass][TMAX+1][nlam-1] sum_s w_s CUM_s of each
                                    // class (several species; null: sum per event)
+    const double *wcls;            // [nclass*ng] weights of each class
+    const double *Kp;              // [ng][TMAX+1] the species' Planck means
+    int ng;
 };
+
+// The Planck-mean row sum_s w_s Kp_s(T) of a species mix (computeE,
+// Radiation.f:611-627): the class table, or where the context keeps none the
+// species' rows summed at each read, in the order the tables are built from
+// (so both give the same bits).
+__device__ __attribute__((noinline)) inline double kp_sum(const double *Kp, const double *w, int ng, int j) {
+    double e = 0.0;
+    for (int s = 0; s < ng; s++) e = e + (Kp[(size_t)s * (TMAXD + 1) + j] * w[s]) * 1.0;
+    return e;
+}
+
+struct KpRow {
+    const double *t, *Kp, *w;
+    int ng;
+    __device__ __forceinline__ double operator[](int j) const { return t ? t[j] : kp_sum(Kp, w, ng, j); }
+};
+
+__device__ __forceinline__ KpRow kp_row(const DevModel &md, int k) {
+    return KpRow{md.KpCls ? md.KpCls + (size_t)k * (TMAXD + 1) : nullptr, md.Kp, md.wcls + (size_t)k * md.ng, md.ng};
+}
+
+// Where the diffusion coefficients {KDext, KDabs} of a species mix at integer
+// T come from: the class table t, or (t null) DiffCoeffCell evaluated at each
+// read from the species' opacities.  on: Kext and g are known.
+struct KDSrc {
+    int on;
+    const double2 *t;                  // [nclass*(TMAX+1)]
+    const double *Kabs, *Ksca, *Kext;  // [ng][nlam]
+    const double *gsym, *wcls;         // [ng][nlam], [nclass*ng]
+    const double *BB, *nu;             // [TMAX+1][nlam], [nlam]
+    int nlam, ng, scattering;
+};
+
+// DiffCoeffCell (Dtable.f:97-182) of class k at integer T:
+// KDext = int dB/dT dnu / int dB/dT / (Kext - g^2 Ksca) dnu,
+// KDabs = int dB/dT Kabs dnu / int dB/dT dnu, trapezoids over nu.
+// The species sums are those the MC pass sees: KabsTot/KscaTot, except at the
+// four temperatures InitRandomWalk evaluates first (Kext - Ksca from the
+// species tables, kept by the one-species cache of Dtable.f:107-112, 175-179).
+__device__ __attribute__((noinline)) inline double2 dev_diffcoeff(const KDSrc &S, int k, int iT) {
+    if (iT < 1) iT = 1;
+    if (iT > TMAXD - 1) iT = TMAXD - 1;
+    const int nlam = S.nlam, ng = S.ng;
+    const bool kabstot = !(iT == 100 || iT == 600 || iT == 1100 || iT == 1600);
+    const double *w = S.wcls + (size_t)k * ng;
+    const double *B0 = S.BB + (size_t)iT * nlam, *B1 = S.BB + (size_t)(iT + 1) * nlam;
+    double i1 = 0.0, i2 = 0.0, i3 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+    for (int l = 0; l < nlam; l++) {
+        double ka = 0.0, ks = 0.0, ke = 0.0, gg = 0.0;
+        if (kabstot) {
+            for (int s = 0; s < ng; s++) {
+                ka = ka + (S.Kabs[(size_t)s * nlam + l] * w[s]) * 1.0;
+                ks = ks + (S.Ksca[(size_t)s * nlam + l] * w[s]) * 1.0;
+            }
+            ke = ka + ks;
+        } else {
+            for (int s = 0; s < ng; s++) {
+                ke = ke + (S.Kext[(size_t)s * nlam + l] * w[s]) * 1.0;
+                ks = ks + (S.Ksca[(size_t)s * nlam + l] * w[s]) * 1.0;
+            }
+            ka = ke - ks;
+        }
+        for (int s = 0; s < ng; s++)
+            gg = gg + ((S.gsym[(size_t)s * nlam + l] * S.Ksca[(size_t)s * nlam + l]) * w[s]) * 1.0;
+        const double gl = S.scattering ? gg / ks : 0.0;
+        const double dB = B1[l] - B0[l];
+        const double s1 = dB / (ke - (gl * gl) * ks), s2 = dB, s3 = dB * (ka - 0.0);
+        if (l > 0) {
+            const double h = fabs(S.nu[l - 1] - S.nu[l]) * 0.5;
+            i1 = i1 + h * (p1 + s1);
+            i2 = i2 + h * (p2 + s2);
+            i3 = i3 + h * (p3 + s3);
+        }
+        p1 = s1;
+        p2 = s2;
+        p3 = s3;
+    }
+    return make_double2(i2 / i1, i3 / i2);
+}
+
+__device__ __forceinline__ double2 kd_at(const KDSrc &S, int k, int iT) {
+    return S.t ? S.t[(size_t)k * (TMAXD + 1) + iT] : dev_diffcoeff(S, k, iT);
+}
 
 struct DevTally {
     unsigned long long *EJv;    // [ncell]
@@ -121,7 +208,7 @@ struct DevRW {
     double factRW;
     const int *thick;     // [ncell]
     const int *cls;       // [ncell] species-mix class of the cell
-    const double2 *KD;    // [nclass*(TMAX+1)] {KDext, KDabs} (DiffCoeffCell)
+    KDSrc KD;             // {KDext, KDabs} of each class (DiffCoeffCell)
     const double *R;      // [nr+1] radial edges (AU)
     const double *sinT;   // [nt+1] sin(theta) of the cone edges
     const double *y, *phi; // [NY_RW] (InitRandomWalk)
@@ -218,15 +305,14 @@ static inline void free_list(std::vector<void *> &list) {
 // C%thick (RandomWalk.f:174-189): the shell is wider than factRW Rosseland
 // lengths at any of T = 100, 600, 1100, 1600 K.
 static __global__ void k_thick_cells(int nr, int nt, const double *__restrict__ R, const double *__restrict__ dens,
-                                     const int *__restrict__ cls, const double2 *__restrict__ KD, double factRW,
-                                     int *thick) {
+                                     const int *__restrict__ cls, const KDSrc KD, double factRW, int *thick) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nr * nt) return;
     const int i = c / nt;
     int th = 0;
     if (i > 0) {
         for (int iT = 100; iT <= 1600; iT += 500) {
-            const double lr = 1.0 / ((MCX_AU * KD[(size_t)cls[c] * (TMAXD + 1) + iT].x) * dens[c]);
+            const double lr = 1.0 / ((MCX_AU * kd_at(KD, cls[c], iT).x) * dens[c]);
             if ((R[i + 1] - R[i]) > factRW * lr) th = 1;
         }
     }

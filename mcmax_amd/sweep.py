@@ -60,6 +60,12 @@ class Sweep:
             dust.close()
         for k, m in enumerate(self.models):
             t = transport.Transport(device)
+            # DiskStructure gives most cells a species mix of their own: the
+            # per-mix tables over T (80-240 kB each, 16 MB with the
+            # reemission cumulative) would take GBs per model.  At most 1 MiB
+            # per table: every model sums the species' cumulatives per probe
+            # (one kernel variant for the whole batch in every iteration).
+            t.class_table_bytes = 1 << 20
             t.set_model(m)
             self.trs.append(t)
             if log and (k + 1) % 16 == 0:

@@ -4,6 +4,7 @@ Lucy temperature inversion (Main.f:498-558).  Everything runs in libmcx.so on
 the GPU; this module only moves arrays across the C-ABI.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -32,6 +33,12 @@ class Transport:
             raise MCXError(rc, "mcx_create failed (no GPU visible?)")
         self._keep = []
         self.model = None
+        # the largest per-species-mix table (over integer T) that set_model
+        # builds, in bytes; None: the library's 4 GiB.  Many contexts on one
+        # GPU set it low: the Planck means and diffusion coefficients are then
+        # evaluated at each read (the same values) and the reemission sums
+        # the species' cumulatives at each probe (the same up to rounding).
+        self.class_table_bytes = None
 
     def close(self):
         if self.ctx:
@@ -69,7 +76,18 @@ class Transport:
         g = _lib.Grid(nr=m.nr, nt=m.nt, R=ptr(keep["R"]), Theta=ptr(keep["Th"]), dens=ptr(keep["dens"]),
                       V=ptr(keep["V"]), w=ptr(keep["w"]), Rstar_cm=float(m.Rstar_cm), Tstar=float(m.Tstar),
                       thet=ptr(keep["thet"]))
-        self._check(self.lib.mcx_set_model(self.ctx, C.byref(g)))
+        cap = self.class_table_bytes
+        old = os.environ.get("MCX_CLASS_TABLE_BYTES")
+        if cap is not None:
+            os.environ["MCX_CLASS_TABLE_BYTES"] = str(int(cap))
+        try:
+            self._check(self.lib.mcx_set_model(self.ctx, C.byref(g)))
+        finally:
+            if cap is not None:
+                if old is None:
+                    del os.environ["MCX_CLASS_TABLE_BYTES"]
+                else:
+                    os.environ["MCX_CLASS_TABLE_BYTES"] = old
         self.model = m
 
     def transport(self, nphot, seed=1, nlanes=0, maxinteract=0, randomwalk=None, factRW=None, coop_lanes=-1,

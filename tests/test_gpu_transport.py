@@ -227,6 +227,47 @@ def test_serial_parity_dhs_species(gpu, tiny_dhs_model, oracle_lib):
     _serial_compare(gpu, m, oracle_lib, 300, seed=11)
 
 
+@pytest.mark.parametrize("cap", [0, 1 << 20])
+def test_serial_parity_dhs_species_no_class_tables(gpu, tiny_dhs_rw_model, oracle_lib, cap):
+    """The random-walk model without per-mix tables over T (the setting many
+    contexts on one GPU use): the Planck means and diffusion coefficients
+    evaluated at each read, the reemission summing the three species' rows
+    at every probe of the wavelength search (cap 0: no tables at all; 1 MiB:
+    the Planck-mean and diffusion tables of the few mixes, no cumulatives)."""
+    m = tiny_dhs_rw_model
+    gpu.class_table_bytes = cap
+    try:
+        g, o = _serial_compare(gpu, m, oracle_lib, 200, seed=13)
+        assert o["nrw"] > 20
+        _serial_compare(gpu, m, oracle_lib, 1000, seed=41, staged=True)
+    finally:
+        gpu.class_table_bytes = None
+        gpu.set_model(m)
+
+
+def test_class_tables_read_or_evaluated(gpu, tiny_dhs_rw_model):
+    """The Planck-mean and diffusion-coefficient rows evaluated at each read
+    equal the tables bit for bit: a concurrent random-walk pass with the
+    tables of the few mixes (1 MiB cap) and with none (cap 0) gives the same
+    tallies and temperatures."""
+    m = tiny_dhs_rw_model
+    out = []
+    try:
+        for cap in (1 << 20, 0):
+            gpu.class_table_bytes = cap
+            gpu.set_model(m)
+            out.append(transport.mc_radiation(gpu, 20000, seed=77))
+    finally:
+        gpu.class_table_bytes = None
+        gpu.set_model(m)
+    a, b = out
+    assert a["nrw"] > 0
+    for k in ("EJv", "EJv2", "Eabs", "Tbw", "T", "spec", "Ni"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in COUNTERS:
+        assert a[k] == b[k], k
+
+
 def test_serial_parity_dhs_species_random_walk(gpu, tiny_dhs_rw_model, oracle_lib):
     g, o = _serial_compare(gpu, tiny_dhs_rw_model, oracle_lib, 200, seed=13)
     assert o["nrw"] > 20
