@@ -1749,6 +1749,20 @@ __global__ void k_diffcoeff(int nclass, const KDSrc S, double2 *KD) {
     KD[idx] = dev_diffcoeff(S, (int)(idx / (TMAXD + 1)), (int)(idx % (TMAXD + 1)));
 }
 
+// without the table: each class's per-wavelength terms of DiffCoeffCell and
+// its coefficients at the thick test's four temperatures
+__global__ void k_diffcoeff_terms(int nclass, const KDSrc S, double2 *AK, double2 *K4) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nak = (size_t)nclass * S.nlam;
+    if (idx < nak) {
+        const int k = (int)(idx / S.nlam), l = (int)(idx % S.nlam);
+        AK[idx] = kd_terms(S, S.wcls + (size_t)k * S.ng, l, true);
+    } else if (idx < nak + (size_t)nclass * 4) {
+        const size_t q = idx - nak;
+        K4[q] = dev_diffcoeff(S, (int)(q / 4), 100 + 500 * (int)(q % 4));
+    }
+}
+
 __global__ void k_fill_u4(size_t n, uint4 *p) {
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
         p[k] = make_uint4(0u, 0u, 0u, 0u);
@@ -2159,6 +2173,18 @@ extern "C" int mcx_set_model(mcx_ctx *ctx, const mcx_grid *gr) {
                                    S, KD);
                 HIPCHK(ctx, hipGetLastError());
                 S.t = KD;
+            } else {
+                // the per-wavelength terms (16 nlam B per class) make an
+                // evaluation ~ nlam divisions instead of 3 ng nlam loads
+                double2 *AK, *K4;
+                const size_t nak = (size_t)nclass * nlam, n = nak + (size_t)nclass * 4;
+                if ((r = dalloc(ctx, ctx->mod_alloc, &AK, nak))) return r;
+                if ((r = dalloc(ctx, ctx->mod_alloc, &K4, (size_t)nclass * 4))) return r;
+                hipLaunchKernelGGL(k_diffcoeff_terms, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                                   nclass, S, AK, K4);
+                HIPCHK(ctx, hipGetLastError());
+                S.AK = AK;
+                S.K4 = K4;
             }
             ctx->rw.KD = S;
             ctx->rw.sinT = dsin;

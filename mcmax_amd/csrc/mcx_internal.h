@@ -119,50 +119,65 @@ __device__ __forceinline__ KpRow kp_row(const DevModel &md, int k) {
 
 // Where the diffusion coefficients {KDext, KDabs} of a species mix at integer
 // T come from: the class table t, or (t null) DiffCoeffCell evaluated at each
-// read from the species' opacities.  on: Kext and g are known.
+// read, from the class's per-wavelength terms AK and, at the four
+// temperatures of the thick test, from K4.  on: Kext and g are known.
 struct KDSrc {
     int on;
     const double2 *t;                  // [nclass*(TMAX+1)]
+    const double2 *AK;                 // [nclass*nlam] {Kext - g^2 Ksca, Kabs} of the mix
+    const double2 *K4;                 // [nclass*4] at T = 100, 600, 1100, 1600
     const double *Kabs, *Ksca, *Kext;  // [ng][nlam]
     const double *gsym, *wcls;         // [ng][nlam], [nclass*ng]
     const double *BB, *nu;             // [TMAX+1][nlam], [nlam]
     int nlam, ng, scattering;
 };
 
+// The mix's terms of DiffCoeffCell at wavelength l (Dtable.f:120-160): the
+// Rosseland denominator Kext - g^2 Ksca and Kabs.  The species sums are
+// those the MC pass sees (KabsTot/KscaTot), except at the four temperatures
+// InitRandomWalk evaluates first (kabstot false: Kext - Ksca from the species
+// tables, kept by the one-species cache of Dtable.f:107-112, 175-179).
+__device__ __forceinline__ double2 kd_terms(const KDSrc &S, const double *w, int l, bool kabstot) {
+    const int nlam = S.nlam, ng = S.ng;
+    double ka = 0.0, ks = 0.0, ke = 0.0, gg = 0.0;
+    if (kabstot) {
+        for (int s = 0; s < ng; s++) {
+            ka = ka + (S.Kabs[(size_t)s * nlam + l] * w[s]) * 1.0;
+            ks = ks + (S.Ksca[(size_t)s * nlam + l] * w[s]) * 1.0;
+        }
+        ke = ka + ks;
+    } else {
+        for (int s = 0; s < ng; s++) {
+            ke = ke + (S.Kext[(size_t)s * nlam + l] * w[s]) * 1.0;
+            ks = ks + (S.Ksca[(size_t)s * nlam + l] * w[s]) * 1.0;
+        }
+        ka = ke - ks;
+    }
+    for (int s = 0; s < ng; s++)
+        gg = gg + ((S.gsym[(size_t)s * nlam + l] * S.Ksca[(size_t)s * nlam + l]) * w[s]) * 1.0;
+    const double gl = S.scattering ? gg / ks : 0.0;
+    return make_double2(ke - (gl * gl) * ks, ka - 0.0);
+}
+
 // DiffCoeffCell (Dtable.f:97-182) of class k at integer T:
 // KDext = int dB/dT dnu / int dB/dT / (Kext - g^2 Ksca) dnu,
 // KDabs = int dB/dT Kabs dnu / int dB/dT dnu, trapezoids over nu.
-// The species sums are those the MC pass sees: KabsTot/KscaTot, except at the
-// four temperatures InitRandomWalk evaluates first (Kext - Ksca from the
-// species tables, kept by the one-species cache of Dtable.f:107-112, 175-179).
+// With AK the per-wavelength terms are read, else summed over the species
+// (the same bits).
 __device__ __attribute__((noinline)) inline double2 dev_diffcoeff(const KDSrc &S, int k, int iT) {
     if (iT < 1) iT = 1;
     if (iT > TMAXD - 1) iT = TMAXD - 1;
-    const int nlam = S.nlam, ng = S.ng;
+    const int nlam = S.nlam;
     const bool kabstot = !(iT == 100 || iT == 600 || iT == 1100 || iT == 1600);
-    const double *w = S.wcls + (size_t)k * ng;
+    const bool tab = kabstot && S.AK;
+    const double *w = S.wcls + (size_t)k * S.ng;
+    const double2 *ak = tab ? S.AK + (size_t)k * nlam : nullptr;
     const double *B0 = S.BB + (size_t)iT * nlam, *B1 = S.BB + (size_t)(iT + 1) * nlam;
     double i1 = 0.0, i2 = 0.0, i3 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
     for (int l = 0; l < nlam; l++) {
-        double ka = 0.0, ks = 0.0, ke = 0.0, gg = 0.0;
-        if (kabstot) {
-            for (int s = 0; s < ng; s++) {
-                ka = ka + (S.Kabs[(size_t)s * nlam + l] * w[s]) * 1.0;
-                ks = ks + (S.Ksca[(size_t)s * nlam + l] * w[s]) * 1.0;
-            }
-            ke = ka + ks;
-        } else {
-            for (int s = 0; s < ng; s++) {
-                ke = ke + (S.Kext[(size_t)s * nlam + l] * w[s]) * 1.0;
-                ks = ks + (S.Ksca[(size_t)s * nlam + l] * w[s]) * 1.0;
-            }
-            ka = ke - ks;
-        }
-        for (int s = 0; s < ng; s++)
-            gg = gg + ((S.gsym[(size_t)s * nlam + l] * S.Ksca[(size_t)s * nlam + l]) * w[s]) * 1.0;
-        const double gl = S.scattering ? gg / ks : 0.0;
+        const double2 q = tab ? ak[l] : kd_terms(S, w, l, kabstot);
         const double dB = B1[l] - B0[l];
-        const double s1 = dB / (ke - (gl * gl) * ks), s2 = dB, s3 = dB * (ka - 0.0);
+        const double s1 = dB / q.x, s2 = dB, s3 = dB * q.y;
         if (l > 0) {
             const double h = fabs(S.nu[l - 1] - S.nu[l]) * 0.5;
             i1 = i1 + h * (p1 + s1);
@@ -177,7 +192,9 @@ __device__ __attribute__((noinline)) inline double2 dev_diffcoeff(const KDSrc &S
 }
 
 __device__ __forceinline__ double2 kd_at(const KDSrc &S, int k, int iT) {
-    return S.t ? S.t[(size_t)k * (TMAXD + 1) + iT] : dev_diffcoeff(S, k, iT);
+    if (S.t) return S.t[(size_t)k * (TMAXD + 1) + iT];
+    if (S.K4 && (iT == 100 || iT == 600 || iT == 1100 || iT == 1600)) return S.K4[(size_t)k * 4 + (iT - 100) / 500];
+    return dev_diffcoeff(S, k, iT);
 }
 
 struct DevTally {
