@@ -222,11 +222,11 @@ int mcx_transport(mcx_ctx *ctx, const mcx_run *run);
  * which holds one model in its module globals, Modules.f:141-150, and runs
  * MCRadiation for it, MCRadiation.f:6).  Each of the nmodels contexts holds
  * one model (mcx_set_tables + mcx_set_model, grids of any size, same species
- * case); one pass of run->nphot packets runs for all of them, each stage in
- * one launch with blocks_per_model workgroups per model (0: the device's
- * resident workgroups shared out), on the first context's stream.  The
- * long histories that end each stage then run side by side instead of one
- * model after another.  Model m uses the Philox key run->seed + m *
+ * case); one pass of run->nphot packets runs for all of them in one launch
+ * with blocks_per_model workgroups per model (0: the device's resident
+ * workgroups shared out), on the first context's stream.  The long
+ * histories that end each model's stages then run side by side instead of
+ * one model after another.  Model m uses the Philox key run->seed + m *
  * seed_stride; its tallies are bit-identical to those of mcx_transport on
  * its own context with that key.  Read them with mcx_get_tallies /
  * mcx_temperature on the model's context.  status[m] (may be NULL)
@@ -240,6 +240,15 @@ int mcx_transport(mcx_ctx *ctx, const mcx_run *run);
  * mcx_set_model only if the grid sizes stay the same. */
 int mcx_batch_transport(mcx_ctx *const *ctxs, int32_t nmodels, const mcx_run *run, int32_t blocks_per_model,
                         int64_t seed_stride, int32_t use_graph, int32_t *status);
+
+/* mcx_batch_transport with each model's own share of the launch: model m
+ * runs on model_blocks[m] >= 1 workgroups (a sweep gives the models with
+ * more work more of the device).  Results do not depend on the shares.  A
+ * graph (use_graph) is replayed while the total number of workgroups stays
+ * the same; the shares themselves may change between calls. */
+int mcx_batch_transport_blocks(mcx_ctx *const *ctxs, int32_t nmodels, const mcx_run *run,
+                               const int32_t *model_blocks, int64_t seed_stride, int32_t use_graph,
+                               int32_t *status);
 
 /* Copy the last pass's tallies to the host (any pointer may be NULL). */
 int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *out);

@@ -117,7 +117,7 @@ EXPORTS = (
     "mcx_transport", "mcx_get_tallies", "mcx_temperature", "mcx_last_timing", "mcx_sed",
     "mcx_set_temperature", "mcx_diffuse", "mcx_temp_average", "mcx_disk_structure", "mcx_compare_cells",
     "mcx_compute_parts", "mcx_image", "mcx_last_image_timing", "mcx_batch_transport",
-    "mcx_mlp_create", "mcx_mlp_destroy", "mcx_mlp_last_error", "mcx_mlp_set_params", "mcx_mlp_get_params",
+    "mcx_batch_transport_blocks", "mcx_mlp_create", "mcx_mlp_destroy", "mcx_mlp_last_error", "mcx_mlp_set_params", "mcx_mlp_get_params",
     "mcx_mlp_forward", "mcx_mlp_gradient", "mcx_mlp_set_data", "mcx_mlp_train", "mcx_mlp_last_timing",
     "mcx_mlp_forward_noalloc",
 )
@@ -154,6 +154,8 @@ def load():
     lib.mcx_transport.argtypes = [vp, C.POINTER(Run)]
     lib.mcx_batch_transport.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Run), C.c_int32, C.c_int64, C.c_int32,
                                         C.POINTER(C.c_int32)]
+    lib.mcx_batch_transport_blocks.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Run), C.POINTER(C.c_int32),
+                                               C.c_int64, C.c_int32, C.POINTER(C.c_int32)]
     lib.mcx_get_tallies.argtypes = [vp, C.POINTER(Tallies)]
     lib.mcx_temperature.argtypes = [vp, C.POINTER(Temps)]
     lib.mcx_last_timing.argtypes = [vp, _dp, _dp]
@@ -189,7 +191,7 @@ def load():
     for n in ("mcx_set_tables", "mcx_set_model", "mcx_transport", "mcx_get_tallies",
               "mcx_temperature", "mcx_last_timing", "mcx_sed", "mcx_set_temperature", "mcx_diffuse",
               "mcx_temp_average", "mcx_disk_structure", "mcx_compare_cells", "mcx_compute_parts", "mcx_image",
-              "mcx_last_image_timing", "mcx_batch_transport"):
+              "mcx_last_image_timing", "mcx_batch_transport", "mcx_batch_transport_blocks"):
         getattr(lib, n).restype = C.c_int
     _lib = lib
     return lib

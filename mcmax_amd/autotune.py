@@ -29,12 +29,15 @@ def _key(trs, nphot, maxinteract=0):
     return f"{len(trs)}:{nphot}:{maxinteract}:{h.hexdigest()[:16]}"
 
 
+RESIDENT = 2 * 256  # resident transport workgroups: two per CU on the 256 CUs
+
+
 def tune(trs, nphot, candidates=None, probe_nphot=None, randomwalk=None, log=None, use_cache=True, maxinteract=0):
     """blocks per model for a batched pass of nphot packets over the models
     of trs: a probe pass of probe_nphot packets (default nphot/16, at least
     4096) per candidate, the fastest kept."""
     nm = len(trs)
-    resident = 2 * 256  # two workgroups per CU on the 256 CUs
+    resident = RESIDENT
     if candidates is None:
         base = max(1, resident // nm)
         candidates = sorted({max(1, base // 2), base, base * 2})

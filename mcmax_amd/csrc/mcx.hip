@@ -50,6 +50,8 @@ struct BatchModel {
 struct TransportArgs {
     const BatchModel *bm; // [models]
     int bpm;              // blocks per model
+    const int *bmap;      // [blocks] the model of each workgroup (null: blockIdx / bpm)
+    int nmodels;
     long long pid_end;
     long long pid_base;
     long long nphot;
@@ -566,11 +568,14 @@ struct RingRng {
     } while (0)
 #define TAUDRAW(u) (F32 ? tau_draw_f(u) : -log(u))
 
+// The workgroup's share of model mi's pass: its lanes take the model's
+// packets until none is left, then the workgroup's tallies and counters are
+// flushed.  stamp: the model's first workgroup (it starts the timeline).
 // F32: the direction arithmetic in single precision (precision mode MCX_FP32)
 template <int MODE, bool F32>
-__global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
+__device__ __forceinline__ void transport_model(const TransportArgs &A, const int mi, const bool stamp) {
     extern __shared__ double lds[];
-    const BatchModel &Bm = A.bm[blockIdx.x / A.bpm];
+    const BatchModel &Bm = A.bm[mi];
     const DevTables &tb = Bm.tb;
     const DevModel &md = Bm.md;
     const DevTally &ty = Bm.ty;
@@ -642,7 +647,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     __shared__ double stau;
     for (int k = threadIdx.x; k < C_NCOUNT; k += blockDim.x) scnt[k] = 0ull;
     if (threadIdx.x == 0) stau = 0.0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) ty.counters[G_TLINE] = wall_clock64();
+    if (threadIdx.x == 0 && stamp) ty.counters[G_TLINE] = wall_clock64();
     __syncthreads();
 #define CNT(k) atomicAdd(&scnt[k], 1ull)
     double tautot = 0.0;
@@ -1660,6 +1665,7 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
     for (int k = threadIdx.x; k < C_NCOUNT; k += blockDim.x)
         if (scnt[k]) atomicAdd(&ty.counters[k], scnt[k]);
     if (threadIdx.x == 0 && stau != 0.0) atomicAdd(ty.tautot, stau);
+    __syncthreads();
 #undef CNT
 #undef P_Q
 #undef P_U
@@ -1684,6 +1690,55 @@ __global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
 #undef P_W0
 #undef P_W1
 #undef P_W2
+}
+
+// One launch for the pass of every model: workgroup b starts on its model
+// (the block map, or b / bpm).  With help on, a workgroup whose model has no
+// packet left to take joins the model with the most packets not yet taken
+// (its own tables in LDS, its lanes' visit tables cleared), until every
+// packet of every model is taken.  A packet's result does not depend on the
+// lane or workgroup that runs it.
+// (HELP: the instance for batched launches; the loop costs registers, so a
+// single model's pass runs the instance without it.)
+template <int MODE, bool F32, bool HELP>
+__global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
+    int mi = A.bmap ? A.bmap[blockIdx.x] : (int)(blockIdx.x / A.bpm);
+    bool stamp = A.bmap ? (blockIdx.x == 0 || A.bmap[blockIdx.x - 1] != mi) : (blockIdx.x % A.bpm == 0);
+    if constexpr (!HELP) {
+        transport_model<MODE, F32>(A, mi, stamp);
+        return;
+    }
+    __shared__ int s_next;
+    for (;;) {
+        transport_model<MODE, F32>(A, mi, stamp);
+        if (threadIdx.x == 0) {
+            long long best = 0;
+            int bm = -1;
+            // past the time limit every packet is given up: no more joining
+            const bool late = A.deadline && wall_clock64() > A.deadline;
+            for (int m = 0; m < A.nmodels && !late; m++) {
+                const unsigned long long c =
+                    __hip_atomic_load(A.bm[m].next_pid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const long long left = A.pid_end - (long long)c;
+                if (left > best) {
+                    best = left;
+                    bm = m;
+                }
+            }
+            s_next = bm;
+        }
+        __syncthreads();
+        mi = s_next;
+        __syncthreads();
+        if (mi < 0) return;
+        stamp = false;
+        // the visit tables are tagged by packet number: clear this
+        // workgroup's lanes' tables before packets of another model
+        uint4 *vb = A.bm[mi].ty.vtab + ((size_t)blockIdx.x * blockDim.x << VIS_LOG2);
+        const size_t nv = (size_t)blockDim.x << VIS_LOG2;
+        for (size_t k = threadIdx.x; k < nv; k += blockDim.x) vb[k] = make_uint4(0u, 0u, 0u, 0u);
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1895,6 +1950,7 @@ extern "C" void mcx_destroy(mcx_ctx *ctx) {
     free_list(ctx->mod_alloc);
     free_list(ctx->tab_alloc);
     if (ctx->d_bms) (void)hipFree(ctx->d_bms);
+    if (ctx->d_bmap) (void)hipFree(ctx->d_bmap);
     if (ctx->gexec) (void)hipGraphExecDestroy((hipGraphExec_t)ctx->gexec);
     if (ctx->graph) (void)hipGraphDestroy((hipGraph_t)ctx->graph);
     release_blas(ctx);
@@ -2447,11 +2503,23 @@ static int pass_status(mcx_ctx *ctx) {
     return MCX_OK;
 }
 
+// Whether the workgroups of a batched launch move to other models once their
+// own has no packet left (on, unless the environment's MCX_BATCH_HELP is 0).
+static int batch_help(int nm) {
+    static const bool off = [] {
+        const char *e = getenv("MCX_BATCH_HELP");
+        return e && e[0] == '0';
+    }();
+    return (nm > 1 && !off) ? 1 : 0;
+}
+
 // Launch shape of a pass.
 struct LaunchCfg {
     void (*kern)(TransportArgs);
     int threads, rb, bpm;
     size_t lds, lanes;
+    int blocks;       // workgroups of a batched launch with a block map
+    const int *bmap;  // [blocks] model of each workgroup, or null (bpm each)
 };
 
 // The pass's device work after the descriptors are in place: set-up, the
@@ -2470,6 +2538,8 @@ static void enqueue_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, const
     TransportArgs A;
     A.bm = dbm;
     A.bpm = L.bpm;
+    A.bmap = L.bmap;
+    A.nmodels = nm;
     A.nphot = run->nphot;
     A.maxinteract = run->maxinteract;
     A.deadline = 0ull;
@@ -2484,7 +2554,8 @@ static void enqueue_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, const
     A.pid_base = 0;
     A.pid_end = run->nphot;
     const int blocks = (nm == 1) ? (int)std::max(1ll, std::min((long long)L.bpm, ((long long)run->nphot + L.threads - 1) / L.threads))
-                                 : L.bpm * nm;
+                       : L.bmap ? L.blocks
+                                : L.bpm * nm;
     if (nm == 1) A.bpm = blocks;
     hipLaunchKernelGGL(L.kern, dim3(blocks), dim3(L.threads), L.lds, s, A);
     for (int m = 0; m < nm; m++)
@@ -2501,14 +2572,17 @@ static int graph_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, const La
     mcx_ctx *ctx = ctxs[0];
     hipStream_t s = ctx->stream;
     char kb[256];
-    snprintf(kb, sizeof(kb), "%d %lld %d %d %.17g %d %d %lld %d %d %d %d %zu %zu", nm, (long long)run->nphot, L.bpm,
-             run->randomwalk, run->factRW, run->precision, run->maxinteract, (long long)run->stage0, run->stage_ratio,
-             run->coop_lanes, L.threads, L.rb, L.lds, L.lanes);
+    snprintf(kb, sizeof(kb), "%d %lld %d %d %.17g %d %d %lld %d %d %d %d %zu %zu %d %p", nm, (long long)run->nphot,
+             L.bpm, run->randomwalk, run->factRW, run->precision, run->maxinteract, (long long)run->stage0,
+             run->stage_ratio, run->coop_lanes, L.threads, L.rb, L.lds, L.lanes, L.bmap ? L.blocks : 0,
+             (const void *)L.bmap);
     std::string key(kb);
     // the descriptors carry every model buffer's address: the graph depends
-    // on the models' sizes, the descriptors' and the visit tables' places
+    // on the models' sizes and on the places of the descriptors, the visit
+    // tables and the stage table (which mcx_set_model reallocates)
     key += " " + std::to_string((unsigned long long)(uintptr_t)ctx->d_bms) + " " +
-           std::to_string((unsigned long long)(uintptr_t)ctx->ty.vtab);
+           std::to_string((unsigned long long)(uintptr_t)ctx->ty.vtab) + " " +
+           std::to_string((unsigned long long)(uintptr_t)ctx->d_stages);
     for (int m = 0; m < nm; m++)
         key += " " + std::to_string(ctxs[m]->md.nr) + "x" + std::to_string(ctxs[m]->md.nt) + "x" +
                std::to_string(ctxs[m]->nlam) + (ctxs[m]->d_thick ? "t" : "");
@@ -2555,8 +2629,8 @@ static int graph_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, const La
 // One pass of nm models in common launches (one per stage), on the first
 // model's stream: bpm blocks per model (0: the device's resident blocks
 // shared out), the staged schedule or one lane.
-static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_req, int64_t seed_stride,
-                    int use_graph, int32_t *status) {
+static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_req, const int32_t *mblocks,
+                    int64_t seed_stride, int use_graph, int32_t *status) {
     mcx_ctx *ctx = ctxs[0];
     if (run->nphot <= 0 || run->nphot >= (1ll << 31)) return fail(ctx, MCX_EINVAL, "nphot must be in 1 .. 2^31-1");
     if (run->precision != MCX_FP64 && run->precision != MCX_FP32) return fail(ctx, MCX_EINVAL, "unknown precision mode");
@@ -2589,10 +2663,20 @@ static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_re
     }
     anyrw = run->randomwalk != 0;
     void (*kern)(TransportArgs);
-    if (run->precision == MCX_FP32)
-        kern = (mode == 0) ? k_transport<0, true> : (mode == 1) ? k_transport<1, true> : k_transport<2, true>;
-    else
-        kern = (mode == 0) ? k_transport<0, false> : (mode == 1) ? k_transport<1, false> : k_transport<2, false>;
+    if (batch_help(serial ? 1 : nm)) {
+        if (run->precision == MCX_FP32)
+            kern = (mode == 0)   ? k_transport<0, true, true>
+                   : (mode == 1) ? k_transport<1, true, true>
+                                 : k_transport<2, true, true>;
+        else
+            kern = (mode == 0)   ? k_transport<0, false, true>
+                   : (mode == 1) ? k_transport<1, false, true>
+                                 : k_transport<2, false, true>;
+    } else if (run->precision == MCX_FP32) {
+        kern = (mode == 0) ? k_transport<0, true, false> : (mode == 1) ? k_transport<1, true, false> : k_transport<2, true, false>;
+    } else {
+        kern = (mode == 0) ? k_transport<0, false, false> : (mode == 1) ? k_transport<1, false, false> : k_transport<2, false, false>;
+    }
     // launch shape: a full device of resident wavefronts, or one lane
     int threads = serial ? 64 : 256; // one wavefront: lane 0 traces, the others serve its lookups
     size_t lds = sizeof(double) * ((size_t)(nrmax + 1) + 2 * (size_t)(ntmax + 1) + 2 * 181 + (size_t)(nlmax - 1) +
@@ -2615,8 +2699,19 @@ static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_re
         if (run->nlanes > 1) resident = std::max(1, std::min(resident, (run->nlanes + threads - 1) / threads));
     }
     const int bpm = serial ? 1 : (bpm_req > 0 ? bpm_req : std::max(1, resident / nm));
+    // an explicit share of the workgroups per model: the block map
+    long long nblk = (long long)bpm * nm;
+    if (mblocks) {
+        if (serial) return fail(ctx, MCX_EINVAL, "a one-lane pass runs one model");
+        nblk = 0;
+        for (int m = 0; m < nm; m++) {
+            if (mblocks[m] < 1) return fail(ctx, MCX_EINVAL, "model " + std::to_string(m) + ": blocks must be >= 1");
+            nblk += mblocks[m];
+        }
+        if (nblk > (1ll << 20)) return fail(ctx, MCX_EINVAL, "more than 2^20 workgroups in one launch");
+    }
     // the lanes' visit tables (cleared per pass: tags are packet numbers)
-    const size_t lanes = (size_t)bpm * nm * threads;
+    const size_t lanes = (size_t)nblk * threads;
     if (lanes > ctx->vtab_lanes) {
         HIPCHK(ctx, hipStreamSynchronize(s));
         free_list(ctx->run_alloc);
@@ -2648,9 +2743,26 @@ static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_re
     // the descriptors hold each model's key: a graph replays with the keys
     // of this call
     HIPCHK(ctx, hipMemcpy(ctx->d_bms, hb.data(), sizeof(BatchModel) * (size_t)nm, hipMemcpyHostToDevice));
+    const int *dmap = nullptr;
+    if (mblocks) {
+        std::vector<int> hmap((size_t)nblk);
+        for (int m = 0, b = 0; m < nm; m++)
+            for (int k = 0; k < mblocks[m]; k++) hmap[b++] = m;
+        if ((size_t)nblk > ctx->bmap_cap) {
+            HIPCHK(ctx, hipStreamSynchronize(s));
+            if (ctx->d_bmap) (void)hipFree(ctx->d_bmap);
+            ctx->d_bmap = nullptr;
+            ctx->bmap_cap = 0;
+            if (hipMalloc(&ctx->d_bmap, sizeof(int) * (size_t)nblk) != hipSuccess)
+                return fail(ctx, MCX_ENOMEM, "block map");
+            ctx->bmap_cap = (size_t)nblk;
+        }
+        HIPCHK(ctx, hipMemcpy(ctx->d_bmap, hmap.data(), sizeof(int) * (size_t)nblk, hipMemcpyHostToDevice));
+        dmap = (const int *)ctx->d_bmap;
+    }
     if (use_graph) {
         if (serial) return fail(ctx, MCX_EINVAL, "graph replay runs concurrent passes only");
-        LaunchCfg L{kern, threads, rb, bpm, lds, lanes};
+        LaunchCfg L{kern, threads, rb, bpm, lds, lanes, (int)nblk, dmap};
         return graph_pass(ctxs, nm, run, L, hb, status);
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
@@ -2662,6 +2774,8 @@ static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_re
     TransportArgs A;
     A.bm = (const BatchModel *)ctx->d_bms;
     A.bpm = bpm;
+    A.bmap = dmap;
+    A.nmodels = nm;
     A.nphot = run->nphot;
     A.maxinteract = run->maxinteract;
     A.deadline = 0ull;
@@ -2690,6 +2804,7 @@ static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_re
     A.pid_end = run->nphot;
     const int blocks = serial ? 1
                        : (nm == 1) ? (int)std::max(1ll, std::min((long long)bpm, ((long long)run->nphot + threads - 1) / threads))
+                       : dmap      ? (int)nblk
                                    : bpm * nm;
     if (nm == 1) A.bpm = blocks;
     HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
@@ -2745,14 +2860,21 @@ static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_re
 
 extern "C" int mcx_transport(mcx_ctx *ctx, const mcx_run *run) {
     if (!ctx || !run) return MCX_EINVAL;
-    return run_pass(&ctx, 1, run, 0, 0, 0, nullptr);
+    return run_pass(&ctx, 1, run, 0, nullptr, 0, 0, nullptr);
 }
 
 extern "C" int mcx_batch_transport(mcx_ctx *const *ctxs, int32_t nmodels, const mcx_run *run,
                                    int32_t blocks_per_model, int64_t seed_stride, int32_t use_graph,
                                    int32_t *status) {
     if (!ctxs || !run || nmodels < 1 || !ctxs[0]) return MCX_EINVAL;
-    return run_pass(ctxs, nmodels, run, blocks_per_model, seed_stride, use_graph, status);
+    return run_pass(ctxs, nmodels, run, blocks_per_model, nullptr, seed_stride, use_graph, status);
+}
+
+extern "C" int mcx_batch_transport_blocks(mcx_ctx *const *ctxs, int32_t nmodels, const mcx_run *run,
+                                          const int32_t *model_blocks, int64_t seed_stride, int32_t use_graph,
+                                          int32_t *status) {
+    if (!ctxs || !run || nmodels < 1 || !ctxs[0] || !model_blocks) return MCX_EINVAL;
+    return run_pass(ctxs, nmodels, run, 0, model_blocks, seed_stride, use_graph, status);
 }
 
 extern "C" int mcx_get_tallies(mcx_ctx *ctx, mcx_tallies *o) {

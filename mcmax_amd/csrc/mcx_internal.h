@@ -274,6 +274,8 @@ struct mcx_ctx {
     unsigned long long *d_next = nullptr;
     void *d_bms = nullptr;      // launch descriptors of the models of a pass run from this context
     size_t bm_cap = 0;
+    void *d_bmap = nullptr;     // the model of each workgroup (explicit shares of a batched launch)
+    size_t bmap_cap = 0;
     void *graph = nullptr, *gexec = nullptr; // captured pass (mcx_batch_transport with a graph)
     std::string graph_key;
     double unitE = 0.0, E0 = 0.0;

@@ -14,7 +14,7 @@ import time
 
 import numpy as np
 
-from . import configs, iterate, transport
+from . import autotune, configs, iterate, transport
 
 # the parameter ranges of the sweep (uniform unless noted)
 RANGES = dict(nrad=(60, 240), ntheta=(40, 120), mdust=(1e-6, 1e-2, "log"), rin=(0.05, 1.0, "log"),
@@ -95,29 +95,63 @@ class Sweep:
 
     maxinteract = 0  # > 0: the reference's interaction roulette (keyword maxinteract)
 
-    def run(self, niter=3, seed=1, graph=True, blocks_per_model=0, seed_stride=1):
+    @staticmethod
+    def shares(work, total, minimum=2):
+        """Workgroups per model in proportion to its work (at least minimum
+        each), summing to total."""
+        work = np.maximum(np.asarray(work, dtype=float), 1.0)
+        n = len(work)
+        if total < minimum * n:
+            raise ValueError("fewer workgroups than the minimum shares")
+        extra = total - minimum * n
+        raw = extra * work / work.sum()
+        out = minimum + np.floor(raw).astype(np.int64)
+        left = total - int(out.sum())
+        out[np.argsort(-(raw - np.floor(raw)))[:left]] += 1
+        return out.astype(np.int32)
+
+    def run(self, niter=3, seed=1, graph=True, blocks_per_model=0, seed_stride=1, balance=True):
         """niter structure iterations of every model, each pass batched.
-        Returns dict(T: last temperatures, transport_s, total_s)."""
+        balance: from the second pass on, each model's share of the launch
+        follows the lane time it took in the pass before (its workgroups
+        times its finish; the total stays that of the first pass,
+        blocks_per_model each, 0: the device's resident workgroups shared
+        out).  Returns dict(T: last temperatures,
+        transport_s, total_s, shares, finish_ms: per pass the models'
+        finish times)."""
         states = [iterate.density_state(m) for m in self.models]
         t_tr, t0 = 0.0, time.perf_counter()
         T = [None] * len(self.models)
+        shares, hist, finish = None, [], []
         for it in range(1, niter + 1):
             t1 = time.perf_counter()
             st = transport.batch_transport(self.trs, self.nphot, seed=seed + 1000 * it, seed_stride=seed_stride,
                                            blocks_per_model=blocks_per_model, graph=graph,
-                                           maxinteract=self.maxinteract)
+                                           maxinteract=self.maxinteract, model_blocks=shares)
             if np.any(st != 0):
                 raise RuntimeError(f"batched pass: status {st}")
             t_tr += time.perf_counter() - t1
+            tals = [t.tallies() for t in self.trs]
+            fin = [tl["t_done_ms"][4] for tl in tals]
+            finish.append(fin)
+            hist.append(None if shares is None else shares.tolist())
             if self.log:
-                self.log(f"iteration {it}: batched pass of {len(self.models)} models {time.perf_counter() - t1:.1f} s")
+                self.log(f"iteration {it}: batched pass of {len(self.models)} models {time.perf_counter() - t1:.1f} s "
+                         f"(models done after {min(fin) / 1e3:.1f} s median {np.median(fin) / 1e3:.1f} s last "
+                         f"{max(fin) / 1e3:.1f} s)")
+            if balance and it < niter:
+                # the lane time each model took: its workgroups x its finish
+                bpm = blocks_per_model or max(1, autotune.RESIDENT // len(self.models))
+                prev = np.full(len(self.models), bpm) if shares is None else shares
+                shares = self.shares(prev * np.asarray(fin), int(prev.sum()), minimum=min(2, bpm))
             for k in range(len(self.models)):
                 T[k] = self._after_pass(k, states, it)
                 if self.log and (k + 1) % 64 == 0:
                     self.log(f"iteration {it}: {k + 1} models' structure solved")
             if self.log:
                 self.log(f"iteration {it}: {time.perf_counter() - t0:.1f} s")
-        return dict(T=T, transport_s=t_tr, total_s=time.perf_counter() - t0)
+        return dict(T=T, transport_s=t_tr, total_s=time.perf_counter() - t0, shares=hist, finish_ms=finish)
+
 
     def run_one(self, k, niter=3, seed=1, seed_stride=1):
         """model k alone, its passes model by model (mcx_transport) with the

@@ -302,12 +302,15 @@ class Transport:
 
 
 def batch_transport(trs, nphot, seed=1, seed_stride=1, blocks_per_model=0, randomwalk=None, factRW=None,
-                    maxinteract=0, precision=0, max_ms=0.0, stage0=0, stage_ratio=0, graph=False, epoch_deposits=0):
+                    maxinteract=0, precision=0, max_ms=0.0, stage0=0, stage_ratio=0, graph=False, epoch_deposits=0,
+                    model_blocks=None):
     """One pass of nphot packets for every model of the Transport objects
     trs (one model each) in common launches (mcx_batch_transport); model m
     uses the key seed + m seed_stride.  graph=True: captured once, then
-    replayed (mcx_batch_transport's use_graph).  Returns the per-model
-    status codes; the tallies stay on each model's context."""
+    replayed (mcx_batch_transport's use_graph).  model_blocks: each model's
+    workgroups (mcx_batch_transport_blocks) instead of blocks_per_model for
+    all.  Returns the per-model status codes; the tallies stay on each
+    model's context."""
     lib = trs[0].lib
     m0 = trs[0].model
     rw = bool(getattr(m0, "randomwalk", False)) if randomwalk is None else bool(randomwalk)
@@ -317,8 +320,15 @@ def batch_transport(trs, nphot, seed=1, seed_stride=1, blocks_per_model=0, rando
                  max_ms=float(max_ms), epoch_deposits=int(epoch_deposits))
     arr = (C.c_void_p * len(trs))(*[t.ctx for t in trs])
     st = np.zeros(len(trs), dtype=np.int32)
-    rc = lib.mcx_batch_transport(arr, len(trs), C.byref(r), int(blocks_per_model), int(seed_stride), int(bool(graph)),
-                                 st.ctypes.data_as(C.POINTER(C.c_int32)))
+    if model_blocks is not None:
+        mb = np.ascontiguousarray(model_blocks, dtype=np.int32)
+        if mb.shape != (len(trs),):
+            raise ValueError("model_blocks: one count per model")
+        rc = lib.mcx_batch_transport_blocks(arr, len(trs), C.byref(r), mb.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            int(seed_stride), int(bool(graph)), st.ctypes.data_as(C.POINTER(C.c_int32)))
+    else:
+        rc = lib.mcx_batch_transport(arr, len(trs), C.byref(r), int(blocks_per_model), int(seed_stride),
+                                     int(bool(graph)), st.ctypes.data_as(C.POINTER(C.c_int32)))
     if rc not in (0, _lib.MCX_EGEOM):
         raise MCXError(rc, lib.mcx_last_error(trs[0].ctx).decode())
     return st

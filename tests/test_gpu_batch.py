@@ -22,8 +22,9 @@ def models(workdir):
     return [configs.load_config("tiny", f"{workdir}/batch{k}", **sp) for k, sp in enumerate(specs)]
 
 
-@pytest.mark.parametrize("bpm", [1, 3])
+@pytest.mark.parametrize("bpm", [1, 3, (5, 1, 2, 3)])
 def test_batch_equals_single(models, bpm):
+    """bpm: workgroups per model, or each model's own share (block map)."""
     trs = []
     try:
         for m in models:
@@ -31,10 +32,13 @@ def test_batch_equals_single(models, bpm):
             t.set_model(m)
             trs.append(t)
         n, seed, stride = 20000, 77, 11
-        st = transport.batch_transport(trs, n, seed=seed, seed_stride=stride, blocks_per_model=bpm,
-                                       randomwalk=True, factRW=2.0, max_ms=120000.0)
+        shares = dict(model_blocks=list(bpm)) if isinstance(bpm, tuple) else dict(blocks_per_model=bpm)
+        st = transport.batch_transport(trs, n, seed=seed, seed_stride=stride, randomwalk=True, factRW=2.0,
+                                       max_ms=120000.0, **shares)
         assert np.all(st == 0), st
         batch = [t.tallies() for t in trs]
+        # every model's timeline starts with its own first workgroup
+        assert all(0.0 < b["t_done_ms"][4] < 120000.0 for b in batch), [b["t_done_ms"] for b in batch]
         Tb = [t.temperature()[0] for t in trs]
         for k, (t, m) in enumerate(zip(trs, models)):
             t.transport(n, seed=seed + stride * k, randomwalk=True, factRW=2.0, max_ms=120000.0)
@@ -54,10 +58,13 @@ def test_batch_equals_single(models, bpm):
 
 
 def test_batch_graph_replays(models):
-    """The pass captured in a HIP graph and replayed: three replays with
-    different keys, eager passes and synchronisations between them, and a new
-    density for one model (same grid) before the last; every replay equals
-    the eager batched pass with the same keys, bit for bit."""
+    """The pass captured in a HIP graph and replayed: five replays with
+    different keys, eager passes and synchronisations between them, a new
+    density for one model (same grid) before the third, the first model
+    (whose context holds the stage table) set up again without class tables
+    (its buffers move), then two replays with explicit shares of the
+    workgroups per model that change between them; every replay equals the
+    eager batched pass with the same keys, bit for bit."""
     trs = []
     try:
         for m in models[:3]:
@@ -66,13 +73,19 @@ def test_batch_graph_replays(models):
             trs.append(t)
         n = 12000
         kw = dict(seed_stride=5, blocks_per_model=2, randomwalk=True, factRW=2.0)
-        for rep, seed in enumerate((101, 202, 303)):
+        for rep, seed in enumerate((101, 202, 303, 404, 505)):
+            if rep >= 3:
+                # each model's own share; the total stays 6, the shares move
+                kw = dict(seed_stride=5, model_blocks=[(3, 2, 1), (1, 1, 4)][rep - 3], randomwalk=True,
+                          factRW=2.0)
             if rep == 2:
                 m2 = models[1]
                 dens = m2.dens.copy()
                 m2.dens = dens * 1.5
                 trs[1].set_model(m2, tables=False)
                 m2.dens = dens
+                trs[0].class_table_bytes = 0
+                trs[0].set_model(models[0], tables=False)
             st = transport.batch_transport(trs, n, seed=seed, graph=True, **kw)
             assert np.all(st == 0), st
             got = [t.tallies() for t in trs]
