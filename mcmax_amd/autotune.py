@@ -29,7 +29,7 @@ def _key(trs, nphot, maxinteract=0):
     return f"{len(trs)}:{nphot}:{maxinteract}:{h.hexdigest()[:16]}"
 
 
-RESIDENT = 2 * 256  # resident transport workgroups: two per CU on the 256 CUs
+RESIDENT = 256  # resident transport workgroups: one per CU on the 256 CUs
 
 
 def tune(trs, nphot, candidates=None, probe_nphot=None, randomwalk=None, log=None, use_cache=True, maxinteract=0):

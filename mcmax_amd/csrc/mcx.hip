@@ -1699,9 +1699,13 @@ __device__ __forceinline__ void transport_model(const TransportArgs &A, const in
 // packet of every model is taken.  A packet's result does not depend on the
 // lane or workgroup that runs it.
 // (HELP: the instance for batched launches; the loop costs registers, so a
-// single model's pass runs the instance without it.)
+// single model's pass runs the instance without it.)  One workgroup of four
+// waves per CU: the packet state needs more than the 256 VGPRs of two waves
+// per SIMD, and at one wave per SIMD the rest lives in AGPRs instead of
+// scratch (measured: the production pass 10% and a lone lane's step 5%
+// faster than at two waves per SIMD with ~35 VGPRs spilled).
 template <int MODE, bool F32, bool HELP>
-__global__ void __launch_bounds__(256, 2) k_transport(TransportArgs A) {
+__global__ void __launch_bounds__(256, 1) k_transport(TransportArgs A) {
     int mi = A.bmap ? A.bmap[blockIdx.x] : (int)(blockIdx.x / A.bpm);
     bool stamp = A.bmap ? (blockIdx.x == 0 || A.bmap[blockIdx.x - 1] != mi) : (blockIdx.x % A.bpm == 0);
     if constexpr (!HELP) {
@@ -2682,11 +2686,10 @@ static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_re
     size_t lds = sizeof(double) * ((size_t)(nrmax + 1) + 2 * (size_t)(ntmax + 1) + 2 * 181 + (size_t)(nlmax - 1) +
                                    NPL * (size_t)threads);
     if (anyrw) lds += sizeof(double) * ((size_t)(nrmax + 1) + (size_t)(ntmax + 1));
-    // rings of Philox draws: 16 per thread when they fit beside the rest
-    // with two workgroups per CU (half the CU's 160 KiB of LDS), else fewer
+    // rings of Philox draws: 16 per thread when they fit beside the rest in
+    // the CU's LDS (one workgroup per CU), else fewer
     int rb = 16;
-    const size_t lds_two = std::min(ctx->lds_max, (size_t)80 * 1024);
-    while (rb > 4 && lds + sizeof(double) * (size_t)rb * threads > (serial ? ctx->lds_max : lds_two)) rb >>= 1;
+    while (rb > 4 && lds + sizeof(double) * (size_t)rb * threads > ctx->lds_max) rb >>= 1;
     lds += sizeof(double) * (size_t)rb * threads;
     if (lds > ctx->lds_max)
         return fail(ctx, MCX_EINVAL, "grid too large for the kernel's LDS tables (" + std::to_string(lds) + " bytes)");
