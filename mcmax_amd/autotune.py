@@ -29,18 +29,22 @@ def _key(trs, nphot, maxinteract=0):
     return f"{len(trs)}:{nphot}:{maxinteract}:{h.hexdigest()[:16]}"
 
 
-RESIDENT = 256  # resident transport workgroups: one per CU on the 256 CUs
+RESIDENT = 2 * 256  # resident workgroups of a batched launch: two per CU on the 256 CUs
 
 
 def tune(trs, nphot, candidates=None, probe_nphot=None, randomwalk=None, log=None, use_cache=True, maxinteract=0):
     """blocks per model for a batched pass of nphot packets over the models
     of trs: a probe pass of probe_nphot packets (default nphot/16, at least
-    4096) per candidate, the fastest kept."""
+    4096) per candidate; the fewest workgroups within 3% of the fastest
+    kept."""
     nm = len(trs)
     resident = RESIDENT
     if candidates is None:
+        # launches that fill the device: the resident workgroups shared out,
+        # or twice as many (the later ones start as the first retire); fewer
+        # would leave CUs idle once the probe's short tails are gone
         base = max(1, resident // nm)
-        candidates = sorted({max(1, base // 2), base, base * 2})
+        candidates = [base, 2 * base]
     key = _key(trs, nphot, maxinteract)
     path = _cache_path()
     cache = {}
@@ -61,7 +65,10 @@ def tune(trs, nphot, candidates=None, probe_nphot=None, randomwalk=None, log=Non
         times[bpm] = time.perf_counter() - t0
         if log:
             log(f"autotune: {bpm} blocks per model: {times[bpm]:.2f} s")
-    best = min(times, key=times.get)
+    # a probe is short and its time noisy: the fewest workgroups within 3%
+    # of the fastest
+    fastest = min(times.values())
+    best = min(k for k, v in times.items() if v <= 1.03 * fastest)
     rec = {"blocks_per_model": best, "probe_nphot": probe, "seconds": {str(k): v for k, v in times.items()}}
     cache[key] = rec
     if use_cache:

@@ -570,10 +570,10 @@ struct RingRng {
 
 // The workgroup's share of model mi's pass: its lanes take the model's
 // packets until none is left, then the workgroup's tallies and counters are
-// flushed.  stamp: the model's first workgroup (it starts the timeline).
+// flushed.  The first workgroup to start on the model starts its timeline.
 // F32: the direction arithmetic in single precision (precision mode MCX_FP32)
 template <int MODE, bool F32>
-__device__ __forceinline__ void transport_model(const TransportArgs &A, const int mi, const bool stamp) {
+__device__ __forceinline__ void transport_model(const TransportArgs &A, const int mi) {
     extern __shared__ double lds[];
     const BatchModel &Bm = A.bm[mi];
     const DevTables &tb = Bm.tb;
@@ -647,7 +647,7 @@ __device__ __forceinline__ void transport_model(const TransportArgs &A, const in
     __shared__ double stau;
     for (int k = threadIdx.x; k < C_NCOUNT; k += blockDim.x) scnt[k] = 0ull;
     if (threadIdx.x == 0) stau = 0.0;
-    if (threadIdx.x == 0 && stamp) ty.counters[G_TLINE] = wall_clock64();
+    if (threadIdx.x == 0) atomicCAS(&ty.counters[G_TLINE], 0ull, wall_clock64());
     __syncthreads();
 #define CNT(k) atomicAdd(&scnt[k], 1ull)
     double tautot = 0.0;
@@ -1699,22 +1699,24 @@ __device__ __forceinline__ void transport_model(const TransportArgs &A, const in
 // packet of every model is taken.  A packet's result does not depend on the
 // lane or workgroup that runs it.
 // (HELP: the instance for batched launches; the loop costs registers, so a
-// single model's pass runs the instance without it.)  One workgroup of four
-// waves per CU: the packet state needs more than the 256 VGPRs of two waves
-// per SIMD, and at one wave per SIMD the rest lives in AGPRs instead of
-// scratch (measured: the production pass 10% and a lone lane's step 5%
-// faster than at two waves per SIMD with ~35 VGPRs spilled).
+// single model's pass runs the instance without it.)  A single model's pass
+// runs one workgroup of four waves per CU: the packet state needs more than
+// the 256 VGPRs of two waves per SIMD, and at one wave per SIMD the rest
+// lives in AGPRs instead of scratch (measured: the production pass 10% and
+// a lone lane's step 5% faster than at two waves per SIMD with ~35 VGPRs
+// spilled).  A batched launch is bound by its lanes' throughput instead
+// (the 64-model sweep: 137 s at two workgroups per CU, 163 s at one), so
+// the HELP instance runs two.
 template <int MODE, bool F32, bool HELP>
-__global__ void __launch_bounds__(256, 1) k_transport(TransportArgs A) {
+__global__ void __launch_bounds__(256, HELP ? 2 : 1) k_transport(TransportArgs A) {
     int mi = A.bmap ? A.bmap[blockIdx.x] : (int)(blockIdx.x / A.bpm);
-    bool stamp = A.bmap ? (blockIdx.x == 0 || A.bmap[blockIdx.x - 1] != mi) : (blockIdx.x % A.bpm == 0);
     if constexpr (!HELP) {
-        transport_model<MODE, F32>(A, mi, stamp);
+        transport_model<MODE, F32>(A, mi);
         return;
     }
     __shared__ int s_next;
     for (;;) {
-        transport_model<MODE, F32>(A, mi, stamp);
+        transport_model<MODE, F32>(A, mi);
         if (threadIdx.x == 0) {
             long long best = 0;
             int bm = -1;
@@ -1735,7 +1737,6 @@ __global__ void __launch_bounds__(256, 1) k_transport(TransportArgs A) {
         mi = s_next;
         __syncthreads();
         if (mi < 0) return;
-        stamp = false;
         // the visit tables are tagged by packet number: clear this
         // workgroup's lanes' tables before packets of another model
         uint4 *vb = A.bm[mi].ty.vtab + ((size_t)blockIdx.x * blockDim.x << VIS_LOG2);
@@ -2687,9 +2688,11 @@ static int run_pass(mcx_ctx *const *ctxs, int nm, const mcx_run *run, int bpm_re
                                    NPL * (size_t)threads);
     if (anyrw) lds += sizeof(double) * ((size_t)(nrmax + 1) + (size_t)(ntmax + 1));
     // rings of Philox draws: 16 per thread when they fit beside the rest in
-    // the CU's LDS (one workgroup per CU), else fewer
+    // the CU's LDS (one workgroup per CU; a batched launch's two workgroups
+    // per CU share it), else fewer
     int rb = 16;
-    while (rb > 4 && lds + sizeof(double) * (size_t)rb * threads > ctx->lds_max) rb >>= 1;
+    const size_t lds_cap = batch_help(serial ? 1 : nm) ? std::min(ctx->lds_max, (size_t)80 * 1024) : ctx->lds_max;
+    while (rb > 4 && lds + sizeof(double) * (size_t)rb * threads > lds_cap) rb >>= 1;
     lds += sizeof(double) * (size_t)rb * threads;
     if (lds > ctx->lds_max)
         return fail(ctx, MCX_EINVAL, "grid too large for the kernel's LDS tables (" + std::to_string(lds) + " bytes)");
