@@ -254,6 +254,49 @@ def converge_bench(a, work):
                             "errT < epsiter twice, decides when the loop stops)"}
 
 
+def converged_small(work, seed=11):
+    """The metric's second half on the CPU-runnable plain-method workload
+    (small model: 100 r x 60 theta cells, 100 wavelengths, one analytic
+    grain, no vertical-structure iteration, 1e5 packets, the modified random
+    walk as in the production pass): wall time of the
+    device's passes (each pass + Lucy inversion, 1e5 packets, doubled until
+    the field is within 1%) until the median |dT|/T against the FP64
+    oracle's converged field (tests/golden/converged_small.npz, 2e6 packets
+    in 8 sub-runs) is at most 1%."""
+    import numpy as np
+    from mcmax_amd import configs, transport
+    path = os.path.join(ROOT, "tests", "golden", "converged_small.npz")
+    if not os.path.exists(path):
+        return None
+    ref = np.load(path)
+    tr = transport.Transport(0)
+    try:
+        t0 = time.perf_counter()
+        m = configs.load_config("small", os.path.join(work, "small"), tr=tr, randomwalk=True)
+        tr.set_model(m)
+        setup_s = time.perf_counter() - t0
+        passes, total, n = [], 0.0, 100000
+        while True:
+            t1 = time.perf_counter()
+            tr.transport(n, seed=seed + len(passes))
+            T, _, _ = tr.temperature()
+            dt = time.perf_counter() - t1
+            total += dt
+            med = float(np.median(np.abs(T / ref["T"] - 1.0)))
+            passes.append({"nphot": n, "wall_s": dt, "median_dT_over_T_vs_oracle": med})
+            if med <= 0.01 or n >= 3200000:
+                break
+            n *= 2
+    finally:
+        tr.close()
+    return {"workload": "small plain-method model (100 r x 60 theta, 100 wavelengths, one analytic grain), "
+                        "modified random walk (factRW=10), passes of 1e5 packets doubled until converged",
+            "reference": f"FP64 oracle field, {int(ref['nphot'])} packets in {int(ref['subruns'])} sub-runs",
+            "criterion": "median |dT|/T <= 1% against the reference", "converged": passes[-1][
+                "median_dT_over_T_vs_oracle"] <= 0.01,
+            "time_s": total, "setup_s": setup_s, "passes": passes}
+
+
 def image_bench(a, work, lam0=1.6):
     """An IMAGE observation with the reference's defaults (determineoutput,
     Telescope.f:590-657: 500 x 500 pixels, 1000 samples per polar cell,
@@ -672,9 +715,10 @@ def main():
         "timeline": {"maxsteps": int(tal["maxsteps"]), "t_done_ms": tal["t_done_ms"],
                      "done_fractions": [0.5, 0.9, 0.99, 0.999, 1.0]},
     }
+    tr.close()
+    out["time_to_converged"] = converged_small(work)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, int(tal["nsteps"]), a.nphot)
-    tr.close()
     print(json.dumps(out))
 
 
