@@ -407,7 +407,8 @@ def sweep_bench(a, work):
     (uneven grids 60-240 x 40-120, dust masses 1e-6..1e-2 Msun, stars
     3000-10000 K), the production dust, --nphot packets per model per
     iteration, 3 structure iterations with every pass batched (graph
-    replay, blocks per model from the autotuner).  Then --sweep-check models
+    replay, blocks per model from the autotuner; with --cohort N the models
+    go N to a launch, launch after launch).  Then --sweep-check models
     run alone, model by model (mcx_transport) with the same keys: their
     final temperatures must equal the batched sweep's bit for bit, and their
     mean time per model gives the model-by-model time of the whole sweep."""
@@ -421,8 +422,9 @@ def sweep_bench(a, work):
     setup_s = time.perf_counter() - t0
     log(f"{a.sweep} models set up in {setup_s:.1f} s")
     init = [(m.dens.copy(), m.w.copy()) for m in S.models]
-    bpm, tune_rec = autotune.tune(S.trs, a.nphot, log=log, maxinteract=a.maxinteract)
-    res = S.run(niter=3, seed=a.seed, graph=True, blocks_per_model=bpm)
+    size = a.cohort if a.cohort and a.cohort < a.sweep else a.sweep
+    bpm, tune_rec = autotune.tune(S.trs[:size], a.nphot, log=log, maxinteract=a.maxinteract)
+    res = S.run(niter=3, seed=a.seed, graph=True, blocks_per_model=bpm, cohort=size)
     log(f"batched sweep: {res['total_s']:.1f} s ({res['transport_s']:.1f} s in the passes)")
     rng = np.random.default_rng(7)
     check = sorted(rng.choice(a.sweep, size=min(a.sweep_check, a.sweep), replace=False).tolist())
@@ -443,7 +445,7 @@ def sweep_bench(a, work):
             "config": {"workload": f"{a.sweep} disk models drawn from the sweep ranges (seed 20261203), production "
                                    f"dust, {a.nphot} packets per model per iteration, 3 iterations, modified random "
                                    f"walk {not a.plain}", "models": a.sweep, "cells_min": min(ncell),
-                       "cells_max": max(ncell), "maxinteract": a.maxinteract},
+                       "cells_max": max(ncell), "maxinteract": a.maxinteract, "models_per_launch": size},
             "packets_per_s": 3 * a.nphot * a.sweep / res["transport_s"],
             "transport_s": res["transport_s"], "setup_s": setup_s, "blocks_per_model": bpm, "autotune": tune_rec,
             "model_by_model_s_per_model": alone, "checked_models": check, "identical_to_alone": identical,
@@ -585,6 +587,8 @@ def main():
                     help="FP32 against FP64 passes with the same seeds (per-cell T and MC SED errors) instead")
     ap.add_argument("--sweep", type=int, default=0,
                     help="time the batched many-model sweep of this many models instead (--nphot per model)")
+    ap.add_argument("--cohort", type=int, default=0,
+                    help="models per batched launch in --sweep (0: all in one)")
     ap.add_argument("--sweep-check", type=int, default=4,
                     help="models of --sweep also run alone (bit-identity and the model-by-model time)")
     ap.add_argument("--atomics", action="store_true",

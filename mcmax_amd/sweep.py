@@ -110,48 +110,58 @@ class Sweep:
         out[np.argsort(-(raw - np.floor(raw)))[:left]] += 1
         return out.astype(np.int32)
 
-    def run(self, niter=3, seed=1, graph=True, blocks_per_model=0, seed_stride=1, balance=True):
-        """niter structure iterations of every model, each pass batched.
-        balance: from the second pass on, each model's share of the launch
-        follows the lane time it took in the pass before (its workgroups
-        times its finish; the total stays that of the first pass,
-        blocks_per_model each, 0: the device's resident workgroups shared
-        out).  Returns dict(T: last temperatures,
-        transport_s, total_s, shares, finish_ms: per pass the models'
-        finish times)."""
+    def run(self, niter=3, seed=1, graph=True, blocks_per_model=0, seed_stride=1, balance=True, cohort=0):
+        """niter structure iterations of every model, each pass batched:
+        the models in cohorts of at most `cohort` per launch (0: all in
+        one), launch after launch.  balance: from the second pass on, each
+        model's share of its launch follows the lane time it took in the
+        pass before (its workgroups times its finish; the launch's total
+        stays that of the first pass, blocks_per_model each, 0: the
+        device's resident workgroups shared out).  Model k's key is
+        seed + 1000 iteration + seed_stride k whatever its cohort.  Returns
+        dict(T: last temperatures, transport_s, total_s, shares, finish_ms:
+        per pass the models' finish times)."""
+        nm = len(self.models)
+        size = cohort if cohort and cohort < nm else nm
+        groups = [list(range(a, min(a + size, nm))) for a in range(0, nm, size)]
         states = [iterate.density_state(m) for m in self.models]
         t_tr, t0 = 0.0, time.perf_counter()
-        T = [None] * len(self.models)
-        shares, hist, finish = None, [], []
+        T = [None] * nm
+        shares = [None] * len(groups)
+        hist, finish = [], []
         for it in range(1, niter + 1):
-            t1 = time.perf_counter()
-            st = transport.batch_transport(self.trs, self.nphot, seed=seed + 1000 * it, seed_stride=seed_stride,
-                                           blocks_per_model=blocks_per_model, graph=graph,
-                                           maxinteract=self.maxinteract, model_blocks=shares)
-            if np.any(st != 0):
-                raise RuntimeError(f"batched pass: status {st}")
-            t_tr += time.perf_counter() - t1
-            tals = [t.tallies() for t in self.trs]
-            fin = [tl["t_done_ms"][4] for tl in tals]
+            fin = [0.0] * nm
+            hist.append([None if sh is None else sh.tolist() for sh in shares])
+            for g, ks in enumerate(groups):
+                t1 = time.perf_counter()
+                trs = [self.trs[k] for k in ks]
+                st = transport.batch_transport(trs, self.nphot, seed=seed + 1000 * it + seed_stride * ks[0],
+                                               seed_stride=seed_stride, blocks_per_model=blocks_per_model,
+                                               graph=graph, maxinteract=self.maxinteract, model_blocks=shares[g])
+                if np.any(st != 0):
+                    raise RuntimeError(f"batched pass: status {st}")
+                t_tr += time.perf_counter() - t1
+                f = [trs[q].tallies()["t_done_ms"][4] for q in range(len(ks))]
+                for q, k in enumerate(ks):
+                    fin[k] = f[q]
+                if self.log:
+                    what = f"cohort {g + 1} of {len(groups)}: " if len(groups) > 1 else ""
+                    self.log(f"iteration {it}: {what}batched pass of {len(ks)} models "
+                             f"{time.perf_counter() - t1:.1f} s (models done after {min(f) / 1e3:.1f} s median "
+                             f"{np.median(f) / 1e3:.1f} s last {max(f) / 1e3:.1f} s)")
+                if balance and it < niter:
+                    # the lane time each model took: its workgroups x its finish
+                    bpm = blocks_per_model or max(1, autotune.RESIDENT // len(ks))
+                    prev = np.full(len(ks), bpm) if shares[g] is None else shares[g]
+                    shares[g] = self.shares(prev * np.asarray(f), int(prev.sum()), minimum=min(2, bpm))
             finish.append(fin)
-            hist.append(None if shares is None else shares.tolist())
-            if self.log:
-                self.log(f"iteration {it}: batched pass of {len(self.models)} models {time.perf_counter() - t1:.1f} s "
-                         f"(models done after {min(fin) / 1e3:.1f} s median {np.median(fin) / 1e3:.1f} s last "
-                         f"{max(fin) / 1e3:.1f} s)")
-            if balance and it < niter:
-                # the lane time each model took: its workgroups x its finish
-                bpm = blocks_per_model or max(1, autotune.RESIDENT // len(self.models))
-                prev = np.full(len(self.models), bpm) if shares is None else shares
-                shares = self.shares(prev * np.asarray(fin), int(prev.sum()), minimum=min(2, bpm))
-            for k in range(len(self.models)):
+            for k in range(nm):
                 T[k] = self._after_pass(k, states, it)
                 if self.log and (k + 1) % 64 == 0:
                     self.log(f"iteration {it}: {k + 1} models' structure solved")
             if self.log:
                 self.log(f"iteration {it}: {time.perf_counter() - t0:.1f} s")
         return dict(T=T, transport_s=t_tr, total_s=time.perf_counter() - t0, shares=hist, finish_ms=finish)
-
 
     def run_one(self, k, niter=3, seed=1, seed_stride=1):
         """model k alone, its passes model by model (mcx_transport) with the
