@@ -423,6 +423,21 @@ def sweep_bench(a, work):
     log(f"{a.sweep} models set up in {setup_s:.1f} s")
     init = [(m.dens.copy(), m.w.copy()) for m in S.models]
     size = a.cohort if a.cohort and a.cohort < a.sweep else a.sweep
+    if a.alone_only:
+        # only the model-by-model timing: --sweep-check models alone
+        rng = np.random.default_rng(11)
+        check = sorted(rng.choice(a.sweep, size=min(a.sweep_check, a.sweep), replace=False).tolist())
+        alone = []
+        for k in check:
+            r = S.run_one(k, niter=3, seed=a.seed)
+            alone.append(r["total_s"])
+            log(f"model {k} alone: {r['total_s']:.1f} s")
+        S.close()
+        return {"metric": f"{a.sweep}-model sweep, model by model: wall time per model (3 structure iterations)",
+                "value": float(np.mean(alone)), "unit": "s", "higher_is_better": False, "n_gpus": 1,
+                "models": a.sweep, "checked_models": check, "model_by_model_s_per_model": alone,
+                "model_by_model_estimate_s": float(np.mean(alone)) * a.sweep, "nphot": a.nphot,
+                "maxinteract": a.maxinteract}
     bpm, tune_rec = autotune.tune(S.trs[:size], a.nphot, log=log, maxinteract=a.maxinteract)
     res = S.run(niter=3, seed=a.seed, graph=True, blocks_per_model=bpm, cohort=size)
     log(f"batched sweep: {res['total_s']:.1f} s ({res['transport_s']:.1f} s in the passes)")
@@ -587,6 +602,8 @@ def main():
                     help="FP32 against FP64 passes with the same seeds (per-cell T and MC SED errors) instead")
     ap.add_argument("--sweep", type=int, default=0,
                     help="time the batched many-model sweep of this many models instead (--nphot per model)")
+    ap.add_argument("--alone-only", action="store_true",
+                    help="with --sweep: only time --sweep-check models alone (model by model), no batched sweep")
     ap.add_argument("--cohort", type=int, default=0,
                     help="models per batched launch in --sweep (0: all in one)")
     ap.add_argument("--sweep-check", type=int, default=4,
