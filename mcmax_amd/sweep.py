@@ -110,6 +110,20 @@ class Sweep:
         out[np.argsort(-(raw - np.floor(raw)))[:left]] += 1
         return out.astype(np.int32)
 
+    @staticmethod
+    def cohorts(nm, size):
+        """The models' launches: consecutive groups of at most size models
+        (0: one launch for all)."""
+        size = size if size and size < nm else nm
+        return [list(range(a, min(a + size, nm))) for a in range(0, nm, size)]
+
+    @staticmethod
+    def cohort_key(seed, it, seed_stride, first):
+        """The base key of a launch whose first model is `first`: its model q
+        then draws with seed + 1000 it + seed_stride (first + q), the key
+        the model has whatever its launch (and alone, run_one)."""
+        return seed + 1000 * it + seed_stride * first
+
     def run(self, niter=3, seed=1, graph=True, blocks_per_model=0, seed_stride=1, balance=True, cohort=0):
         """niter structure iterations of every model, each pass batched:
         the models in cohorts of at most `cohort` per launch (0: all in
@@ -122,8 +136,7 @@ class Sweep:
         dict(T: last temperatures, transport_s, total_s, shares, finish_ms:
         per pass the models' finish times)."""
         nm = len(self.models)
-        size = cohort if cohort and cohort < nm else nm
-        groups = [list(range(a, min(a + size, nm))) for a in range(0, nm, size)]
+        groups = self.cohorts(nm, cohort)
         states = [iterate.density_state(m) for m in self.models]
         t_tr, t0 = 0.0, time.perf_counter()
         T = [None] * nm
@@ -135,7 +148,7 @@ class Sweep:
             for g, ks in enumerate(groups):
                 t1 = time.perf_counter()
                 trs = [self.trs[k] for k in ks]
-                st = transport.batch_transport(trs, self.nphot, seed=seed + 1000 * it + seed_stride * ks[0],
+                st = transport.batch_transport(trs, self.nphot, seed=self.cohort_key(seed, it, seed_stride, ks[0]),
                                                seed_stride=seed_stride, blocks_per_model=blocks_per_model,
                                                graph=graph, maxinteract=self.maxinteract, model_blocks=shares[g])
                 if np.any(st != 0):

@@ -31,3 +31,24 @@ def test_shares_too_few():
     with pytest.raises(ValueError):
         Sweep.shares([1, 2, 3], 5)
 
+
+
+def test_cohorts_cover_every_model_once():
+    for nm in (1, 5, 128, 256, 257):
+        for size in (0, 1, 7, 128, 300):
+            groups = Sweep.cohorts(nm, size)
+            flat = [k for g in groups for k in g]
+            assert flat == list(range(nm))
+            cap = size if size and size < nm else nm
+            assert all(1 <= len(g) <= cap for g in groups)
+    assert Sweep.cohorts(256, 128) == [list(range(128)), list(range(128, 256))]
+
+
+def test_cohort_keys_match_model_keys():
+    # model k of launch g draws with the launch's base key + stride * q,
+    # q its place in the launch: the key run_one gives it alone
+    seed, stride, it = 17, 3, 2
+    for g in Sweep.cohorts(10, 4):
+        base = Sweep.cohort_key(seed, it, stride, g[0])
+        for q, k in enumerate(g):
+            assert base + stride * q == seed + 1000 * it + stride * k
