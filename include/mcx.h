@@ -203,14 +203,17 @@ int mcx_set_tables(mcx_ctx *ctx, const mcx_tables *tab);
 
 /* Replaces the reads of D%R, D%Theta, C(i,j)%dens/V/w (MCRadiation.f:901,
  * 1150).  Builds KabsTot/KscaTot (MCRadiation.f:210-229) and, per distinct
- * species mix, the Planck-mean table sum_s w_s Kp_s(T) (computeE,
- * Radiation.f:611-627) and with several species the reemission cumulative
- * sum_s w_s CUM_s (16 MB per mix for 200 wavelengths).  The cumulatives are
- * built while they fit in 4 GiB, or in the bytes the environment variable
- * MCX_CLASS_TABLE_BYTES gives (read at each call; many contexts on one GPU
- * set it low); otherwise the pass sums the species at every probe, which
- * changes results by rounding only.  Models of one batched launch need the
- * same choice. */
+ * species mix, tables over integer T: the Planck means sum_s w_s Kp_s(T)
+ * (computeE, Radiation.f:611-627; 80 kB per mix), the diffusion
+ * coefficients (DiffCoeffCell, Dtable.f:97-182; 160 kB) and with several
+ * species the reemission cumulative sum_s w_s CUM_s (16 MB per mix for 200
+ * wavelengths).  Each table is built while it fits in 4 GiB, or in the bytes
+ * the environment variable MCX_CLASS_TABLE_BYTES gives (read at each call;
+ * many contexts on one GPU set it low).  Without a table the Planck means
+ * and diffusion coefficients are evaluated at each read (the same values
+ * bit for bit) and the reemission sums the species at every probe (the same
+ * up to rounding).  Models of one batched launch need the same reemission
+ * choice. */
 int mcx_set_model(mcx_ctx *ctx, const mcx_grid *grid);
 
 /* One transport pass: MCRadiation.f:269-668 on the device.  Tallies stay in
